@@ -1,0 +1,317 @@
+/*
+ * tealeaf_hip.h -- C ABI of libtealeaf_hip.so, the MI355X (gfx950) kernel
+ * suite for TeaLeaf's implicit heat-conduction solve.
+ *
+ * Each tlhip2d_* / tlhip3d_* kernel entry point replaces one run_* entry
+ * point of the reference kernel interface, cited below as
+ * 2d/kernel_interface.h:LINE or 3d/kernel_interface.h:LINE.  Arguments are
+ * the ones the reference's omp3 suite passes for that entry point
+ * (2d/c_kernels/omp3/kernel_interface.c): grid sizes x, y (, z) INCLUDING the
+ * halo, halo_depth, scalars, and field pointers.
+ *
+ *  - Field pointers are DEVICE pointers to x*y (x*y*z) doubles, row-major,
+ *    index = kk + jj*x (+ ii*x*y), exactly the reference layout.
+ *  - Scalar results are HOST pointers and are ACCUMULATED (+=), as the
+ *    reference does; the Jacobi error is ASSIGNED (2d/c_kernels/omp3/jacobi.c:99).
+ *    A scalar is final when the call returns.
+ *  - The Chebyshev and PPCG sweeps fuse the reference's two passes into one
+ *    and therefore read the stencil field from one buffer and write the
+ *    updated field to another (u_in/u_out, sd_in/sd_out).  The output buffer
+ *    ends up equal, halo included, to what the reference computes in place;
+ *    callers swap the two pointers afterwards.
+ *  - Every function returns 0 on success or a TLHIP_ERR_* code, with a
+ *    message from tlhip_last_error().  (The reference calls die() instead;
+ *    the hip_cdna4 suite glue does exactly that on a non-zero return.)
+ *  - Work is queued on one in-order HIP stream.  Calls are sequential, as the
+ *    reference's are; the library is not re-entrant.
+ *
+ * The struct layouts below are shared with the CPU oracle (oracle/oracle.h).
+ */
+#ifndef TEALEAF_HIP_H
+#define TEALEAF_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLHIP_OK 0
+#define TLHIP_ERR_ARG 1     /* invalid argument or shape */
+#define TLHIP_ERR_HIP 2     /* HIP runtime / launch failure */
+#define TLHIP_ERR_SOLVER 3  /* solver failure (negative eigenvalues, QL) */
+
+/* One material state, as 2d/settings.h:133-144 / 3d/settings.h:131-145.
+ * geometry: 0 rectangle/cuboid, 1 circular/spherical, 2 point.  Extents
+ * already include the reference parser's d/100 shrink (2d/parse_config.c:222-260). */
+typedef struct {
+    double density, energy;
+    double x_min, y_min, z_min, x_max, y_max, z_max;
+    double radius;
+    int geometry;
+    int defined;
+} tlhip_state;
+
+/* ------------------------------------------------------------ runtime */
+/* Select the device (rank -> device, as the reference's CUDA suite does,
+ * 2d/c_kernels/cuda/kernel_initialise.cu:57-66) and create the stream.
+ * Called implicitly with device 0 by any other entry point. */
+int tlhip_init(int device);
+int tlhip_finalise(void);
+int tlhip_device(void);
+const char* tlhip_last_error(void);
+int tlhip_synchronize(void);
+/* Run the suite's work on a caller-owned stream (e.g. a framework's). */
+int tlhip_set_stream(void* stream);
+void* tlhip_get_stream(void);
+/* Tuning knobs: rows (2D) / planes (3D) per stencil tile; <= 0 restores
+ * the automatic choice. */
+int tlhip_set_tile_rows(int rows);
+int tlhip_set_tile_planes(int planes);
+
+/* Device memory, zero-filled (replaces the allocate_buffer calls of
+ * run_kernel_initialise, 2d/kernel_interface.h:17 / 3d/kernel_interface.h:13,
+ * and run_kernel_finalise, 2d/kernel_interface.h:19). */
+int tlhip_alloc(size_t bytes, void** out);
+int tlhip_free(void* ptr);
+int tlhip_copy_to_device(void* dst, const void* src, size_t bytes);
+int tlhip_copy_to_host(void* dst, const void* src, size_t bytes);
+int tlhip_copy_device(void* dst, const void* src, size_t bytes);
+/* Lane-shift self test: out[l] = in[l-1] (lane 0: -1), out[64+l] = in[l+1]
+ * (lane 63: -1); 64 doubles in, 128 out, device pointers. */
+int tlhip_selftest_lanes(const double* in_dev, double* out_dev);
+
+/* ------------------------------------------------------------ 2D */
+/* 2d/kernel_interface.h:13 run_set_chunk_data */
+int tlhip2d_set_chunk_data(int x, int y, int halo_depth, int left, int bottom,
+        double grid_x_min, double grid_y_min, double dx, double dy,
+        double* cell_x, double* cell_y, double* vertex_x, double* vertex_y,
+        double* volume, double* x_area, double* y_area);
+/* 2d/kernel_interface.h:15 run_set_chunk_state (states: host array) */
+int tlhip2d_set_chunk_state(int x, int y, const double* vertex_x,
+        const double* vertex_y, const double* cell_x, const double* cell_y,
+        double* density, double* energy0, double* u, int num_states,
+        const tlhip_state* states);
+/* 2d/kernel_interface.h:23 run_local_halos.  neighbours[4] and fields[6]
+ * are host arrays (faces 2d/shared.h:25-29, field ids :31-36); every
+ * flagged field and external face is done in one launch. */
+int tlhip2d_local_halos(int x, int y, int depth, int halo_depth,
+        const int* neighbours, const bool* fields, double* density,
+        double* energy0, double* energy, double* u, double* p, double* sd);
+/* 2d/kernel_interface.h:25 run_pack_or_unpack (buffer: host memory) */
+int tlhip2d_pack_or_unpack(int x, int y, int depth, int halo_depth, int face,
+        bool pack, double* field, double* buffer);
+/* 2d/kernel_interface.h:28 run_store_energy */
+int tlhip2d_store_energy(int x, int y, const double* energy0, double* energy);
+/* 2d/kernel_interface.h:30 run_field_summary */
+int tlhip2d_field_summary(int x, int y, int halo_depth, const double* volume,
+        const double* density, const double* energy0, const double* u,
+        double* vol, double* mass, double* ie, double* temp);
+/* 2d/kernel_interface.h:35 run_cg_init */
+int tlhip2d_cg_init(int x, int y, int halo_depth, int coefficient, double rx,
+        double ry, double* rro, const double* density, const double* energy,
+        double* u, double* p, double* r, double* w, double* kx, double* ky);
+/* 2d/kernel_interface.h:38 run_cg_calc_w */
+int tlhip2d_cg_calc_w(int x, int y, int halo_depth, double* pw,
+        const double* p, double* w, const double* kx, const double* ky);
+/* 2d/kernel_interface.h:40 run_cg_calc_ur */
+int tlhip2d_cg_calc_ur(int x, int y, int halo_depth, double alpha,
+        double* rrn, double* u, const double* p, double* r, const double* w);
+/* 2d/kernel_interface.h:42 run_cg_calc_p */
+int tlhip2d_cg_calc_p(int x, int y, int halo_depth, double beta, double* p,
+        const double* r);
+/* 2d/kernel_interface.h:46 run_cheby_init (in place, as the reference) */
+int tlhip2d_cheby_init(int x, int y, int halo_depth, double theta, double* u,
+        const double* u0, double* p, double* r, double* w, const double* kx,
+        const double* ky);
+/* 2d/kernel_interface.h:48 run_cheby_iterate, fused; u_out != u_in.
+ * w may be NULL: the reference never reads w before rewriting it. */
+int tlhip2d_cheby_iterate(int x, int y, int halo_depth, double alpha,
+        double beta, const double* u_in, double* u_out, const double* u0,
+        double* p, double* r, double* w, const double* kx, const double* ky);
+/* 2d/kernel_interface.h:52 run_jacobi_init */
+int tlhip2d_jacobi_init(int x, int y, int halo_depth, int coefficient,
+        double rx, double ry, const double* density, const double* energy,
+        double* u0, double* u, double* kx, double* ky);
+/* 2d/kernel_interface.h:54 run_jacobi_iterate (error is assigned) */
+int tlhip2d_jacobi_iterate(int x, int y, int halo_depth, double* error,
+        const double* kx, const double* ky, const double* u0, double* u,
+        double* r);
+/* 2d/kernel_interface.h:58 run_ppcg_init */
+int tlhip2d_ppcg_init(int x, int y, int halo_depth, double theta,
+        const double* r, double* sd);
+/* 2d/kernel_interface.h:60 run_ppcg_inner_iteration, fused; sd_out != sd_in */
+int tlhip2d_ppcg_inner_iteration(int x, int y, int halo_depth, double alpha,
+        double beta, double* u, double* r, const double* kx, const double* ky,
+        const double* sd_in, double* sd_out);
+/* 2d/kernel_interface.h:64 run_copy_u */
+int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);
+/* 2d/kernel_interface.h:66 run_calculate_residual */
+int tlhip2d_calculate_residual(int x, int y, int halo_depth, const double* u,
+        const double* u0, double* r, const double* kx, const double* ky);
+/* 2d/kernel_interface.h:68 run_calculate_2norm */
+int tlhip2d_calculate_2norm(int x, int y, int halo_depth,
+        const double* buffer, double* norm);
+/* 2d/kernel_interface.h:70 run_finalise */
+int tlhip2d_finalise(int x, int y, int halo_depth, double* energy,
+        const double* density, const double* u);
+
+/* ------------------------------------------------------------ 3D */
+/* 3d/kernel_interface.h:9 run_set_chunk_data */
+int tlhip3d_set_chunk_data(int x, int y, int z, int halo_depth, int left,
+        int bottom, int back, double grid_x_min, double grid_y_min,
+        double grid_z_min, double dx, double dy, double dz, double* cell_x,
+        double* cell_y, double* cell_z, double* vertex_x, double* vertex_y,
+        double* vertex_z, double* volume, double* x_area, double* y_area,
+        double* z_area);
+/* 3d/kernel_interface.h:11 run_set_chunk_state */
+int tlhip3d_set_chunk_state(int x, int y, int z, const double* vertex_x,
+        const double* vertex_y, const double* vertex_z, const double* cell_x,
+        const double* cell_y, const double* cell_z, double* density,
+        double* energy0, double* u, int num_states, const tlhip_state* states);
+/* 3d/kernel_interface.h:17 run_local_halos (neighbours[6], faces 3d/shared.h:24-30) */
+int tlhip3d_local_halos(int x, int y, int z, int depth, int halo_depth,
+        const int* neighbours, const bool* fields, double* density,
+        double* energy0, double* energy, double* u, double* p, double* sd);
+/* 3d/kernel_interface.h:19 run_pack_or_unpack (buffer: host memory) */
+int tlhip3d_pack_or_unpack(int x, int y, int z, int depth, int halo_depth,
+        int face, bool pack, double* field, double* buffer);
+/* 3d/kernel_interface.h:22 run_store_energy */
+int tlhip3d_store_energy(int x, int y, int z, const double* energy0,
+        double* energy);
+/* 3d/kernel_interface.h:24 run_field_summary */
+int tlhip3d_field_summary(int x, int y, int z, int halo_depth,
+        const double* volume, const double* density, const double* energy0,
+        const double* u, double* vol, double* mass, double* ie, double* temp);
+/* 3d/kernel_interface.h:29 run_cg_solver_init */
+int tlhip3d_cg_solver_init(int x, int y, int z, int halo_depth,
+        int coefficient, double rx, double ry, double rz, double* rro,
+        const double* density, const double* energy, double* u, double* p,
+        double* r, double* w, double* kx, double* ky, double* kz);
+/* 3d/kernel_interface.h:32 run_cg_solver_calc_w */
+int tlhip3d_cg_solver_calc_w(int x, int y, int z, int halo_depth, double* pw,
+        const double* p, double* w, const double* kx, const double* ky,
+        const double* kz);
+/* 3d/kernel_interface.h:34 run_cg_solver_calc_ur */
+int tlhip3d_cg_solver_calc_ur(int x, int y, int z, int halo_depth,
+        double alpha, double* rrn, double* u, const double* p, double* r,
+        const double* w);
+/* 3d/kernel_interface.h:36 run_cg_solver_calc_p */
+int tlhip3d_cg_solver_calc_p(int x, int y, int z, int halo_depth, double beta,
+        double* p, const double* r);
+/* 3d/kernel_interface.h:40 run_cheby_solver_init (in place) */
+int tlhip3d_cheby_solver_init(int x, int y, int z, int halo_depth,
+        double theta, double* u, const double* u0, double* p, double* r,
+        double* w, const double* kx, const double* ky, const double* kz);
+/* 3d/kernel_interface.h:42 run_cheby_solver_iterate, fused; u_out != u_in */
+int tlhip3d_cheby_solver_iterate(int x, int y, int z, int halo_depth,
+        double alpha, double beta, const double* u_in, double* u_out,
+        const double* u0, double* p, double* r, double* w, const double* kx,
+        const double* ky, const double* kz);
+/* 3d/kernel_interface.h:46 run_jacobi_solver_init */
+int tlhip3d_jacobi_solver_init(int x, int y, int z, int halo_depth,
+        int coefficient, double rx, double ry, double rz,
+        const double* density, const double* energy, double* u0, double* u,
+        double* kx, double* ky, double* kz);
+/* 3d/kernel_interface.h:48 run_jacobi_solver_iterate (error is assigned) */
+int tlhip3d_jacobi_solver_iterate(int x, int y, int z, int halo_depth,
+        double* error, const double* kx, const double* ky, const double* kz,
+        const double* u0, double* u, double* r);
+/* 3d/kernel_interface.h:52 run_ppcg_init */
+int tlhip3d_ppcg_init(int x, int y, int z, int halo_depth, double theta,
+        const double* r, double* sd);
+/* 3d/kernel_interface.h:54 run_ppcg_inner_iteration, fused; sd_out != sd_in */
+int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int halo_depth,
+        double alpha, double beta, double* u, double* r, const double* kx,
+        const double* ky, const double* kz, const double* sd_in,
+        double* sd_out);
+/* 3d/kernel_interface.h:58 run_solver_copy_u */
+int tlhip3d_solver_copy_u(int x, int y, int z, int halo_depth, double* u0,
+        const double* u);
+/* 3d/kernel_interface.h:60 run_calculate_residual */
+int tlhip3d_calculate_residual(int x, int y, int z, int halo_depth,
+        const double* u, const double* u0, double* r, const double* kx,
+        const double* ky, const double* kz);
+/* 3d/kernel_interface.h:62 run_calculate_2norm */
+int tlhip3d_calculate_2norm(int x, int y, int z, int halo_depth,
+        const double* buffer, double* norm);
+/* 3d/kernel_interface.h:64 run_solver_finalise */
+int tlhip3d_solver_finalise(int x, int y, int z, int halo_depth,
+        double* energy, const double* density, const double* u);
+
+/* ------------------------------------------------------------ timestep driver
+ * A native host driver over the entry points above, for callers without the
+ * reference host (benchmarks, tests, Python).  It follows the single-chunk,
+ * NO_MPI control flow of 2d/diffuse.c:22-80 and 2d/drivers/ (3D:
+ * 3d/diffuse.c, 3d/drivers/), so that iteration counts and results match
+ * the reference host driving the same kernels.  No counterpart in the
+ * reference kernel interface. */
+
+/* Same layout as oracle/oracle.h tl_settings. solver: 0 Jacobi, 1 CG,
+ * 2 Chebyshev, 3 PPCG (2d/settings.h:48-54). */
+typedef struct {
+    int dims;
+    int x_cells, y_cells, z_cells;
+    double xmin, ymin, zmin, xmax, ymax, zmax;
+    double dt_init;
+    int end_step;
+    int max_iters;
+    double eps;
+    int solver;
+    int coefficient;
+    int presteps;
+    int ppcg_inner_steps;
+    double eps_lim;
+    int error_switch;
+    int check_result;
+    int summary_frequency;
+    int halo_depth;
+} tlhip_settings;
+
+/* Per-timestep report: the counts the reference prints (2d/diffuse.c:70-78,
+ * 2d/drivers/cg_driver.c:27) plus how many sweeps actually ran. */
+typedef struct {
+    int printed_cg_iters;
+    int printed_inner_iters;
+    int est_iters;
+    int n_cg_steps;
+    int n_cheby_steps;
+    int n_ppcg_outer;
+    int n_ppcg_inner;
+    int n_jacobi;
+    double error;
+    double exact_error;
+    double eigmin, eigmax;
+    int summary_done;
+    double vol, mass, ie, temp;
+} tlhip_step_report;
+
+typedef struct tlhip_problem tlhip_problem;
+
+/* Allocate a chunk on the device and initialise it as
+ * 2d/initialise.c:11-31 does (set_chunk_data, set_chunk_state, halo prime,
+ * store_energy). */
+int tlhip_problem_create(const tlhip_settings* settings, const tlhip_state* states,
+        int num_states, tlhip_problem** out);
+/* Replace density and energy0 (host arrays of the full chunk incl. halo) and
+ * redo u = e*rho, the halo prime and store_energy. */
+int tlhip_problem_inject(tlhip_problem* p, const double* density,
+        const double* energy0);
+/* One timestep (tt 0-based), as 2d/diffuse.c:22-80. */
+int tlhip_problem_step(tlhip_problem* p, int tt, tlhip_step_report* report);
+/* vol, mass, ie, temp of the current state (2d/drivers/field_summary_driver.c). */
+int tlhip_problem_field_summary(tlhip_problem* p, double* out4);
+/* Number of cells of one field (incl. halo). */
+int tlhip_problem_field_len(tlhip_problem* p, long* n);
+/* Copy a field to host memory.  id: 0 density, 1 energy0, 2 energy, 3 u,
+ * 4 u0, 5 p, 6 r, 7 w, 8 kx, 9 ky, 10 kz, 11 sd. */
+int tlhip_problem_get_field(tlhip_problem* p, int id, double* host_out);
+/* Device pointer of a field (same ids), valid until the next step. */
+int tlhip_problem_field_ptr(tlhip_problem* p, int id, void** dev_out);
+int tlhip_problem_destroy(tlhip_problem* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,642 @@
+// Native timestep driver over the hip_cdna4 entry points.
+//
+// Follows the single-chunk, NO_MPI control flow of the reference host:
+// 2d/diffuse.c:22-80 (solve), 2d/drivers/cg_driver.c, cheby_driver.c,
+// ppcg_driver.c, jacobi_driver.c, eigenvalue_driver.c,
+// solve_finished_driver.c, halo_update_driver.c (3D: 3d/diffuse.c and
+// 3d/drivers/, which differ only by testing sqrt(error) for convergence).
+// The driver state that changes results is kept as the reference keeps it:
+// the fields-to-exchange flags persist between driver calls, the CG and
+// Chebyshev coefficient arrays persist across timesteps, and alpha/beta and
+// the eigenvalue estimate are computed on the host in FP64.
+//
+// Every field lives on the device.  The only device->host traffic is the
+// scalars the algorithm needs on the host (two per CG iteration).
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "tealeaf_hip.h"
+
+int tl_fail(int code, const char* fmt, ...);
+
+#define TRY(expr)                      \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc) return _rc;           \
+    } while (0)
+
+namespace {
+
+enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
+
+double* dalloc(long n, int* rc)
+{
+    void* p = nullptr;
+    if (*rc == 0) *rc = tlhip_alloc(sizeof(double) * (size_t)(n > 0 ? n : 1), &p);
+    return (double*)p;
+}
+
+// ------------------------------------------------------------ eigenvalues
+// Symmetric tridiagonal eigenvalues by implicit QL with Wilkinson shifts
+// (Numerical Recipes in C, 2nd ed. sec. 11.3), as 2d/drivers/eigenvalue_driver.c:71-121
+// uses it, including carrying on through an exact zero rotation radius.
+// d: diagonal -> eigenvalues; e: sub-diagonal in e[1..n-1].
+int ql_eigenvalues(std::vector<double>& d, std::vector<double>& e)
+{
+    const int n = (int)d.size();
+    for (int i = 0; i + 1 < n; ++i) e[i] = e[i + 1];
+    if (n) e[n - 1] = 0.0;
+    auto sgn = [](double a, double b) { return b < 0 ? -std::fabs(a) : std::fabs(a); };
+    for (int l = 0; l < n; ++l) {
+        int sweeps = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) + dd == dd) break;
+            }
+            if (m == l) break;
+            if (sweeps++ == 30) return tl_fail(TLHIP_ERR_SOLVER, "Too many iterations in TQLI routine");
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double rad = std::sqrt(g * g + 1.0);
+            g = d[m] - d[l] + e[l] / (g + sgn(rad, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * e[i], b = c * e[i];
+                rad = std::sqrt(f * f + g * g);
+                e[i + 1] = rad;
+                if (rad == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    continue;
+                }
+                s = f / rad;
+                c = g / rad;
+                g = d[i + 1] - p;
+                rad = (d[i] - g) * s + 2.0 * c * b;
+                p = s * rad;
+                d[i + 1] = g + p;
+                g = c * rad - b;
+            }
+            d[l] = d[l] - p;
+            e[l] = g;
+            e[m] = 0.0;
+        } while (m != l);
+    }
+    return 0;
+}
+
+}  // namespace
+
+struct tlhip_problem {
+    tlhip_settings s{};
+    int dims = 2, x = 0, y = 0, z = 1, hd = 2;
+    long n = 0;
+    double dx = 0, dy = 0, dz = 0;
+    double *density0 = nullptr, *density = nullptr, *energy0 = nullptr, *energy = nullptr;
+    double *u = nullptr, *u0 = nullptr, *p = nullptr, *r = nullptr, *mi = nullptr, *w = nullptr;
+    double *kx = nullptr, *ky = nullptr, *kz = nullptr, *sd = nullptr;
+    double *u_alt = nullptr, *sd_alt = nullptr;
+    double *volume = nullptr, *x_area = nullptr, *y_area = nullptr, *z_area = nullptr;
+    double *cell_x = nullptr, *cell_y = nullptr, *cell_z = nullptr;
+    double *vertex_x = nullptr, *vertex_y = nullptr, *vertex_z = nullptr;
+    std::vector<double> cg_alphas, cg_betas, cheby_alphas, cheby_betas;
+    double theta = 0, eigmin = 0, eigmax = 0;
+    int neighbours[6] = {-1, -1, -1, -1, -1, -1};
+    bool fields[6] = {};
+    tlhip_step_report* rep = nullptr;
+
+    // -------------------------------------------------------- kernels
+    int local_halos(int depth)
+    {
+        if (dims == 2)
+            return tlhip2d_local_halos(x, y, depth, hd, neighbours, fields, density, energy0,
+                                       energy, u, p, sd);
+        return tlhip3d_local_halos(x, y, z, depth, hd, neighbours, fields, density, energy0,
+                                   energy, u, p, sd);
+    }
+    // halo_update_driver.c:6-25
+    int halo_update(int depth)
+    {
+        for (bool f : fields)
+            if (f) return local_halos(depth);
+        return 0;
+    }
+    void reset_fields() { std::memset(fields, 0, sizeof(fields)); }
+    int calc_w(double* pw)
+    {
+        return dims == 2 ? tlhip2d_cg_calc_w(x, y, hd, pw, p, w, kx, ky)
+                         : tlhip3d_cg_solver_calc_w(x, y, z, hd, pw, p, w, kx, ky, kz);
+    }
+    int calc_ur(double alpha, double* rrn)
+    {
+        return dims == 2 ? tlhip2d_cg_calc_ur(x, y, hd, alpha, rrn, u, p, r, w)
+                         : tlhip3d_cg_solver_calc_ur(x, y, z, hd, alpha, rrn, u, p, r, w);
+    }
+    int calc_p(double beta)
+    {
+        return dims == 2 ? tlhip2d_cg_calc_p(x, y, hd, beta, p, r)
+                         : tlhip3d_cg_solver_calc_p(x, y, z, hd, beta, p, r);
+    }
+    int copy_u()
+    {
+        return dims == 2 ? tlhip2d_copy_u(x, y, hd, u0, u) : tlhip3d_solver_copy_u(x, y, z, hd, u0, u);
+    }
+    int residual()
+    {
+        return dims == 2 ? tlhip2d_calculate_residual(x, y, hd, u, u0, r, kx, ky)
+                         : tlhip3d_calculate_residual(x, y, z, hd, u, u0, r, kx, ky, kz);
+    }
+    int norm2(const double* b, double* out)
+    {
+        return dims == 2 ? tlhip2d_calculate_2norm(x, y, hd, b, out)
+                         : tlhip3d_calculate_2norm(x, y, z, hd, b, out);
+    }
+    int store_energy()
+    {
+        return dims == 2 ? tlhip2d_store_energy(x, y, energy0, energy)
+                         : tlhip3d_store_energy(x, y, z, energy0, energy);
+    }
+    int field_summary(double* o)
+    {
+        o[0] = o[1] = o[2] = o[3] = 0.0;
+        return dims == 2
+                   ? tlhip2d_field_summary(x, y, hd, volume, density, energy0, u, &o[0], &o[1], &o[2], &o[3])
+                   : tlhip3d_field_summary(x, y, z, hd, volume, density, energy0, u, &o[0], &o[1], &o[2], &o[3]);
+    }
+
+    // -------------------------------------------------------- CG (cg_driver.c)
+    int cg_init_driver(double rx, double ry, double rz, double* rro)
+    {
+        *rro = 0.0;
+        if (dims == 2)
+            TRY(tlhip2d_cg_init(x, y, hd, s.coefficient, rx, ry, rro, density, energy, u, p, r, w, kx, ky));
+        else
+            TRY(tlhip3d_cg_solver_init(x, y, z, hd, s.coefficient, rx, ry, rz, rro, density,
+                                       energy, u, p, r, w, kx, ky, kz));
+        reset_fields();
+        fields[F_U] = fields[F_P] = true;
+        TRY(halo_update(1));
+        return copy_u();
+    }
+
+    int cg_main_step(int tt, double* rro, double* error)
+    {
+        double pw = 0.0;
+        TRY(calc_w(&pw));
+        const double alpha = *rro / pw;
+        double rrn = 0.0;
+        cg_alphas[tt] = alpha;
+        TRY(calc_ur(alpha, &rrn));
+        const double beta = rrn / *rro;
+        cg_betas[tt] = beta;
+        TRY(calc_p(beta));
+        *error = rrn;
+        *rro = rrn;
+        rep->n_cg_steps++;
+        return 0;
+    }
+
+    // 3D tests sqrt(error) (3d/drivers/cg_solver_driver.c:24-25), 2D error.
+    bool converged(double* error)
+    {
+        if (dims == 3) *error = std::sqrt(*error);
+        return std::fabs(*error) < s.eps;
+    }
+
+    int cg_driver(double rx, double ry, double rz, double* error)
+    {
+        double rro;
+        TRY(cg_init_driver(rx, ry, rz, &rro));
+        int tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            TRY(cg_main_step(tt, &rro, error));
+            TRY(halo_update(1));
+            if (converged(error)) break;
+        }
+        rep->printed_cg_iters = tt;
+        return 0;
+    }
+
+    bool switch_now(int started, int tt, double error) const
+    {
+        if (started) return true;
+        if (s.error_switch) return error < s.eps_lim && tt > 20;   // CG_ITERS_FOR_EIGENVALUES
+        return tt > s.presteps && error < 1.0;                     // ERROR_SWITCH_MAX
+    }
+
+    // eigenvalue_driver.c:11-67: Lanczos tridiagonal of the CG run so far.
+    int eigenvalues(int ncg)
+    {
+        std::vector<double> d(ncg, 0.0), e(ncg, 0.0);
+        for (int i = 0; i < ncg; ++i) {
+            d[i] = 1.0 / cg_alphas[i];
+            if (i > 0) d[i] += cg_betas[i - 1] / cg_alphas[i - 1];
+            if (i < ncg - 1) e[i + 1] = std::sqrt(cg_betas[i]) / cg_alphas[i];
+        }
+        TRY(ql_eigenvalues(d, e));
+        double lo = DBL_MAX, hi = DBL_MIN;
+        for (double v : d) {
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        if (lo < 0.0 || hi < 0.0) return tl_fail(TLHIP_ERR_SOLVER, "Calculated negative eigenvalues.");
+        eigmin = lo * 0.95;
+        eigmax = hi * 1.05;
+        rep->eigmin = eigmin;
+        rep->eigmax = eigmax;
+        return 0;
+    }
+
+    // cheby_driver.c:163-183
+    void cheby_coefs(int count)
+    {
+        theta = (eigmax + eigmin) / 2.0;
+        const double delta = (eigmax - eigmin) / 2.0;
+        const double sigma = theta / delta;
+        double rho_old = 1.0 / sigma;
+        for (int i = 0; i < count && i < (int)cheby_alphas.size(); ++i) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+            cheby_alphas[i] = rho_new * rho_old;
+            cheby_betas[i] = 2.0 * rho_new / delta;
+            rho_old = rho_new;
+        }
+    }
+
+    // cheby_driver.c:146-160, single-precision logs and rounding as there.
+    int est_iterations(double error, double bb) const
+    {
+        const double cn = eigmax / eigmin;
+        const double it_alpha = DBL_EPSILON * bb / (4.0 * error);
+        const double gamm = (std::sqrt(cn) - 1.0) / (std::sqrt(cn) + 1.0);
+        const float a = logf((float)it_alpha), b = logf((float)gamm);
+        return (int)roundf((float)((double)a / (2.0 * (double)b)));
+    }
+
+    // -------------------------------------------------------- Chebyshev
+    int cheby_step(int k, bool calc_norm, double* error)
+    {
+        double* uo = u_alt;
+        if (dims == 2)
+            TRY(tlhip2d_cheby_iterate(x, y, hd, cheby_alphas[k], cheby_betas[k], u, uo, u0, p, r,
+                                      nullptr, kx, ky));
+        else
+            TRY(tlhip3d_cheby_solver_iterate(x, y, z, hd, cheby_alphas[k], cheby_betas[k], u, uo,
+                                             u0, p, r, nullptr, kx, ky, kz));
+        u_alt = u;
+        u = uo;
+        rep->n_cheby_steps++;
+        if (calc_norm) {
+            *error = 0.0;
+            TRY(norm2(r, error));
+        }
+        return 0;
+    }
+
+    int cheby_driver(double rx, double ry, double rz, double* error)
+    {
+        double rro;
+        TRY(cg_init_driver(rx, ry, rz, &rro));
+        int est = 0, ncheby = 0, tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            if (!switch_now(ncheby, tt, *error)) {
+                TRY(cg_main_step(tt, &rro, error));
+            } else if (++ncheby == 1) {
+                double bb = 0.0;
+                TRY(eigenvalues(tt));
+                cheby_coefs(s.max_iters - tt);
+                TRY(norm2(u0, &bb));
+                if (dims == 2)
+                    TRY(tlhip2d_cheby_init(x, y, hd, theta, u, u0, p, r, w, kx, ky));
+                else
+                    TRY(tlhip3d_cheby_solver_init(x, y, z, hd, theta, u, u0, p, r, w, kx, ky, kz));
+                reset_fields();
+                fields[F_U] = true;
+                TRY(halo_update(1));
+                TRY(cheby_step(ncheby, true, error));
+                est = est_iterations(*error, bb);
+            } else {
+                TRY(cheby_step(ncheby, ncheby >= est && (tt + 1) % 10 == 0, error));
+            }
+            TRY(halo_update(1));
+            if (converged(error)) break;
+        }
+        rep->printed_cg_iters = tt - ncheby + 1;
+        rep->printed_inner_iters = ncheby;
+        rep->est_iters = est;
+        return 0;
+    }
+
+    // -------------------------------------------------------- PPCG
+    int ppcg_main_step(double* rro, double* error)
+    {
+        double pw = 0.0;
+        TRY(calc_w(&pw));
+        const double alpha = *rro / pw;
+        double rrn = 0.0;
+        TRY(calc_ur(alpha, &rrn));
+
+        if (dims == 2) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
+        else TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
+        reset_fields();
+        fields[F_SD] = true;
+        for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
+            TRY(halo_update(1));
+            double* so = sd_alt;
+            if (dims == 2)
+                TRY(tlhip2d_ppcg_inner_iteration(x, y, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
+                                                 kx, ky, sd, so));
+            else
+                TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp], cheby_betas[pp], u,
+                                                 r, kx, ky, kz, sd, so));
+            sd_alt = sd;
+            sd = so;
+            rep->n_ppcg_inner++;
+        }
+        reset_fields();
+        fields[F_P] = true;
+
+        rrn = 0.0;
+        TRY(norm2(r, &rrn));
+        const double beta = rrn / *rro;
+        TRY(calc_p(beta));
+        *error = rrn;
+        *rro = rrn;
+        rep->n_ppcg_outer++;
+        return 0;
+    }
+
+    int ppcg_driver(double rx, double ry, double rz, double* error)
+    {
+        double rro;
+        TRY(cg_init_driver(rx, ry, rz, &rro));
+        int nppcg = 0, tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            if (!switch_now(nppcg, tt, *error)) {
+                TRY(cg_main_step(tt, &rro, error));
+            } else {
+                if (++nppcg == 1) {
+                    TRY(eigenvalues(tt));
+                    cheby_coefs(s.ppcg_inner_steps);
+                    TRY(residual());
+                    reset_fields();
+                    fields[F_P] = true;
+                    TRY(halo_update(1));
+                }
+                TRY(ppcg_main_step(&rro, error));
+            }
+            TRY(halo_update(1));
+            if (converged(error)) break;
+        }
+        rep->printed_cg_iters = tt - nppcg + 1;
+        rep->printed_inner_iters = nppcg;
+        return 0;
+    }
+
+    // -------------------------------------------------------- Jacobi
+    int jacobi_driver(double rx, double ry, double rz, double* error)
+    {
+        if (dims == 2)
+            TRY(tlhip2d_jacobi_init(x, y, hd, s.coefficient, rx, ry, density, energy, u0, u, kx, ky));
+        else
+            TRY(tlhip3d_jacobi_solver_init(x, y, z, hd, s.coefficient, rx, ry, rz, density, energy,
+                                           u0, u, kx, ky, kz));
+        TRY(copy_u());
+        reset_fields();
+        fields[F_U] = true;
+        int tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            if (dims == 2) TRY(tlhip2d_jacobi_iterate(x, y, hd, error, kx, ky, u0, u, r));
+            else TRY(tlhip3d_jacobi_solver_iterate(x, y, z, hd, error, kx, ky, kz, u0, u, r));
+            rep->n_jacobi++;
+            if (tt % 50 == 0) {
+                TRY(halo_update(1));
+                TRY(residual());
+                TRY(norm2(r, error));
+            }
+            TRY(halo_update(1));
+            if (converged(error)) break;
+        }
+        rep->printed_cg_iters = tt;
+        return 0;
+    }
+
+    // -------------------------------------------------------- lifecycle
+    // initialise.c:24-30: prime halos of density/energy0/energy, store energy.
+    int prime()
+    {
+        reset_fields();
+        fields[F_DENSITY] = fields[F_ENERGY0] = fields[F_ENERGY1] = true;
+        TRY(halo_update(2));
+        return store_energy();
+    }
+
+    int step(int tt, tlhip_step_report* report)
+    {
+        std::memset(report, 0, sizeof(*report));
+        rep = report;
+        const double dt = s.dt_init;
+        const double rx = dt / (dx * dx), ry = dt / (dy * dy);
+        const double rz = dims == 3 ? dt / (dz * dz) : 0.0;
+        reset_fields();
+        fields[F_ENERGY1] = fields[F_DENSITY] = true;
+        TRY(halo_update(2));
+        double error = 1e+10;
+        switch (s.solver) {
+        case 0: TRY(jacobi_driver(rx, ry, rz, &error)); break;
+        case 1: TRY(cg_driver(rx, ry, rz, &error)); break;
+        case 2: TRY(cheby_driver(rx, ry, rz, &error)); break;
+        case 3: TRY(ppcg_driver(rx, ry, rz, &error)); break;
+        default: return tl_fail(TLHIP_ERR_ARG, "unknown solver %d", s.solver);
+        }
+        report->error = error;
+        // solve_finished_driver.c:7-43
+        double exact = 0.0;
+        if (s.check_result) {
+            TRY(residual());
+            TRY(norm2(r, &exact));
+        }
+        report->exact_error = exact;
+        if (dims == 2) TRY(tlhip2d_finalise(x, y, hd, energy, density, u));
+        else TRY(tlhip3d_solver_finalise(x, y, z, hd, energy, density, u));
+        fields[F_ENERGY1] = true;
+        TRY(halo_update(1));
+        if (s.summary_frequency > 0 && tt % s.summary_frequency == 0) {
+            double o[4];
+            TRY(field_summary(o));
+            report->summary_done = 1;
+            report->vol = o[0];
+            report->mass = o[1];
+            report->ie = o[2];
+            report->temp = o[3];
+        }
+        return 0;
+    }
+
+    double* field(int id)
+    {
+        double* f[] = {density, energy0, energy, u, u0, p, r, w, kx, ky, kz, sd};
+        return (id >= 0 && id < 12) ? f[id] : nullptr;
+    }
+
+    void release()
+    {
+        double* all[] = {density0, density, energy0, energy, u, u0, p, r, mi, w, kx, ky, kz,
+                         sd, u_alt, sd_alt, volume, x_area, y_area, z_area, cell_x, cell_y,
+                         cell_z, vertex_x, vertex_y, vertex_z};
+        for (double* a : all)
+            if (a) (void)tlhip_free(a);
+    }
+};
+
+extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* states,
+                                    int num_states, tlhip_problem** out)
+{
+    if (!s || !states || !out || num_states < 1)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: invalid arguments");
+    if (s->dims != 2 && s->dims != 3)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: dims must be 2 or 3");
+    if (s->x_cells < 1 || s->y_cells < 1 || (s->dims == 3 && s->z_cells < 1) || s->halo_depth < 1 ||
+        s->max_iters < 1)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: invalid sizes");
+    *out = nullptr;
+    TRY(tlhip_init(-1));
+    auto* P = new tlhip_problem();
+    P->s = *s;
+    P->dims = s->dims;
+    P->hd = s->halo_depth;
+    P->x = s->x_cells + 2 * P->hd;
+    P->y = s->y_cells + 2 * P->hd;
+    P->z = s->dims == 3 ? s->z_cells + 2 * P->hd : 1;
+    P->n = (long)P->x * P->y * P->z;
+    P->dx = (s->xmax - s->xmin) / (double)s->x_cells;
+    P->dy = (s->ymax - s->ymin) / (double)s->y_cells;
+    P->dz = s->dims == 3 ? (s->zmax - s->zmin) / (double)s->z_cells : 0.0;
+    int rc = 0;
+    const long n = P->n;
+    double** big[] = {&P->density0, &P->density, &P->energy0, &P->energy, &P->u, &P->u0, &P->p,
+                      &P->r, &P->mi, &P->w, &P->kx, &P->ky, &P->sd, &P->u_alt, &P->sd_alt,
+                      &P->volume, &P->x_area, &P->y_area};
+    for (double** b : big) *b = dalloc(n, &rc);
+    if (P->dims == 3) {
+        P->kz = dalloc(n, &rc);
+        P->z_area = dalloc(n, &rc);
+    }
+    P->cell_x = dalloc(P->x + 1, &rc);
+    P->cell_y = dalloc(P->y + 1, &rc);
+    P->cell_z = dalloc(P->z + 1, &rc);
+    P->vertex_x = dalloc(P->x + 1, &rc);
+    P->vertex_y = dalloc(P->y + 1, &rc);
+    P->vertex_z = dalloc(P->z + 1, &rc);
+    const size_t mi = (size_t)s->max_iters + 1;
+    P->cg_alphas.assign(mi, 0.0);
+    P->cg_betas.assign(mi, 0.0);
+    P->cheby_alphas.assign(mi, 0.0);
+    P->cheby_betas.assign(mi, 0.0);
+    if (!rc) {
+        if (P->dims == 2) {
+            rc = tlhip2d_set_chunk_data(P->x, P->y, P->hd, 0, 0, s->xmin, s->ymin, P->dx, P->dy,
+                                        P->cell_x, P->cell_y, P->vertex_x, P->vertex_y, P->volume,
+                                        P->x_area, P->y_area);
+            if (!rc)
+                rc = tlhip2d_set_chunk_state(P->x, P->y, P->vertex_x, P->vertex_y, P->cell_x,
+                                             P->cell_y, P->density, P->energy0, P->u, num_states,
+                                             states);
+        } else {
+            rc = tlhip3d_set_chunk_data(P->x, P->y, P->z, P->hd, 0, 0, 0, s->xmin, s->ymin, s->zmin,
+                                        P->dx, P->dy, P->dz, P->cell_x, P->cell_y, P->cell_z,
+                                        P->vertex_x, P->vertex_y, P->vertex_z, P->volume,
+                                        P->x_area, P->y_area, P->z_area);
+            if (!rc)
+                rc = tlhip3d_set_chunk_state(P->x, P->y, P->z, P->vertex_x, P->vertex_y,
+                                             P->vertex_z, P->cell_x, P->cell_y, P->cell_z,
+                                             P->density, P->energy0, P->u, num_states, states);
+        }
+    }
+    if (!rc) rc = P->prime();
+    if (!rc) rc = tlhip_synchronize();
+    if (rc) {
+        P->release();
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return 0;
+}
+
+namespace {
+// u = e*rho on [1, n-1) of every axis (2d/c_kernels/omp3/set_chunk_state.c:76-83),
+// computed on the host for an injected state and uploaded.
+void host_u(const tlhip_problem* P, const double* d, const double* e, std::vector<double>& u)
+{
+    u.assign((size_t)P->n, 0.0);
+    const int z0 = P->dims == 3 ? 1 : 0, z1 = P->dims == 3 ? P->z - 1 : 1;
+    for (int ii = z0; ii < z1; ++ii)
+        for (int jj = 1; jj < P->y - 1; ++jj)
+            for (int kk = 1; kk < P->x - 1; ++kk) {
+                const long c = kk + (long)jj * P->x + (long)ii * P->x * P->y;
+                u[(size_t)c] = e[c] * d[c];
+            }
+}
+}  // namespace
+
+extern "C" int tlhip_problem_inject(tlhip_problem* P, const double* density, const double* energy0)
+{
+    if (!P || !density || !energy0) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_inject: null argument");
+    const size_t bytes = sizeof(double) * (size_t)P->n;
+    TRY(tlhip_copy_to_device(P->density, density, bytes));
+    TRY(tlhip_copy_to_device(P->energy0, energy0, bytes));
+    std::vector<double> u;
+    host_u(P, density, energy0, u);
+    TRY(tlhip_copy_to_device(P->u, u.data(), bytes));
+    TRY(P->prime());
+    return tlhip_synchronize();
+}
+
+extern "C" int tlhip_problem_step(tlhip_problem* P, int tt, tlhip_step_report* report)
+{
+    if (!P || !report) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_step: null argument");
+    TRY(P->step(tt, report));
+    return tlhip_synchronize();
+}
+
+extern "C" int tlhip_problem_field_summary(tlhip_problem* P, double* out4)
+{
+    if (!P || !out4) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_field_summary: null argument");
+    return P->field_summary(out4);
+}
+
+extern "C" int tlhip_problem_field_len(tlhip_problem* P, long* n)
+{
+    if (!P || !n) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_field_len: null argument");
+    *n = P->n;
+    return 0;
+}
+
+extern "C" int tlhip_problem_get_field(tlhip_problem* P, int id, double* host_out)
+{
+    if (!P || !host_out) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_get_field: null argument");
+    double* f = P->field(id);
+    if (!f) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_get_field: no field %d", id);
+    return tlhip_copy_to_host(host_out, f, sizeof(double) * (size_t)P->n);
+}
+
+extern "C" int tlhip_problem_field_ptr(tlhip_problem* P, int id, void** dev_out)
+{
+    if (!P || !dev_out) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_field_ptr: null argument");
+    double* f = P->field(id);
+    if (!f) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_field_ptr: no field %d", id);
+    *dev_out = f;
+    return 0;
+}
+
+extern "C" int tlhip_problem_destroy(tlhip_problem* P)
+{
+    if (!P) return 0;
+    const int rc = tlhip_synchronize();
+    P->release();
+    delete P;
+    return rc;
+}

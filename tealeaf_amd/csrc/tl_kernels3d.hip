@@ -1,0 +1,862 @@
+// 3D kernels of the hip_cdna4 suite and their C-ABI entry points.
+//
+// Each tlhip3d_* function replaces one run_* entry point of
+// 3d/kernel_interface.h; arguments follow the omp3 binding
+// (3d/c_kernels/omp3/kernel_interface.c) with device pointers for fields and
+// host pointers for scalar results.
+//
+// k_stencil3d applies the 7-point operator.  A wave owns a strip of 64*V
+// x-cells in one y-row and marches along z keeping planes z-1, z, z+1 of the
+// stencil field (and kz at z, z+1) in registers; x-neighbours come from the
+// adjacent lane by DPP and y-neighbours are read from the cache lines the
+// waves of the adjacent rows fetch (a block is 4 consecutive rows of one
+// strip, and tiles are dealt out XCD-contiguously).
+#include "tl_ops.h"
+#include "tealeaf_hip.h"
+
+#include <stdint.h>
+
+int tl_staging(size_t doubles, double** out);
+
+using namespace tl;
+
+namespace {
+
+template <int V>
+struct Nb3 {
+    DV<V> c, up, dn, no, so, l, r;        // k, k+1 (z), k-1, j+1 (y), j-1, x-1, x+1
+    DV<V> kxc, kxr, kyc, kyn, kzc, kzu;   // kx[i], kx[i+1], ky[i], ky[i+x], kz[i], kz[i+xy]
+};
+
+// One row of the operator, association as 3d/shared.h:52-58 (SMVP).
+template <int V>
+__device__ __forceinline__ double apply_a(const Nb3<V>& n, int v)
+{
+    const double diag = ((1.0 + (n.kxr.v[v] + n.kxc.v[v])) + (n.kyn.v[v] + n.kyc.v[v]))
+                      + (n.kzu.v[v] + n.kzc.v[v]);
+    return diag * n.c.v[v] - (n.kxr.v[v] * n.r.v[v] + n.kxc.v[v] * n.l.v[v])
+                           - (n.kyn.v[v] * n.no.v[v] + n.kyc.v[v] * n.so.v[v])
+                           - (n.kzu.v[v] * n.up.v[v] + n.kzc.v[v] * n.dn.v[v]);
+}
+
+struct Tile3 {
+    int ns;    // strips along x
+    int nbj;   // blocks along y (kWaves rows each)
+    int nbk;   // tiles along z
+    int tz;    // planes per tile
+};
+
+__device__ __forceinline__ int xcd_tile(int bid, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, k = bid & 7, i = bid >> 3;
+    return k * q + (k < r ? k : r) + i;
+}
+
+template <int V, class Op>
+__global__ void __launch_bounds__(kBlock)
+k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __restrict__ kx,
+            const double* __restrict__ ky, const double* __restrict__ kz, Op op, TlReduce rd)
+{
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile(blockIdx.x, t.ns * t.nbj * t.nbk);
+    const int sx = tile % t.ns;
+    const int rest = tile / t.ns;
+    const int jb = rest % t.nbj, kb = rest / t.nbj;
+    const int j = g.hd + jb * kWaves + wave;
+    const int xend = g.x - g.hd;
+    const int c0 = g.hd + sx * (TL_WAVE * V) + lane * V;
+    int nval = xend - c0;
+    nval = nval < 0 ? 0 : (nval > V ? V : nval);
+    const bool ld_ok = c0 + V <= g.x;
+    const int k0 = g.hd + kb * t.tz;
+    const int k1 = min(k0 + t.tz, g.z - g.hd);
+    const long X = g.x, XY = (long)g.x * g.y;
+    constexpr int NR = Op::NRED > 0 ? Op::NRED : 1;
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = 0.0;
+
+    if (j < g.y - g.hd) {   // uniform across the wave
+        const long jrow = (long)j * X + c0;
+        DV<V> dn = zdv<V>(), cc = zdv<V>(), kzc = zdv<V>();
+        if (ld_ok) {
+            dn = ldv<V>(a, (long)(k0 - 1) * XY + jrow);
+            cc = ldv<V>(a, (long)k0 * XY + jrow);
+            kzc = ldv<V>(kz, (long)k0 * XY + jrow);
+        }
+        for (int k = k0; k < k1; ++k) {
+            const long b = (long)k * XY + jrow;       // this lane's first cell
+            DV<V> up = zdv<V>(), kzu = zdv<V>(), kxc = zdv<V>(), kyc = zdv<V>(), kyn = zdv<V>();
+            DV<V> no = zdv<V>(), so = zdv<V>();
+            double el = 0.0, er = 0.0, ek = 0.0;
+            if (ld_ok) {
+                up = ldv<V>(a, b + XY);
+                kzu = ldv<V>(kz, b + XY);
+                kxc = ldv<V>(kx, b);
+                kyc = ldv<V>(ky, b);
+                kyn = ldv<V>(ky, b + X);
+                no = ldv<V>(a, b + X);
+                so = ldv<V>(a, b - X);
+            }
+            if (lane == 0 && ld_ok) el = a[b - 1];
+            if (lane == TL_WAVE - 1 && c0 + V < g.x) {
+                er = a[b + V];
+                ek = kx[b + V];
+            }
+            Nb3<V> n;
+            n.c = cc;
+            n.up = up;
+            n.dn = dn;
+            n.no = no;
+            n.so = so;
+            n.kxc = kxc;
+            n.kyc = kyc;
+            n.kyn = kyn;
+            n.kzc = kzc;
+            n.kzu = kzu;
+            const double al = tl_lane_prev(cc.v[V - 1], el);
+            const double ar = tl_lane_next(cc.v[0], er);
+            const double kr = tl_lane_next(kxc.v[0], ek);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                n.l.v[v] = v == 0 ? al : cc.v[v - 1];
+                n.r.v[v] = v == V - 1 ? ar : cc.v[v + 1];
+                n.kxr.v[v] = v == V - 1 ? kr : kxc.v[v + 1];
+            }
+            if (nval > 0) op.template cells<V>(b, nval, n, acc);
+            dn = cc;
+            cc = up;
+            kzc = kzu;
+        }
+    }
+    if constexpr (Op::NRED > 0) tl_reduce_finish<Op::NRED, kBlock>(acc, rd);
+}
+
+// 3D Jacobi update (jacobi_solver.c:72-116).
+struct OpJacobi3 {
+    static constexpr int NRED = 1;
+    const double* __restrict__ u0;
+    double* __restrict__ u;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    {
+        const DV<V> b = ldv<V>(u0, i);
+        DV<V> uo;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double num = ((b.v[v] + (n.kxr.v[v] * n.r.v[v] + n.kxc.v[v] * n.l.v[v]))
+                              + (n.kyn.v[v] * n.no.v[v] + n.kyc.v[v] * n.so.v[v]))
+                              + (n.kzu.v[v] * n.up.v[v] + n.kzc.v[v] * n.dn.v[v]);
+            const double den = ((1.0 + (n.kxc.v[v] + n.kxr.v[v])) + (n.kyc.v[v] + n.kyn.v[v]))
+                             + (n.kzc.v[v] + n.kzu.v[v]);
+            uo.v[v] = num / den;
+            if (v < nval) acc[0] += fabs(uo.v[v] - n.c.v[v]);
+        }
+        stv<V>(u, i, uo, nval);
+    }
+};
+
+// ------------------------------------------------------------ whole-array kernels
+
+__device__ __forceinline__ void decode3(long c, const TlGrid3& g, int& kk, int& jj, int& ii)
+{
+    const long xy = (long)g.x * g.y;
+    ii = (int)(c / xy);
+    const long rem = c - (long)ii * xy;
+    jj = (int)(rem / g.x);
+    kk = (int)(rem - (long)jj * g.x);
+}
+
+// cg_solver_init passes 1-3 in one sweep (cg_solver.c:31-83).
+__global__ void __launch_bounds__(kBlock)
+k_cg_init_fields3(TlGrid3 g, int coef, double rx, double ry, double rz,
+                  const double* __restrict__ density, const double* __restrict__ energy,
+                  double* __restrict__ u, double* __restrict__ p, double* __restrict__ r,
+                  double* __restrict__ w, double* __restrict__ kx, double* __restrict__ ky,
+                  double* __restrict__ kz)
+{
+    const long xy = (long)g.x * g.y, n = xy * g.z;
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock) {
+        int kk, jj, ii;
+        decode3(c, g, kk, jj, ii);
+        const double rho = density[c];
+        p[c] = 0.0;
+        r[c] = 0.0;
+        u[c] = energy[c] * rho;
+        if (ii >= 1 && ii < g.z - 1 && jj >= 1 && jj < g.y - 1 && kk >= 1 && kk < g.x - 1) {
+            const double wc = coef == 1 ? rho : 1.0 / rho;
+            w[c] = wc;
+            if (ii >= g.hd && jj >= g.hd && kk >= g.hd) {
+                const double wl = coef == 1 ? density[c - 1] : 1.0 / density[c - 1];
+                const double wd = coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x];
+                const double wb = coef == 1 ? density[c - xy] : 1.0 / density[c - xy];
+                kx[c] = rx * (wl + wc) / (2.0 * wl * wc);
+                ky[c] = ry * (wd + wc) / (2.0 * wd * wc);
+                kz[c] = rz * (wb + wc) / (2.0 * wb * wc);
+            }
+        }
+    }
+}
+
+// jacobi_solver_init (jacobi_solver.c:9-69).
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_init3(TlGrid3 g, int coef, double rx, double ry, double rz,
+               const double* __restrict__ density, const double* __restrict__ energy,
+               double* __restrict__ u0, double* __restrict__ u, double* __restrict__ kx,
+               double* __restrict__ ky, double* __restrict__ kz)
+{
+    const long xy = (long)g.x * g.y, n = xy * g.z;
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock) {
+        int kk, jj, ii;
+        decode3(c, g, kk, jj, ii);
+        if (ii < 1 || ii >= g.z - 1 || jj < 1 || jj >= g.y - 1 || kk < 1 || kk >= g.x - 1) continue;
+        const double t = energy[c] * density[c];
+        u0[c] = t;
+        u[c] = t;
+        if (ii >= g.hd && jj >= g.hd && kk >= g.hd) {
+            const bool cond = coef == 1;
+            const double wc = cond ? density[c] : 1.0 / density[c];
+            const double wl = cond ? density[c - 1] : 1.0 / density[c - 1];
+            const double wd = cond ? density[c - g.x] : 1.0 / density[c - g.x];
+            const double wb = cond ? density[c - xy] : 1.0 / density[c - xy];
+            kx[c] = rx * (wl + wc) / (2.0 * wl * wc);
+            ky[c] = ry * (wd + wc) / (2.0 * wd * wc);
+            kz[c] = rz * (wb + wc) / (2.0 * wb * wc);
+        }
+    }
+}
+
+// Halo ring (every cell outside the interior box) src -> dst.
+__global__ void __launch_bounds__(kBlock)
+k_copy_ring3d(TlGrid3 g, const double* __restrict__ src, double* __restrict__ dst)
+{
+    const long xy = (long)g.x * g.y;
+    const int nz = g.z - 2 * g.hd, ny = g.y - 2 * g.hd;
+    const long nplane = 2L * g.hd * xy;                 // hd planes front and back
+    const long nrow = 2L * g.hd * g.x * nz;             // hd rows each side, interior planes
+    const long ncol = 2L * g.hd * ny * nz;              // hd columns each side, interior rows
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < nplane + nrow + ncol;
+         t += (long)gridDim.x * kBlock) {
+        long c;
+        if (t < nplane) {
+            const long pl = t / xy, rem = t % xy;
+            const long ii = pl < g.hd ? pl : g.z - 2 * g.hd + pl;
+            c = ii * xy + rem;
+        } else if (t < nplane + nrow) {
+            const long u = t - nplane;
+            const long per = 2L * g.hd * g.x;
+            const long ii = g.hd + u / per, q = u % per;
+            const long rr = q / g.x, kk = q % g.x;
+            const long jj = rr < g.hd ? rr : g.y - 2 * g.hd + rr;
+            c = ii * xy + jj * g.x + kk;
+        } else {
+            const long u = t - nplane - nrow;
+            const long per = 2L * g.hd;
+            const long line = u / per, q = u % per;
+            const long ii = g.hd + line / ny, jj = g.hd + line % ny;
+            const long kk = q < g.hd ? q : g.x - 2 * g.hd + q;
+            c = ii * xy + jj * g.x + kk;
+        }
+        dst[c] = src[c];
+    }
+}
+
+// set_chunk_data (3d set_chunk_data.c:9-67).
+__global__ void __launch_bounds__(kBlock)
+k_set_chunk_data3(TlGrid3 g, double x0, double y0, double z0, double dx, double dy, double dz,
+                  double* cell_x, double* cell_y, double* cell_z, double* vertex_x,
+                  double* vertex_y, double* vertex_z, double* volume, double* x_area,
+                  double* y_area, double* z_area)
+{
+    const long n = (long)g.x * g.y * g.z;
+    const long tot = n + (g.x + 1) + (g.y + 1) + (g.z + 1);
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < tot; t += (long)gridDim.x * kBlock) {
+        if (t < n) {
+            volume[t] = dx * dy * dz;
+            x_area[t] = dy * dz;
+            y_area[t] = dx * dz;
+            z_area[t] = dx * dy;
+        } else if (t < n + g.x + 1) {
+            const int i = (int)(t - n);
+            const double v = x0 + dx * (i - g.hd);
+            vertex_x[i] = v;
+            if (i < g.x) cell_x[i] = 0.5 * (v + (x0 + dx * (i + 1 - g.hd)));
+        } else if (t < n + g.x + g.y + 2) {
+            const int i = (int)(t - n - g.x - 1);
+            const double v = y0 + dy * (i - g.hd);
+            vertex_y[i] = v;
+            if (i < g.y) cell_y[i] = 0.5 * (v + (y0 + dy * (i + 1 - g.hd)));
+        } else {
+            const int i = (int)(t - n - g.x - g.y - 2);
+            const double v = z0 + dz * (i - g.hd);
+            vertex_z[i] = v;
+            if (i < g.z) cell_z[i] = 0.5 * (v + (z0 + dz * (i + 1 - g.hd)));
+        }
+    }
+}
+
+// set_chunk_state (3d set_chunk_state.c:10-97): last covering state wins.
+__global__ void __launch_bounds__(kBlock)
+k_set_chunk_state3(TlGrid3 g, const double* __restrict__ vx, const double* __restrict__ vy,
+                   const double* __restrict__ vz, const double* __restrict__ cx,
+                   const double* __restrict__ cy, const double* __restrict__ cz,
+                   double* __restrict__ density, double* __restrict__ energy0,
+                   double* __restrict__ u, int nstates, const tlhip_state* __restrict__ st)
+{
+    const long n = (long)g.x * g.y * g.z;
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock) {
+        int kk, jj, ii;
+        decode3(c, g, kk, jj, ii);
+        double e = st[0].energy, d = st[0].density;
+        for (int s = 1; s < nstates; ++s) {
+            bool hit = false;
+            if (st[s].geometry == 0) {
+                hit = vx[kk + 1] >= st[s].x_min && vx[kk] < st[s].x_max &&
+                      vy[jj + 1] >= st[s].y_min && vy[jj] < st[s].y_max &&
+                      vz[ii + 1] >= st[s].z_min && vz[ii] < st[s].z_max;
+            } else if (st[s].geometry == 1) {
+                const double ddx = cx[kk] - st[s].x_min;
+                const double ddy = cy[jj] - st[s].y_min;
+                const double ddz = cz[ii] - st[s].z_min;
+                hit = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) <= st[s].radius;
+            } else if (st[s].geometry == 2) {
+                hit = vx[kk] == st[s].x_min && vy[jj] == st[s].y_min && vz[ii] == st[s].z_min;
+            }
+            if (hit) {
+                e = st[s].energy;
+                d = st[s].density;
+            }
+        }
+        energy0[c] = e;
+        density[c] = d;
+        if (ii >= 1 && ii < g.z - 1 && jj >= 1 && jj < g.y - 1 && kk >= 1 && kk < g.x - 1)
+            u[c] = e * d;
+    }
+}
+
+// All flagged fields, all external faces, one launch (local_halos.c:26-222).
+struct HaloArgs3 {
+    double* f[6];
+    int nf;
+    int faces;   // bit f set when neighbours[f] is external (3d/shared.h:24-30)
+    int depth;
+};
+
+__global__ void __launch_bounds__(kBlock) k_local_halos3d(TlGrid3 g, HaloArgs3 h)
+{
+    double* f = h.f[blockIdx.y];
+    const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
+    const long xy = (long)g.x * g.y;
+    const long nlr = (long)ny * nz * h.depth, ntb = (long)nx * nz * h.depth,
+               nfb = (long)nx * ny * h.depth;
+    const long seg[6] = {nlr, nlr, ntb, ntb, nfb, nfb};
+    const long tot = 2 * (nlr + ntb + nfb);
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < tot; t += (long)gridDim.x * kBlock) {
+        int face = 0;
+        long u = t;
+        while (u >= seg[face]) {
+            u -= seg[face];
+            ++face;
+        }
+        if (!(h.faces & (1 << face))) continue;
+        const int d = (int)(u % h.depth);
+        const long q = u / h.depth;
+        if (face < 2) {          // left / right: over (jj, ii)
+            const long jj = g.hd + q % ny, ii = g.hd + q / ny;
+            const long b = ii * xy + jj * g.x;
+            if (face == 0) f[b + (g.hd - d - 1)] = f[b + (g.hd + d)];
+            else f[b + (g.x - g.hd + d)] = f[b + (g.x - g.hd - 1 - d)];
+        } else if (face < 4) {   // bottom / top: over (kk, ii)
+            const long kk = g.hd + q % nx, ii = g.hd + q / nx;
+            const long b = ii * xy + kk;
+            if (face == 2) f[b + (long)(g.hd - d - 1) * g.x] = f[b + (long)(g.hd + d) * g.x];
+            else f[b + (long)(g.y - g.hd + d) * g.x] = f[b + (long)(g.y - g.hd - 1 - d) * g.x];
+        } else {                 // back / front: over (kk, jj)
+            const long kk = g.hd + q % nx, jj = g.hd + q / nx;
+            const long b = jj * g.x + kk;
+            if (face == 4) f[b + (g.hd - d - 1) * xy] = f[b + (g.hd + d) * xy];
+            else f[b + (g.z - g.hd + d) * xy] = f[b + (g.z - g.hd - 1 - d) * xy];
+        }
+    }
+}
+
+// Face box <-> contiguous buffer (3d pack_halos.c:5-363): x, then y, then z.
+__global__ void __launch_bounds__(kBlock)
+k_box3d(TlGrid3 g, int k0, int j0, int i0, int ex, int ey, int ez, int pack,
+        double* __restrict__ field, double* __restrict__ buf)
+{
+    const long tot = (long)ex * ey * ez;
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < tot; t += (long)gridDim.x * kBlock) {
+        const long dk = t % ex, dj = (t / ex) % ey, di = t / ((long)ex * ey);
+        const long c = (k0 + dk) + (j0 + dj) * g.x + (i0 + di) * (long)g.x * g.y;
+        if (pack) buf[t] = field[c];
+        else field[c] = buf[t];
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+int check_grid(const TlGrid3& g, const char* what)
+{
+    if (g.hd < 1 || g.x < 2 * g.hd || g.y < 2 * g.hd || g.z < 2 * g.hd)
+        return tl_fail(TLHIP_ERR_ARG, "%s: invalid grid x=%d y=%d z=%d halo_depth=%d", what, g.x,
+                       g.y, g.z, g.hd);
+    if ((long)g.x * g.y * g.z > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    return 0;
+}
+
+bool interior_empty(const TlGrid3& g)
+{
+    return g.x - 2 * g.hd <= 0 || g.y - 2 * g.hd <= 0 || g.z - 2 * g.hd <= 0;
+}
+
+TlRows rows_of(const TlGrid3& g)
+{
+    const int ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
+    return TlRows{g.x, g.hd, ny, (long)g.x * g.y, g.hd, (long)ny * nz};
+}
+
+template <typename... P>
+int pick_v(const TlGrid3& g, P... ps)
+{
+    return (g.x % 2 == 0 && g.hd % 2 == 0 && g.hd >= 2 && all_aligned(ps...)) ? 2 : 1;
+}
+
+int g_tz_override = -1;
+
+Tile3 make_tile(const TlGrid3& g, int V)
+{
+    Tile3 t;
+    const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
+    t.ns = (int)tl_cdiv(nx, TL_WAVE * V);
+    t.nbj = (int)tl_cdiv(ny, kWaves);
+    int tz;
+    if (g_tz_override > 0) {
+        tz = g_tz_override;
+    } else {
+        tz = (int)tl_cdiv((long)nz * t.ns * t.nbj, 2048);
+        tz = tz < 4 ? 4 : (tz > 64 ? 64 : tz);
+    }
+    while ((long)t.ns * t.nbj * tl_cdiv(nz, tz) > TL_MAX_RED_BLOCKS) tz *= 2;
+    t.tz = tz;
+    t.nbk = (int)tl_cdiv(nz, tz);
+    return t;
+}
+
+template <class Op>
+int launch_stencil(const TlGrid3& g, int V, const double* a, const double* kx, const double* ky,
+                   const double* kz, const Op& op, const char* what)
+{
+    if (interior_empty(g)) return 0;
+    const Tile3 t = make_tile(g, V);
+    const unsigned nb = (unsigned)(t.ns * t.nbj * t.nbk);
+    if (V == 2)
+        hipLaunchKernelGGL((k_stencil3d<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
+                           kx, ky, kz, op, tl_reduce_slots());
+    else
+        hipLaunchKernelGGL((k_stencil3d<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
+                           kx, ky, kz, op, tl_reduce_slots());
+    return tl_check_launch(what);
+}
+
+template <class Op>
+int launch_interior3(const TlGrid3& g, int V, const Op& op, const char* what)
+{
+    if (interior_empty(g)) return 0;
+    return launch_interior(rows_of(g), g.x - 2 * g.hd, V, op, what);
+}
+
+int copy_ring(const TlGrid3& g, const double* src, double* dst, const char* what)
+{
+    const long n = (long)g.x * g.y * g.z;
+    const long inner = (long)(g.x - 2 * g.hd) * (g.y - 2 * g.hd) * (g.z - 2 * g.hd);
+    hipLaunchKernelGGL(k_copy_ring3d, dim3(flat_blocks(n - inner)), dim3(kBlock), 0, tl_stream(),
+                       g, src, dst);
+    return tl_check_launch(what);
+}
+
+#define REQUIRE TL_REQUIRE
+
+}  // namespace
+
+extern "C" int tlhip_set_tile_planes(int planes)
+{
+    g_tz_override = planes;
+    return 0;
+}
+
+// ---------------------------------------------------------------- init
+
+extern "C" int tlhip3d_set_chunk_data(int x, int y, int z, int hd, int left, int bottom, int back,
+                                      double grid_x_min, double grid_y_min, double grid_z_min,
+                                      double dx, double dy, double dz, double* cell_x,
+                                      double* cell_y, double* cell_z, double* vertex_x,
+                                      double* vertex_y, double* vertex_z, double* volume,
+                                      double* x_area, double* y_area, double* z_area)
+{
+    const char* what = "tlhip3d_set_chunk_data";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(cell_x && cell_y && cell_z && vertex_x && vertex_y && vertex_z && volume && x_area &&
+                y_area && z_area,
+            what, "null field");
+    const double x0 = grid_x_min + dx * (double)left;
+    const double y0 = grid_y_min + dy * (double)bottom;
+    const double z0 = grid_z_min + dz * (double)back;
+    const long tot = (long)x * y * z + x + y + z + 3;
+    hipLaunchKernelGGL(k_set_chunk_data3, dim3(flat_blocks(tot)), dim3(kBlock), 0, tl_stream(), g,
+                       x0, y0, z0, dx, dy, dz, cell_x, cell_y, cell_z, vertex_x, vertex_y,
+                       vertex_z, volume, x_area, y_area, z_area);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip3d_set_chunk_state(int x, int y, int z, const double* vertex_x,
+                                       const double* vertex_y, const double* vertex_z,
+                                       const double* cell_x, const double* cell_y,
+                                       const double* cell_z, double* density, double* energy0,
+                                       double* u, int num_states, const tlhip_state* states)
+{
+    const char* what = "tlhip3d_set_chunk_state";
+    TL_TRY(tl_ensure_init());
+    REQUIRE(x >= 1 && y >= 1 && z >= 1, what, "invalid grid");
+    REQUIRE(num_states >= 1 && states, what, "need at least one state");
+    REQUIRE(vertex_x && vertex_y && vertex_z && cell_x && cell_y && cell_z && density &&
+                energy0 && u,
+            what, "null field");
+    const TlGrid3 g{x, y, z, 1};
+    void* dst = nullptr;
+    TL_TRY(tlhip_alloc(sizeof(tlhip_state) * num_states, &dst));
+    int rc = tlhip_copy_to_device(dst, states, sizeof(tlhip_state) * num_states);
+    if (!rc) {
+        hipLaunchKernelGGL(k_set_chunk_state3, dim3(flat_blocks((long)x * y * z)), dim3(kBlock), 0,
+                           tl_stream(), g, vertex_x, vertex_y, vertex_z, cell_x, cell_y, cell_z,
+                           density, energy0, u, num_states, (const tlhip_state*)dst);
+        rc = tl_check_launch(what);
+    }
+    const int rc2 = tlhip_free(dst);
+    return rc ? rc : rc2;
+}
+
+// ---------------------------------------------------------------- halos
+
+extern "C" int tlhip3d_local_halos(int x, int y, int z, int depth, int hd, const int* neighbours,
+                                   const bool* fields, double* density, double* energy0,
+                                   double* energy, double* u, double* p, double* sd)
+{
+    const char* what = "tlhip3d_local_halos";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(neighbours && fields, what, "null neighbours/fields");
+    REQUIRE(depth >= 0 && depth <= hd, what, "depth must be in [0, halo_depth]");
+    HaloArgs3 h{};
+    double* all[6] = {density, energy0, energy, u, p, sd};
+    for (int i = 0; i < 6; ++i)
+        if (fields[i]) {
+            REQUIRE(all[i], what, "flagged field is null");
+            h.f[h.nf++] = all[i];
+        }
+    for (int f = 0; f < 6; ++f)
+        if (neighbours[f] == -1) h.faces |= 1 << f;
+    h.depth = depth;
+    if (h.nf == 0 || h.faces == 0 || depth == 0 || interior_empty(g)) return 0;
+    const long nx = x - 2 * hd, ny = y - 2 * hd, nz = z - 2 * hd;
+    const long tot = 2 * (ny * nz + nx * nz + nx * ny) * depth;
+    hipLaunchKernelGGL(k_local_halos3d, dim3(flat_blocks(tot), h.nf), dim3(kBlock), 0,
+                       tl_stream(), g, h);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip3d_pack_or_unpack(int x, int y, int z, int depth, int hd, int face, bool pack,
+                                      double* field, double* buffer)
+{
+    const char* what = "tlhip3d_pack_or_unpack";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(field && buffer, what, "null field/buffer");
+    REQUIRE(depth >= 1 && depth <= hd, what, "depth must be in [1, halo_depth]");
+    int k0 = hd, j0 = hd, i0 = hd, ex = x - 2 * hd, ey = y - 2 * hd, ez = z - 2 * hd;
+    switch (face) {
+    case 0: k0 = pack ? hd : hd - depth; ex = depth; break;
+    case 1: k0 = pack ? x - hd - depth : x - hd; ex = depth; break;
+    case 2: j0 = pack ? hd : hd - depth; ey = depth; break;
+    case 3: j0 = pack ? y - hd - depth : y - hd; ey = depth; break;
+    case 4: i0 = pack ? hd : hd - depth; ez = depth; break;
+    case 5: i0 = pack ? z - hd - depth : z - hd; ez = depth; break;
+    default: return tl_fail(TLHIP_ERR_ARG, "%s: Incorrect face provided: %d.", what, face);
+    }
+    const long n = (long)ex * ey * ez;
+    if (n <= 0) return 0;
+    double* stage = nullptr;
+    TL_TRY(tl_staging((size_t)n, &stage));
+    if (!pack) TL_TRY(tlhip_copy_to_device(stage, buffer, sizeof(double) * n));
+    hipLaunchKernelGGL(k_box3d, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), g, k0, j0, i0,
+                       ex, ey, ez, pack ? 1 : 0, field, stage);
+    TL_TRY(tl_check_launch(what));
+    if (pack) TL_TRY(tlhip_copy_to_host(buffer, stage, sizeof(double) * n));
+    return 0;
+}
+
+extern "C" int tlhip3d_store_energy(int x, int y, int z, const double* energy0, double* energy)
+{
+    const char* what = "tlhip3d_store_energy";
+    TL_TRY(tl_ensure_init());
+    REQUIRE(x >= 1 && y >= 1 && z >= 1 && energy0 && energy, what, "invalid arguments");
+    const long n = (long)x * y * z;
+    hipLaunchKernelGGL(k_copy_all, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), n, energy0,
+                       energy);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip3d_field_summary(int x, int y, int z, int hd, const double* volume,
+                                     const double* density, const double* energy0,
+                                     const double* u, double* vol, double* mass, double* ie,
+                                     double* temp)
+{
+    const char* what = "tlhip3d_field_summary";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(volume && density && energy0 && u && vol && mass && ie && temp, what, "null argument");
+    if (interior_empty(g)) return 0;
+    OpFieldSummary op{volume, density, energy0, u};
+    TL_TRY(launch_interior3(g, pick_v(g, volume, density, energy0, u), op, what));
+    double s[4];
+    TL_TRY(tl_fetch(4, s, what));
+    *vol += s[0];
+    *mass += s[1];
+    *ie += s[2];
+    *temp += s[3];
+    return 0;
+}
+
+// ---------------------------------------------------------------- CG
+
+extern "C" int tlhip3d_cg_solver_init(int x, int y, int z, int hd, int coefficient, double rx,
+                                      double ry, double rz, double* rro, const double* density,
+                                      const double* energy, double* u, double* p, double* r,
+                                      double* w, double* kx, double* ky, double* kz)
+{
+    const char* what = "tlhip3d_cg_solver_init";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    if (coefficient != 1 && coefficient != 2)
+        return tl_fail(TLHIP_ERR_ARG, "Coefficient %d is not valid.", coefficient);
+    REQUIRE(rro && density && energy && u && p && r && w && kx && ky && kz, what, "null argument");
+    const long n = (long)x * y * z;
+    hipLaunchKernelGGL(k_cg_init_fields3, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), g,
+                       coefficient, rx, ry, rz, density, energy, u, p, r, w, kx, ky, kz);
+    TL_TRY(tl_check_launch(what));
+    if (interior_empty(g)) return 0;
+    OpCgInit op{w, r, p};
+    TL_TRY(launch_stencil(g, pick_v(g, u, kx, ky, kz, w, r, p), u, kx, ky, kz, op, what));
+    return fetch_add(rro, what);
+}
+
+extern "C" int tlhip3d_cg_solver_calc_w(int x, int y, int z, int hd, double* pw, const double* p,
+                                        double* w, const double* kx, const double* ky,
+                                        const double* kz)
+{
+    const char* what = "tlhip3d_cg_solver_calc_w";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(pw && p && w && kx && ky && kz, what, "null argument");
+    REQUIRE(p != w, what, "p and w must be distinct buffers");
+    if (interior_empty(g)) return 0;
+    OpCalcW op{w};
+    TL_TRY(launch_stencil(g, pick_v(g, p, w, kx, ky, kz), p, kx, ky, kz, op, what));
+    return fetch_add(pw, what);
+}
+
+extern "C" int tlhip3d_cg_solver_calc_ur(int x, int y, int z, int hd, double alpha, double* rrn,
+                                         double* u, const double* p, double* r, const double* w)
+{
+    const char* what = "tlhip3d_cg_solver_calc_ur";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(rrn && u && p && r && w, what, "null argument");
+    if (interior_empty(g)) return 0;
+    OpCalcUr op{u, p, r, w, alpha};
+    TL_TRY(launch_interior3(g, pick_v(g, u, p, r, w), op, what));
+    return fetch_add(rrn, what);
+}
+
+extern "C" int tlhip3d_cg_solver_calc_p(int x, int y, int z, int hd, double beta, double* p,
+                                        const double* r)
+{
+    const char* what = "tlhip3d_cg_solver_calc_p";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(p && r, what, "null argument");
+    OpCalcP op{p, r, beta};
+    return launch_interior3(g, pick_v(g, p, r), op, what);
+}
+
+// ---------------------------------------------------------------- Chebyshev
+
+extern "C" int tlhip3d_cheby_solver_init(int x, int y, int z, int hd, double theta, double* u,
+                                         const double* u0, double* p, double* r, double* w,
+                                         const double* kx, const double* ky, const double* kz)
+{
+    const char* what = "tlhip3d_cheby_solver_init";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && u0 && p && r && w && kx && ky && kz, what, "null argument");
+    OpChebyInit op{u0, w, r, p, theta};
+    TL_TRY(launch_stencil(g, pick_v(g, u, u0, p, r, w, kx, ky, kz), u, kx, ky, kz, op, what));
+    OpSimple up{u, p, 1, nullptr, 0.0};
+    return launch_interior3(g, pick_v(g, u, p), up, what);
+}
+
+extern "C" int tlhip3d_cheby_solver_iterate(int x, int y, int z, int hd, double alpha,
+                                            double beta, const double* u_in, double* u_out,
+                                            const double* u0, double* p, double* r, double* w,
+                                            const double* kx, const double* ky, const double* kz)
+{
+    const char* what = "tlhip3d_cheby_solver_iterate";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u_in && u_out && u0 && p && r && kx && ky && kz, what, "null argument");
+    REQUIRE(u_in != u_out, what, "u_in and u_out must be distinct buffers");
+    OpChebyIter op{u0, p, r, w, u_out, alpha, beta};
+    TL_TRY(launch_stencil(g, pick_v(g, u_in, u_out, u0, p, r, w, kx, ky, kz), u_in, kx, ky, kz,
+                          op, what));
+    return copy_ring(g, u_in, u_out, what);
+}
+
+// ---------------------------------------------------------------- Jacobi
+
+extern "C" int tlhip3d_jacobi_solver_init(int x, int y, int z, int hd, int coefficient, double rx,
+                                          double ry, double rz, const double* density,
+                                          const double* energy, double* u0, double* u, double* kx,
+                                          double* ky, double* kz)
+{
+    const char* what = "tlhip3d_jacobi_solver_init";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    if (coefficient < 1) return tl_fail(TLHIP_ERR_ARG, "Coefficient %d is not valid.", coefficient);
+    REQUIRE(density && energy && u0 && u && kx && ky && kz, what, "null argument");
+    hipLaunchKernelGGL(k_jacobi_init3, dim3(flat_blocks((long)x * y * z)), dim3(kBlock), 0,
+                       tl_stream(), g, coefficient, rx, ry, rz, density, energy, u0, u, kx, ky,
+                       kz);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip3d_jacobi_solver_iterate(int x, int y, int z, int hd, double* error,
+                                             const double* kx, const double* ky, const double* kz,
+                                             const double* u0, double* u, double* r)
+{
+    const char* what = "tlhip3d_jacobi_solver_iterate";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(error && kx && ky && kz && u0 && u && r, what, "null argument");
+    REQUIRE(u != r, what, "u and r must be distinct buffers");
+    const long n = (long)x * y * z;
+    hipLaunchKernelGGL(k_copy_all, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), n, u, r);
+    TL_TRY(tl_check_launch(what));
+    if (interior_empty(g)) {
+        *error = 0.0;
+        return 0;
+    }
+    OpJacobi3 op{u0, u};
+    TL_TRY(launch_stencil(g, pick_v(g, kx, ky, kz, u0, u, r), r, kx, ky, kz, op, what));
+    double v = 0.0;
+    TL_TRY(tl_fetch(1, &v, what));
+    *error = v;   // assigned, as jacobi_solver.c:115
+    return 0;
+}
+
+// ---------------------------------------------------------------- PPCG
+
+extern "C" int tlhip3d_ppcg_init(int x, int y, int z, int hd, double theta, const double* r,
+                                 double* sd)
+{
+    const char* what = "tlhip3d_ppcg_init";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(r && sd, what, "null argument");
+    OpSimple op{sd, r, 2, nullptr, theta};
+    return launch_interior3(g, pick_v(g, r, sd), op, what);
+}
+
+extern "C" int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int hd, double alpha,
+                                            double beta, double* u, double* r, const double* kx,
+                                            const double* ky, const double* kz,
+                                            const double* sd_in, double* sd_out)
+{
+    const char* what = "tlhip3d_ppcg_inner_iteration";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && r && kx && ky && kz && sd_in && sd_out, what, "null argument");
+    REQUIRE(sd_in != sd_out, what, "sd_in and sd_out must be distinct buffers");
+    OpPpcgInner op{r, u, sd_out, alpha, beta};
+    TL_TRY(launch_stencil(g, pick_v(g, u, r, kx, ky, kz, sd_in, sd_out), sd_in, kx, ky, kz, op,
+                          what));
+    return copy_ring(g, sd_in, sd_out, what);
+}
+
+// ---------------------------------------------------------------- shared
+
+extern "C" int tlhip3d_solver_copy_u(int x, int y, int z, int hd, double* u0, const double* u)
+{
+    const char* what = "tlhip3d_solver_copy_u";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u0 && u, what, "null argument");
+    OpSimple op{u0, u, 0, nullptr, 0.0};
+    return launch_interior3(g, pick_v(g, u0, u), op, what);
+}
+
+extern "C" int tlhip3d_calculate_residual(int x, int y, int z, int hd, const double* u,
+                                          const double* u0, double* r, const double* kx,
+                                          const double* ky, const double* kz)
+{
+    const char* what = "tlhip3d_calculate_residual";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && u0 && r && kx && ky && kz, what, "null argument");
+    REQUIRE(u != r, what, "u and r must be distinct buffers");
+    OpResidual op{u0, r};
+    return launch_stencil(g, pick_v(g, u, u0, r, kx, ky, kz), u, kx, ky, kz, op, what);
+}
+
+extern "C" int tlhip3d_calculate_2norm(int x, int y, int z, int hd, const double* buffer,
+                                       double* norm)
+{
+    const char* what = "tlhip3d_calculate_2norm";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(buffer && norm, what, "null argument");
+    if (interior_empty(g)) return 0;
+    OpNorm op{buffer};
+    TL_TRY(launch_interior3(g, pick_v(g, buffer), op, what));
+    return fetch_add(norm, what);
+}
+
+extern "C" int tlhip3d_solver_finalise(int x, int y, int z, int hd, double* energy,
+                                       const double* density, const double* u)
+{
+    const char* what = "tlhip3d_solver_finalise";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(energy && density && u, what, "null argument");
+    OpSimple op{energy, u, 3, density, 0.0};
+    return launch_interior3(g, pick_v(g, energy, density, u), op, what);
+}

@@ -1,0 +1,383 @@
+// Per-cell operations shared by the 2D and 3D kernels.
+//
+// Vector helpers move V consecutive doubles of one row (V=2: one 16-byte
+// access).  Pointwise ops act on interior rows through k_interior; stencil
+// ops receive a neighbourhood `N` (Nb2 or Nb3) and call apply_a(n, v),
+// which each dimension defines with the reference's association.
+#pragma once
+
+#include "tl_common.h"
+
+namespace tl {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / TL_WAVE;
+
+template <int V>
+struct DV {
+    double v[V];
+};
+
+template <int V>
+__device__ __forceinline__ DV<V> ldv(const double* __restrict__ p, long i)
+{
+    DV<V> r;
+    if constexpr (V == 2) {
+        const double2 t = *reinterpret_cast<const double2*>(p + i);
+        r.v[0] = t.x;
+        r.v[1] = t.y;
+    } else {
+        r.v[0] = p[i];
+    }
+    return r;
+}
+
+template <int V>
+__device__ __forceinline__ void stv(double* __restrict__ p, long i, const DV<V>& x, int nval)
+{
+    if constexpr (V == 2) {
+        if (nval == 2)
+            *reinterpret_cast<double2*>(p + i) = make_double2(x.v[0], x.v[1]);
+        else if (nval == 1)
+            p[i] = x.v[0];
+    } else {
+        if (nval >= 1) p[i] = x.v[0];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ DV<V> zdv()
+{
+    DV<V> r;
+#pragma unroll
+    for (int v = 0; v < V; ++v) r.v[v] = 0.0;
+    return r;
+}
+
+// Interior rows of a 2D or 3D chunk.  Row r (0-based over the interior) is
+// at offset (hd + r % ny)*x + (zoff + r / ny)*xy; 2D uses zoff = xy = 0.
+struct TlRows {
+    int x, hd, ny;
+    long xy;
+    int zoff;
+    long nrows;
+};
+
+// Pointwise work over interior rows, block-strided row segments of
+// kBlock*V columns.
+template <int V, class Op>
+__global__ void __launch_bounds__(kBlock)
+k_interior(TlRows g, int segs_per_row, long nseg, Op op, TlReduce rd)
+{
+    constexpr int NR = Op::NRED > 0 ? Op::NRED : 1;
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = 0.0;
+    const int xend = g.x - g.hd;
+    for (long s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const long r = s / segs_per_row;
+        const long base = (long)(g.hd + r % g.ny) * g.x + (long)(g.zoff + r / g.ny) * g.xy;
+        const int c0 = g.hd + (int)(s % segs_per_row) * (kBlock * V) + threadIdx.x * V;
+        int nval = xend - c0;
+        nval = nval < 0 ? 0 : (nval > V ? V : nval);
+        if (nval > 0) op.template cells<V>(base + c0, nval, acc);
+    }
+    if constexpr (Op::NRED > 0) tl_reduce_finish<Op::NRED, kBlock>(acc, rd);
+}
+
+// ------------------------------------------------------------ pointwise ops
+
+struct OpCalcUr {   // u += alpha p ; r -= alpha w ; rrn += r.r  (2d/c_kernels/omp3/cg.c:116-143)
+    static constexpr int NRED = 1;
+    double* __restrict__ u;
+    const double* __restrict__ p;
+    double* __restrict__ r;
+    const double* __restrict__ w;
+    double alpha;
+    template <int V>
+    __device__ void cells(long i, int nval, double* acc) const
+    {
+        DV<V> uu = ldv<V>(u, i), pp = ldv<V>(p, i), rr = ldv<V>(r, i), ww = ldv<V>(w, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            uu.v[v] += alpha * pp.v[v];
+            rr.v[v] -= alpha * ww.v[v];
+            if (v < nval) acc[0] += rr.v[v] * rr.v[v];
+        }
+        stv<V>(u, i, uu, nval);
+        stv<V>(r, i, rr, nval);
+    }
+};
+
+struct OpCalcP {    // p = beta p + r  (2d/c_kernels/omp3/cg.c:146-164)
+    static constexpr int NRED = 0;
+    double* __restrict__ p;
+    const double* __restrict__ r;
+    double beta;
+    template <int V>
+    __device__ void cells(long i, int nval, double*) const
+    {
+        DV<V> pp = ldv<V>(p, i);
+        const DV<V> rr = ldv<V>(r, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v) pp.v[v] = beta * pp.v[v] + rr.v[v];
+        stv<V>(p, i, pp, nval);
+    }
+};
+
+// Small pointwise forms:
+//   0: dst = src                    (copy_u, solver_methods.c:9-25)
+//   1: dst = dst + src              (cheby_calc_u, cheby.c:8-24)
+//   2: dst = src / s                (ppcg_init, ppcg.c:8-25)
+//   3: dst = src / other            (finalise, solver_methods.c:74-91)
+struct OpSimple {
+    static constexpr int NRED = 0;
+    double* __restrict__ dst;
+    const double* __restrict__ src;
+    int mode;
+    const double* __restrict__ other;
+    double s;
+    template <int V>
+    __device__ void cells(long i, int nval, double*) const
+    {
+        const DV<V> a = ldv<V>(src, i);
+        DV<V> o = zdv<V>(), d = zdv<V>();
+        if (mode == 1) o = ldv<V>(dst, i);
+        if (mode == 3) d = ldv<V>(other, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (mode == 0) o.v[v] = a.v[v];
+            else if (mode == 1) o.v[v] = o.v[v] + a.v[v];
+            else if (mode == 2) o.v[v] = a.v[v] / s;
+            else o.v[v] = a.v[v] / d.v[v];
+        }
+        stv<V>(dst, i, o, nval);
+    }
+};
+
+struct OpNorm {     // norm += b.b  (solver_methods.c:51-71)
+    static constexpr int NRED = 1;
+    const double* __restrict__ b;
+    template <int V>
+    __device__ void cells(long i, int nval, double* acc) const
+    {
+        const DV<V> bb = ldv<V>(b, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (v < nval) acc[0] += bb.v[v] * bb.v[v];
+    }
+};
+
+struct OpFieldSummary {   // field_summary.c:9-45
+    static constexpr int NRED = 4;
+    const double* __restrict__ volume;
+    const double* __restrict__ density;
+    const double* __restrict__ energy0;
+    const double* __restrict__ u;
+    template <int V>
+    __device__ void cells(long i, int nval, double* acc) const
+    {
+        const DV<V> vv = ldv<V>(volume, i), dd = ldv<V>(density, i);
+        const DV<V> ee = ldv<V>(energy0, i), uu = ldv<V>(u, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (v < nval) {
+                const double cm = vv.v[v] * dd.v[v];
+                acc[0] += vv.v[v];
+                acc[1] += cm;
+                acc[2] += cm * ee.v[v];
+                acc[3] += cm * uu.v[v];
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------ stencil ops
+// N provides: c (field at the cell, DV<V>) and apply_a(n, v).
+
+struct OpCalcW {   // w = A p ; pw += w.p  (cg.c:88-113)
+    static constexpr int NRED = 1;
+    double* __restrict__ w;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    {
+        DV<V> o;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            o.v[v] = apply_a(n, v);
+            if (v < nval) acc[0] += o.v[v] * n.c.v[v];
+        }
+        stv<V>(w, i, o, nval);
+    }
+};
+
+struct OpCgInit {  // w = A u ; r = u - w ; p = r ; rro += r.p  (cg.c:71-84)
+    static constexpr int NRED = 1;
+    double* __restrict__ w;
+    double* __restrict__ r;
+    double* __restrict__ p;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    {
+        DV<V> wo, ro;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            wo.v[v] = apply_a(n, v);
+            ro.v[v] = n.c.v[v] - wo.v[v];
+            if (v < nval) acc[0] += ro.v[v] * ro.v[v];
+        }
+        stv<V>(w, i, wo, nval);
+        stv<V>(r, i, ro, nval);
+        stv<V>(p, i, ro, nval);
+    }
+};
+
+struct OpChebyInit {   // w = A u ; r = u0 - w ; p = r/theta  (cheby.c:27-54)
+    static constexpr int NRED = 0;
+    const double* __restrict__ u0;
+    double* __restrict__ w;
+    double* __restrict__ r;
+    double* __restrict__ p;
+    double theta;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double*) const
+    {
+        const DV<V> b = ldv<V>(u0, i);
+        DV<V> wo, ro, po;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            wo.v[v] = apply_a(n, v);
+            ro.v[v] = b.v[v] - wo.v[v];
+            po.v[v] = ro.v[v] / theta;
+        }
+        stv<V>(w, i, wo, nval);
+        stv<V>(r, i, ro, nval);
+        stv<V>(p, i, po, nval);
+    }
+};
+
+// Fused Chebyshev step (cheby.c:57-85): w = A u_in, r = u0 - w,
+// p = alpha p + beta r, u_out = u_in + p; w written only if given.
+struct OpChebyIter {
+    static constexpr int NRED = 0;
+    const double* __restrict__ u0;
+    double* __restrict__ p;
+    double* __restrict__ r;
+    double* __restrict__ w;
+    double* __restrict__ u_out;
+    double alpha, beta;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double*) const
+    {
+        const DV<V> b = ldv<V>(u0, i);
+        const DV<V> pi = ldv<V>(p, i);
+        DV<V> wo, ro, po, uo;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            wo.v[v] = apply_a(n, v);
+            ro.v[v] = b.v[v] - wo.v[v];
+            po.v[v] = alpha * pi.v[v] + beta * ro.v[v];
+            uo.v[v] = n.c.v[v] + po.v[v];
+        }
+        if (w) stv<V>(w, i, wo, nval);
+        stv<V>(r, i, ro, nval);
+        stv<V>(p, i, po, nval);
+        stv<V>(u_out, i, uo, nval);
+    }
+};
+
+// Fused PPCG inner step (ppcg.c:28-61): r -= A sd_in, u += sd_in,
+// sd_out = alpha sd_in + beta r.
+struct OpPpcgInner {
+    static constexpr int NRED = 0;
+    double* __restrict__ r;
+    double* __restrict__ u;
+    double* __restrict__ sd_out;
+    double alpha, beta;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double*) const
+    {
+        const DV<V> ri = ldv<V>(r, i);
+        const DV<V> ui = ldv<V>(u, i);
+        DV<V> ro, uo, so;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            ro.v[v] = ri.v[v] - apply_a(n, v);
+            uo.v[v] = ui.v[v] + n.c.v[v];
+            so.v[v] = alpha * n.c.v[v] + beta * ro.v[v];
+        }
+        stv<V>(r, i, ro, nval);
+        stv<V>(u, i, uo, nval);
+        stv<V>(sd_out, i, so, nval);
+    }
+};
+
+struct OpResidual {   // r = u0 - A u  (solver_methods.c:28-48)
+    static constexpr int NRED = 0;
+    const double* __restrict__ u0;
+    double* __restrict__ r;
+    template <int V, class N>
+    __device__ void cells(long i, int nval, const N& n, double*) const
+    {
+        const DV<V> b = ldv<V>(u0, i);
+        DV<V> ro;
+#pragma unroll
+        for (int v = 0; v < V; ++v) ro.v[v] = b.v[v] - apply_a(n, v);
+        stv<V>(r, i, ro, nval);
+    }
+};
+
+// ------------------------------------------------------------ whole-array
+
+__global__ void __launch_bounds__(kBlock)
+k_copy_all(long n, const double* __restrict__ src, double* __restrict__ dst);
+
+static inline unsigned flat_blocks(long n)
+{
+    long b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (unsigned)b;
+}
+
+static inline bool aligned16(const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; }
+
+template <typename... P>
+static inline bool all_aligned(P... ps)
+{
+    return (aligned16(ps) && ...);
+}
+
+// Launch a pointwise op over the interior rows.
+template <class Op>
+static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const char* what)
+{
+    if (nx <= 0 || g.nrows <= 0) return 0;
+    const int spr = (int)tl_cdiv(nx, kBlock * V);
+    const long nseg = (long)spr * g.nrows;
+    const unsigned nb = (unsigned)(nseg < 4096 ? nseg : 4096);
+    if (V == 2)
+        hipLaunchKernelGGL((k_interior<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
+                           nseg, op, tl_reduce_slots());
+    else
+        hipLaunchKernelGGL((k_interior<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
+                           nseg, op, tl_reduce_slots());
+    return tl_check_launch(what);
+}
+
+// Add the published reduction value to *dst (the reference's += contract).
+static inline int fetch_add(double* dst, const char* what)
+{
+    if (!dst) return tl_fail(1, "%s: null scalar pointer", what);
+    double v = 0.0;
+    TL_TRY(tl_fetch(1, &v, what));
+    *dst += v;
+    return 0;
+}
+
+}  // namespace tl
+
+#define TL_REQUIRE(cond, what, msg)                             \
+    do {                                                        \
+        if (!(cond)) return tl_fail(1, "%s: %s", what, msg);    \
+    } while (0)

@@ -1,0 +1,232 @@
+// Process-wide runtime of the hip_cdna4 suite: device selection, the one
+// in-order stream, reduction slots, the pinned result mailbox, error state,
+// and the memory entry points of the C ABI.
+//
+// The reference kernel ABI is single-threaded and sequential (one chunk per
+// rank, 2d/drivers/*_driver.c call run_* one after another), and so is this
+// runtime: calls from several host threads at once are not supported.
+#include "tl_ops.h"
+#include "tealeaf_hip.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+namespace {
+
+struct Runtime {
+    int initialised = 0;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int own_stream = 0;
+    TlReduce red{};
+    double* staging = nullptr;      // device scratch for pack/unpack
+    size_t staging_len = 0;
+};
+
+Runtime g_rt;
+thread_local char g_err[512] = "";
+
+}  // namespace
+
+int tl_fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char* what)
+{
+    return tl_fail(TLHIP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                    \
+    do {                                       \
+        hipError_t _e = (call);                \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+extern "C" int tlhip_init(int device)
+{
+    if (g_rt.initialised) {
+        if (device >= 0 && device != g_rt.device)
+            return tl_fail(TLHIP_ERR_ARG, "tlhip_init: already initialised on device %d",
+                           g_rt.device);
+        return 0;
+    }
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (count < 1) return tl_fail(TLHIP_ERR_HIP, "tlhip_init: no HIP device");
+    const int dev = device < 0 ? 0 : device % count;
+    HIP_TRY(hipSetDevice(dev), "hipSetDevice");
+    // Default priority on purpose: streams of mixed priority are not used.
+    HIP_TRY(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking), "hipStreamCreate");
+    g_rt.own_stream = 1;
+    HIP_TRY(hipMalloc(&g_rt.red.partials, sizeof(double) * TL_MAX_RED * TL_MAX_RED_BLOCKS),
+            "hipMalloc(partials)");
+    HIP_TRY(hipMalloc(&g_rt.red.ticket, 256), "hipMalloc(ticket)");
+    HIP_TRY(hipMemset(g_rt.red.ticket, 0, 256), "hipMemset(ticket)");
+    HIP_TRY(hipMalloc(&g_rt.red.result, sizeof(double) * TL_MAX_RED), "hipMalloc(result)");
+    HIP_TRY(hipHostMalloc(&g_rt.red.mailbox, 256, hipHostMallocMapped | hipHostMallocCoherent),
+            "hipHostMalloc(mailbox)");
+    memset(g_rt.red.mailbox, 0, 256);
+    g_rt.device = dev;
+    g_rt.initialised = 1;
+    return 0;
+}
+
+int tl_ensure_init() { return g_rt.initialised ? 0 : tlhip_init(-1); }
+
+hipStream_t tl_stream() { return g_rt.stream; }
+TlReduce tl_reduce_slots() { return g_rt.red; }
+
+int tl_check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return 0;
+}
+
+int tl_fetch(int n, double* out, const char* what)
+{
+    hipError_t e = hipStreamSynchronize(g_rt.stream);
+    if (e != hipSuccess) return hip_fail(e, what);
+    for (int i = 0; i < n; ++i) out[i] = ((volatile double*)g_rt.red.mailbox)[i];
+    return 0;
+}
+
+extern "C" const char* tlhip_last_error(void) { return g_err; }
+
+extern "C" int tlhip_device(void) { return g_rt.initialised ? g_rt.device : -1; }
+
+extern "C" int tlhip_set_stream(void* stream)
+{
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "hipStreamSynchronize");
+    if (g_rt.own_stream) {
+        HIP_TRY(hipStreamDestroy(g_rt.stream), "hipStreamDestroy");
+        g_rt.own_stream = 0;
+    }
+    g_rt.stream = (hipStream_t)stream;
+    return 0;
+}
+
+extern "C" void* tlhip_get_stream(void) { return (void*)g_rt.stream; }
+
+extern "C" int tlhip_synchronize(void)
+{
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "hipStreamSynchronize");
+    return 0;
+}
+
+extern "C" int tlhip_finalise(void)
+{
+    if (!g_rt.initialised) return 0;
+    // Teardown: report the first failure but release everything regardless.
+    hipError_t first = hipStreamSynchronize(g_rt.stream);
+    hipError_t e[6] = {hipFree(g_rt.red.partials), hipFree(g_rt.red.ticket),
+                       hipFree(g_rt.red.result), hipHostFree(g_rt.red.mailbox),
+                       g_rt.staging ? hipFree(g_rt.staging) : hipSuccess,
+                       g_rt.own_stream ? hipStreamDestroy(g_rt.stream) : hipSuccess};
+    for (hipError_t x : e)
+        if (first == hipSuccess) first = x;
+    g_rt = Runtime{};
+    return first == hipSuccess ? 0 : hip_fail(first, "tlhip_finalise");
+}
+
+// ------------------------------------------------------------ memory ABI
+extern "C" int tlhip_alloc(size_t bytes, void** out)
+{
+    TL_TRY(tl_ensure_init());
+    *out = nullptr;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, bytes ? bytes : 8), "hipMalloc");
+    hipError_t e = hipMemsetAsync(p, 0, bytes ? bytes : 8, g_rt.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_rt.stream);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "tlhip_alloc zero fill");
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int tlhip_free(void* ptr)
+{
+    if (!ptr) return 0;
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "hipStreamSynchronize");
+    HIP_TRY(hipFree(ptr), "hipFree");
+    return 0;
+}
+
+extern "C" int tlhip_copy_to_device(void* dst, const void* src, size_t bytes)
+{
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_rt.stream), "H2D copy");
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "H2D sync");
+    return 0;
+}
+
+extern "C" int tlhip_copy_to_host(void* dst, const void* src, size_t bytes)
+{
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_rt.stream), "D2H copy");
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "D2H sync");
+    return 0;
+}
+
+extern "C" int tlhip_copy_device(void* dst, const void* src, size_t bytes)
+{
+    TL_TRY(tl_ensure_init());
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_rt.stream), "D2D copy");
+    return 0;
+}
+
+// Device staging area for halo pack/unpack (grown on demand, never shrunk).
+int tl_staging(size_t doubles, double** out)
+{
+    TL_TRY(tl_ensure_init());
+    if (doubles > g_rt.staging_len) {
+        HIP_TRY(hipStreamSynchronize(g_rt.stream), "hipStreamSynchronize");
+        if (g_rt.staging) HIP_TRY(hipFree(g_rt.staging), "hipFree(staging)");
+        g_rt.staging = nullptr;
+        g_rt.staging_len = 0;
+        HIP_TRY(hipMalloc(&g_rt.staging, doubles * sizeof(double)), "hipMalloc(staging)");
+        g_rt.staging_len = doubles;
+    }
+    *out = g_rt.staging;
+    return 0;
+}
+
+namespace tl {
+__global__ void __launch_bounds__(kBlock)
+k_copy_all(long n, const double* __restrict__ src, double* __restrict__ dst)
+{
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock)
+        dst[c] = src[c];
+}
+}  // namespace tl
+
+// In-bounds check of the DPP lane-shift helpers: out[l] = in[l-1] (or -1 at
+// lane 0) and out[64+l] = in[l+1] (or -1 at lane 63), for one wavefront.
+__global__ void __launch_bounds__(64) k_lane_selftest(const double* in, double* out)
+{
+    const int l = threadIdx.x;
+    const double v = in[l];
+    out[l] = tl_lane_prev(v, -1.0);
+    out[64 + l] = tl_lane_next(v, -1.0);
+}
+
+extern "C" int tlhip_selftest_lanes(const double* in_dev, double* out_dev)
+{
+    TL_TRY(tl_ensure_init());
+    hipLaunchKernelGGL(k_lane_selftest, dim3(1), dim3(64), 0, g_rt.stream, in_dev, out_dev);
+    TL_TRY(tl_check_launch("k_lane_selftest"));
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "selftest sync");
+    return 0;
+}

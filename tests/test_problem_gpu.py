@@ -1,0 +1,103 @@
+"""Whole-solve parity on the GPU.
+
+* The suite's native driver against the CPU oracle driver on the same deck:
+  every timestep's printed iteration counts within +-1, the field summary
+  (vol, mass, ie, temp) within relative 1e-12.
+* The drop-in: the reference's own C host, unchanged, linked with the
+  hip_cdna4 suite (oracle/_ref/tealeaf{2d,3d}_hip) against the same host
+  with the reference omp3 suite (oracle/_ref/tealeaf{2d,3d}_omp3), and the
+  reference's stored known answers (tea.problems).
+"""
+import numpy as np
+import pytest
+
+from tealeaf_amd import deck as tdeck
+from tests.problem_helpers import (known_answer, oracle_run, printed_iters, ref_run,
+                                   reference_check_passes, rel)
+
+pytestmark = pytest.mark.gpu
+
+SUMMARY_RTOL = 1e-12
+DECKS = [
+    ("2d-cg", tdeck.layered_2d(64, 2, max_iters=5000)),
+    ("2d-cg-odd", tdeck.layered_2d(77, 2, max_iters=5000)),
+    ("2d-cheby", tdeck.layered_2d(96, 3, solver=tdeck.CHEBY, presteps=20, max_iters=2000)),
+    ("2d-ppcg", tdeck.layered_2d(96, 2, solver=tdeck.PPCG, presteps=20, inner=12, max_iters=2000)),
+    ("2d-jacobi", tdeck.layered_2d(40, 2, solver=tdeck.JACOBI, eps=1e-8, max_iters=400)),
+    ("3d-cg", tdeck.cuboid_3d(20, 3)),
+    ("3d-cg-odd", tdeck.cuboid_3d(17, 2)),
+    ("3d-ppcg-recip", tdeck.cuboid_3d(20, 2, solver=tdeck.PPCG, inner=8, coefficient=2)),
+    ("3d-cheby", tdeck.cuboid_3d(20, 2, solver=tdeck.CHEBY)),
+    ("3d-jacobi", tdeck.cuboid_3d(12, 2, solver=tdeck.JACOBI, eps=1e-8, max_iters=300)),
+]
+
+
+def _compare(steps_a, steps_b, solver):
+    assert len(steps_a) == len(steps_b)
+    for a, b in zip(steps_a, steps_b):
+        pa, pb = printed_iters(a, solver), printed_iters(b, solver)
+        for k in pa:
+            assert abs(pa[k] - pb[k]) <= 1, (k, pa, pb)
+
+
+@pytest.mark.parametrize("name,text", DECKS, ids=[d[0] for d in DECKS])
+def test_native_driver_matches_oracle(hip, name, text):
+    from tealeaf_amd.problem import Problem
+    dims = 3 if name.startswith("3d") else 2
+    want = oracle_run(text, dims)
+    with Problem.from_deck(text, dims) as prob:
+        steps = prob.run()
+        summary = prob.field_summary()
+    _compare(steps, want["steps"], want["settings"].solver)
+    for a, b in zip(summary, want["summary"]):
+        assert rel(a, b) < SUMMARY_RTOL, (summary, want["summary"])
+    for a, b in zip(steps, want["steps"]):
+        if b["summary_done"]:
+            assert rel(a["temp"], b["temp"]) < SUMMARY_RTOL
+
+
+def test_native_driver_known_answer_2d(hip):
+    from tealeaf_amd.problem import Problem
+    text = tdeck.layered_2d(64, 1, max_iters=5000)
+    with Problem.from_deck(text, 2) as prob:
+        prob.run()
+        temp = prob.field_summary()[3]
+    want = known_answer(prob.settings)
+    assert reference_check_passes(temp, want), (temp, want)
+
+
+def test_native_driver_known_answer_3d(hip):
+    from tealeaf_amd.problem import Problem
+    text = tdeck.cuboid_3d(25, 20)
+    with Problem.from_deck(text, 3) as prob:
+        prob.run()
+        temp = prob.field_summary()[3]
+    want = known_answer(prob.settings)
+    assert reference_check_passes(temp, want), (temp, want)
+
+
+DROPIN = [
+    ("2d", tdeck.layered_2d(64, 1, max_iters=5000)),
+    ("2d", tdeck.layered_2d(100, 2, solver=tdeck.PPCG, presteps=20, inner=10, max_iters=2000)),
+    ("2d", tdeck.layered_2d(100, 2, solver=tdeck.CHEBY, presteps=20, max_iters=2000)),
+    ("3d", tdeck.cuboid_3d(25, 3)),
+    ("3d", tdeck.cuboid_3d(20, 2, solver=tdeck.PPCG, inner=8, coefficient=2)),
+]
+
+
+@pytest.mark.parametrize("dim,text", DROPIN, ids=[f"{d}-{i}" for i, (d, _) in enumerate(DROPIN)])
+def test_reference_host_drives_hip_suite(hip, dim, text):
+    try:
+        got = ref_run(f"tealeaf{dim}_hip", text)
+        want = ref_run(f"tealeaf{dim}_omp3", text, threads=4)
+    except FileNotFoundError:
+        pytest.skip("reference host binaries not built (oracle/_ref)")
+    assert len(got["steps"]) == len(want["steps"])
+    for a, b in zip(got["steps"], want["steps"]):
+        for k, v in b["iters"].items():
+            assert abs(a["iters"][k] - v) <= 1, (k, a["iters"], b["iters"])
+    for a, b in zip(got["summary"], want["summary"]):
+        assert rel(a, b) < SUMMARY_RTOL, (got["summary"], want["summary"])
+    assert "hip_cdna4" in got["stdout"] or dim == "3d"
+    if "PASSED" in want["stdout"]:
+        assert "PASSED" in got["stdout"]
