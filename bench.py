@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: TeaLeaf's implicit heat-conduction solve on one MI355X.
+
+Workload: the reference's layered-rectangle deck (2d/tea.in geometry) at
+4096 x 4096 cells, FP64, eps 1e-15, solved three ways from identical
+initial states -- CG; Chebyshev after CG presteps (presteps 30); PPCG with
+10 inner steps (presteps 30).  A "step" is one implicit timestep solved by
+all three; the three chunks stay resident in HBM throughout.
+
+value = solver iterations per second over the timed steps, counting
+iterations as the reference host prints them (CG + Chebyshev + PPCG lines).
+The JSON line also carries the per-solver rates, the roofline of the
+dominant kernel (the fused PPCG inner sweep, timed live with HIP events on
+the suite's stream), and the reference omp3 CPU path timed on this host on a
+bounded sample of the same deck.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 4096]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
+# Algorithmic bytes per interior cell per call, FP64 (each distinct array
+# read once + each array written once; see DESIGN.md "Kernels").
+BYTES_PER_CELL = {
+    "tlhip2d_ppcg_inner_iteration": 64,   # read sd, kx, ky, r, u; write r, u, sd'
+    "tlhip2d_cg_calc_w": 32,              # read p, kx, ky; write w
+    "tlhip2d_cg_calc_ur": 48,             # read u, p, r, w; write u, r
+    "tlhip2d_cg_calc_p": 24,              # read p, r; write p
+    "tlhip2d_cheby_iterate": 64,          # read u, u0, p, kx, ky; write u', p, r
+}
+DOMINANT = "tlhip2d_ppcg_inner_iteration"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--solvers", default="cg,cheby,ppcg")
+    ap.add_argument("--cpu-iters", type=int, default=300,
+                    help="CG iterations of the bounded CPU reference sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for counter collection: no CPU baseline, no JSON extras")
+    return ap.parse_args()
+
+
+def gpu_run(args):
+    import torch
+    from tealeaf_amd import deck as tdeck
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.problem import Problem
+
+    L = lib()
+    L.tlhip_init(0)
+    names = {"cg": tdeck.CG, "cheby": tdeck.CHEBY, "ppcg": tdeck.PPCG}
+    solvers = [s for s in args.solvers.split(",") if s]
+    total_steps = args.warmup + args.steps
+    probs = {}
+    for s in solvers:
+        text = tdeck.layered_2d(args.cells, total_steps, solver=names[s], eps=1e-15,
+                                max_iters=10000, inner=10, presteps=30)
+        probs[s] = Problem.from_deck(text, 2)
+    n_int = args.cells * args.cells
+
+    for tt in range(args.warmup):
+        for s in solvers:
+            probs[s].step(tt)
+            print(f"[bench] warmup step {tt} {s} done", file=sys.stderr, flush=True)
+
+    by = {}
+    total_time = 0.0
+    total_iters = 0
+    kernel = {}
+    for s in solvers:
+        timed = DOMINANT if s == "ppcg" else ("tlhip2d_cg_calc_w" if s == "cg" else "tlhip2d_cheby_iterate")
+        L.tlhip_synchronize()
+        torch.cuda.synchronize()
+        L.tlhip_kernel_timer_start(timed.encode())
+        t0 = time.perf_counter()
+        reps = [probs[s].step(tt) for tt in range(args.warmup, total_steps)]
+        L.tlhip_synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms, nl = ctypes.c_double(), ctypes.c_long()
+        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
+        printed = sum(r["printed_cg_iters"] + r["printed_inner_iters"] for r in reps)
+        sweeps = {"cg_steps": sum(r["n_cg_steps"] for r in reps),
+                  "cheby_steps": sum(r["n_cheby_steps"] for r in reps),
+                  "ppcg_outer": sum(r["n_ppcg_outer"] for r in reps),
+                  "ppcg_inner": sum(r["n_ppcg_inner"] for r in reps)}
+        avg_ms = ms.value / max(nl.value, 1)
+        kernel[timed] = {"launches": nl.value, "avg_ms": avg_ms,
+                         "gbs": BYTES_PER_CELL[timed] * n_int / (avg_ms * 1e-3) / 1e9 if nl.value else None}
+        by[s] = {"iterations": printed, "seconds": dt, "iters_per_s": printed / dt,
+                 "per_step": [{"CG": r["printed_cg_iters"], "inner": r["printed_inner_iters"]}
+                              for r in reps], "sweeps": sweeps,
+                 "temp": probs[s].field_summary()[3]}
+        total_time += dt
+        total_iters += printed
+        print(f"[bench] {s}: {printed} iterations in {dt:.3f} s", file=sys.stderr, flush=True)
+    for p in probs.values():
+        p.close()
+    return {"by_solver": by, "kernels": kernel, "time": total_time, "iters": total_iters}
+
+
+def cpu_baseline(cells: int, iters: int):
+    """The reference omp3 CPU path (reference host + omp3 suite, built from
+    source into oracle/_ref) on a bounded sample: one CG timestep of the same
+    deck stopped after `iters` iterations."""
+    exe = ROOT / "oracle" / "_ref" / "tealeaf2d_omp3"
+    if not exe.exists():
+        return None
+    from tealeaf_amd import deck as tdeck
+    text = tdeck.layered_2d(cells, 1, solver=tdeck.CG, eps=1e-15, max_iters=iters)
+    cores = min(os.cpu_count() or 1, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as d:
+        Path(d, "tea.in").write_text(text)
+        Path(d, "tea.problems").write_text((ROOT / "tests" / "golden" / "tea_problems_2d.txt").read_text())
+        p = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, env=env, timeout=600)
+    m_it = re.search(r"CG:\s+(\d+) iterations", p.stdout)
+    m_w = re.search(r"Wallclock:\s+([0-9.]+)s", p.stdout)
+    if not (m_it and m_w):
+        return None
+    printed = int(m_it.group(1))
+    # cg_driver.c prints the loop index: max_iters when the cap ends the loop,
+    # else the index of the converging iteration.
+    n_it = iters if printed >= iters else printed + 1
+    wall = float(m_w.group(1))
+    return {"value": n_it / wall, "unit": "iters/s", "cores": cores, "kind": "reference",
+            "sample": f"reference host + omp3 suite built from source (oracle/_ref/tealeaf2d_omp3), "
+                      f"{cells}^2 layered deck, 1 CG timestep stopped at {n_it} iterations, "
+                      f"timed by the reference's own Wallclock ({wall:.2f} s), "
+                      f"OMP_NUM_THREADS={cores}"}
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel)."""
+    path = ROOT / "profiles" / "ppcg_inner_4096_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text()).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
+              "is a design, not a build (DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    res = gpu_run(args)
+    value = res["iters"] / res["time"]
+    n_int = args.cells * args.cells
+    k = res["kernels"].get(DOMINANT)
+    roofline = None
+    if k and k["launches"]:
+        achieved = k["gbs"]
+        roofline = {"kernel": DOMINANT, "bound": "hbm", "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": load_traffic(), "algorithmic_bytes": BYTES_PER_CELL[DOMINANT] * n_int,
+                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"]}
+    line = {
+        "metric": "solver iters/s + HBM GB/s, 2D 4096²; FP32-refined vs FP64 time-to-1e-10",
+        "value": value,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * res["time"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: the reference layered-rectangle deck (2d/tea.in states) generated in-process",
+        "config": {"workload": f"2D {args.cells}x{args.cells} layered-rectangle implicit solve, "
+                               "FP64, eps 1e-15, solved by CG, Chebyshev (presteps 30) and PPCG "
+                               "(10 inner, presteps 30); one step = one timestep of each",
+                   "cells": n_int, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
+        "roofline": roofline,
+        "cpu_baseline": None if (args.no_cpu or args.profile_run) else cpu_baseline(args.cells, args.cpu_iters),
+        "by_solver": res["by_solver"],
+        "kernels": res["kernels"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

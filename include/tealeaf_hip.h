@@ -78,6 +78,15 @@ int tlhip_free(void* ptr);
 int tlhip_copy_to_device(void* dst, const void* src, size_t bytes);
 int tlhip_copy_to_host(void* dst, const void* src, size_t bytes);
 int tlhip_copy_device(void* dst, const void* src, size_t bytes);
+/* Kernel timer: after start, every launch of the main kernel of the named
+ * entry point (e.g. "tlhip2d_ppcg_inner_iteration") is bracketed by HIP
+ * events on the suite's stream; read synchronises, returns the summed
+ * kernel time and the launch count, and stops timing. */
+int tlhip_kernel_timer_start(const char* entry_point);
+int tlhip_kernel_timer_read(double* total_ms, long* launches);
+/* A plain streaming copy (n doubles) on the suite's stream, timed under the
+ * name "tlhip_stream_copy": the reference point for HBM byte counters. */
+int tlhip_stream_copy(double* dst, const double* src, size_t n);
 /* Lane-shift self test: out[l] = in[l-1] (lane 0: -1), out[64+l] = in[l+1]
  * (lane 63: -1); 64 doubles in, 128 out, device pointers. */
 int tlhip_selftest_lanes(const double* in_dev, double* out_dev);

@@ -1,7 +1,8 @@
 """ctypes loader for the CPU oracle (oracle/liboracle.so) and, when built,
 the reference omp3 kernels compiled from source (oracle/_ref/libomp3_*.so).
 
-Test infrastructure only: the oracle is the checker, never the product.
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline use it, and only as the checker, never as the thing measured.
 """
 from __future__ import annotations
 
@@ -9,8 +10,8 @@ import ctypes
 import re
 from pathlib import Path
 
-ROOT = Path(__file__).resolve().parents[1]
-ORACLE_DIR = ROOT / "oracle"
+ORACLE_DIR = Path(__file__).resolve().parent
+ROOT = ORACLE_DIR.parent
 REF_DIR = ORACLE_DIR / "_ref"
 
 _SCALAR = {"int": ctypes.c_int, "double": ctypes.c_double, "bool": ctypes.c_bool,
@@ -65,3 +66,32 @@ def load_ref(dims: int):
 def ptr(a) -> int:
     """Address of a numpy array's data (or 0 for None)."""
     return 0 if a is None else a.ctypes.data
+
+
+def run_problem(settings, states, steps=None, inject=None):
+    """The oracle driver on a parsed deck: per-step reports + final summary."""
+    from tealeaf_amd import deck as tdeck
+    O = load_oracle()
+    cs, cst = tdeck.c_settings(settings), tdeck.c_states(states)
+    P = O.or_problem_create(ctypes.addressof(cs), ctypes.addressof(cst), len(states))
+    if not P:
+        raise RuntimeError("oracle problem creation failed")
+    try:
+        if inject is not None:
+            import numpy as np
+            dens, en = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in inject)
+            if not dens.size == en.size == O.or_problem_field_len(P):
+                raise ValueError("injected fields have the wrong size")
+            O.or_problem_inject(P, dens.ctypes.data, en.ctypes.data)
+        reps = []
+        for tt in range(settings.end_step if steps is None else steps):
+            rep = tdeck.CStepReport()
+            rc = O.or_problem_step(P, tt, ctypes.addressof(rep))
+            if rc:
+                raise RuntimeError(f"oracle step {tt} failed ({rc})")
+            reps.append(rep.as_dict())
+        out = (ctypes.c_double * 4)()
+        O.or_problem_field_summary(P, ctypes.addressof(out))
+        return {"steps": reps, "summary": list(out), "settings": settings}
+    finally:
+        O.or_problem_destroy(P)

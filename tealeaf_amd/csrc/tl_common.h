@@ -20,8 +20,7 @@
 
 // ---------------------------------------------------------------- runtime
 struct TlReduce {
-    double* partials;         // [max_blocks * TL_MAX_RED]
-    unsigned int* ticket;     // zero between launches (last block resets it)
+    double* partials;         // [max_blocks * TL_MAX_RED], one slot per block
     double* result;           // [TL_MAX_RED] device copy of the last result
     double* mailbox;          // pinned, host-mapped [TL_MAX_RED]
 };
@@ -38,6 +37,10 @@ int tl_check_launch(const char* what);
 // Wait for the stream and copy `n` published reduction values out.
 int tl_fetch(int n, double* out, const char* what);
 int tl_ensure_init();
+// Kernel timer hooks: begin returns a slot (or -1 when `what` is not being
+// timed); end records the closing event for that slot.
+int tl_timer_begin(const char* what);
+void tl_timer_end(int slot);
 
 #define TL_TRY(expr)                      \
     do {                                  \
@@ -93,73 +96,39 @@ __device__ __forceinline__ double tl_wave_sum(double v)
 }
 
 // ---------------------------------------------------------------- reduce
-// Block-level reduction of NV values followed by the grid finish described
-// at the top of this file.  Must be called by every thread of every block
-// (no early return before it).  `nblocks` = gridDim.x*gridDim.y*gridDim.z.
+// Block-level reduction of NV values: every block writes its partial sums
+// to slot blockIdx (linearised) of rd.partials with plain stores.  Must be
+// called by every thread of the block.  The grid total is then formed by
+// a one-block finisher kernel launched right after (tl_finish_reduce), in
+// a fixed order, so results are reproducible.  (A last-block-done ticket
+// inside the same kernel would need an agent-scope release per block, which
+// on a multi-XCD part writes back the whole L2 each time.)
 template <int NV, int BLOCK>
-__device__ __forceinline__ void tl_reduce_finish(const double* v, TlReduce rd)
+__device__ __forceinline__ void tl_reduce_partial(const double* v, TlReduce rd)
 {
     static_assert(BLOCK % TL_WAVE == 0, "block must be whole waves");
     constexpr int NW = BLOCK / TL_WAVE;
     __shared__ double s_part[NW][NV];
-    __shared__ int s_last;
     const int tid = threadIdx.x;
     const int lane = tid & (TL_WAVE - 1), wave = tid / TL_WAVE;
-    const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
     const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const double s = tl_wave_sum(v[i]);
         if (lane == 0) s_part[wave][i] = s;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < NV) {
+        double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) s += s_part[w][i];
-            __hip_atomic_store(&rd.partials[(size_t)bid * NV + i], s,
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        const unsigned t = __hip_atomic_fetch_add(rd.ticket, 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (t == nblocks - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-
-    // Last block: fixed-order sum of all partials.
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    for (unsigned b = tid; b < nblocks; b += BLOCK) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-            acc[i] += __hip_atomic_load(&rd.partials[(size_t)b * NV + i],
-                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();   // s_part reuse
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const double s = tl_wave_sum(acc[i]);
-        if (lane == 0) s_part[wave][i] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) s += s_part[w][i];
-            rd.result[i] = s;
-            __hip_atomic_store(&rd.mailbox[i], s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        __hip_atomic_store(rd.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int w = 0; w < NW; ++w) s += s_part[w][tid];
+        rd.partials[(size_t)bid * NV + tid] = s;
     }
 }
+
+// Launch the one-block finisher that sums `nblocks` partials of `nv` values
+// into rd.result and the host mailbox.
+int tl_finish_reduce(int nv, unsigned nblocks, const char* what);
 
 // ---------------------------------------------------------------- shapes
 struct TlGrid2 {

@@ -44,6 +44,14 @@ __device__ __forceinline__ double apply_a(const Nb2<V>& n, int v)
                            - (n.kyu.v[v] * n.up.v[v] + n.kyc.v[v] * n.dn.v[v]);
 }
 
+// Reads a lane needs to compute one row (besides the rows kept in registers).
+template <int V>
+struct RowIn2 {
+    DV<V> up, kyu, kxc;
+    In4<V> in;
+    double el, er, ek;   // wave-edge neighbours: a[c0-1], a[c0+V], kx[c0+V]
+};
+
 struct Tile2 {
     int ns;    // wave strips along x
     int nbx;   // blocks along x (kWaves strips each)
@@ -83,49 +91,62 @@ k_stencil2d(TlGrid2 g, Tile2 t, const double* __restrict__ a,
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
 
     if (sx < t.ns) {   // uniform across the wave
+        // Computing row j needs the stencil field at j-1, j, j+1, ky at j and
+        // j+1, kx at j, and the op's inputs at j.  The reads of row j+1 are
+        // issued before row j is computed (two rows in flight).
+        const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
+        auto load_row = [&](RowIn2<V>& rw, int j) {
+            const long row = (long)j * X;
+            rw.el = rw.er = rw.ek = 0.0;
+            if (ld_ok) {
+                rw.up = ldv<V>(a, row + X + c0);
+                rw.kyu = ldv<V>(ky, row + X + c0);
+                rw.kxc = ldv<V>(kx, row + c0);
+                rw.in = op.template load<V>(row + c0);
+                if (lane == 0) rw.el = a[row + c0 - 1];
+            } else {
+                rw.up = rw.kyu = rw.kxc = zdv<V>();
+                rw.in = In4<V>{};
+            }
+            if (edge_r) {
+                rw.er = a[row + c0 + V];
+                rw.ek = kx[row + c0 + V];
+            }
+        };
         DV<V> dn = zdv<V>(), cc = zdv<V>(), kyc = zdv<V>();
         if (ld_ok) {
             dn = ldv<V>(a, (long)(j0 - 1) * X + c0);
             cc = ldv<V>(a, (long)j0 * X + c0);
             kyc = ldv<V>(ky, (long)j0 * X + c0);
         }
+        RowIn2<V> nxt;
+        load_row(nxt, j0);
         for (int j = j0; j < j1; ++j) {
-            const long row = (long)j * X;
-            DV<V> up = zdv<V>(), kyu = zdv<V>(), kxc = zdv<V>();
-            double el = 0.0, er = 0.0, ek = 0.0;
-            if (ld_ok) {
-                up = ldv<V>(a, row + X + c0);
-                kyu = ldv<V>(ky, row + X + c0);
-                kxc = ldv<V>(kx, row + c0);
-            }
-            if (lane == 0 && ld_ok) el = a[row + c0 - 1];
-            if (lane == TL_WAVE - 1 && c0 + V < g.x) {
-                er = a[row + c0 + V];
-                ek = kx[row + c0 + V];
-            }
+            const RowIn2<V> cur = nxt;
+            if (j + 1 < j1) load_row(nxt, j + 1);
             Nb2<V> n;
             n.c = cc;
-            n.up = up;
+            n.up = cur.up;
             n.dn = dn;
-            n.kxc = kxc;
+            n.kxc = cur.kxc;
             n.kyc = kyc;
-            n.kyu = kyu;
-            const double al = tl_lane_prev(cc.v[V - 1], el);
-            const double ar = tl_lane_next(cc.v[0], er);
-            const double kr = tl_lane_next(kxc.v[0], ek);
+            n.kyu = cur.kyu;
+            const double al = tl_lane_prev(cc.v[V - 1], cur.el);
+            const double ar = tl_lane_next(cc.v[0], cur.er);
+            const double kr = tl_lane_next(cur.kxc.v[0], cur.ek);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 n.l.v[v] = v == 0 ? al : cc.v[v - 1];
                 n.r.v[v] = v == V - 1 ? ar : cc.v[v + 1];
-                n.kxr.v[v] = v == V - 1 ? kr : kxc.v[v + 1];
+                n.kxr.v[v] = v == V - 1 ? kr : cur.kxc.v[v + 1];
             }
-            if (nval > 0) op.template cells<V>(row + c0, nval, n, acc);
+            if (nval > 0) op.template apply<V>((long)j * X + c0, nval, n, cur.in, acc);
             dn = cc;
-            cc = up;
-            kyc = kyu;
+            cc = cur.up;
+            kyc = cur.kyu;
         }
     }
-    if constexpr (Op::NRED > 0) tl_reduce_finish<Op::NRED, kBlock>(acc, rd);
+    if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
 }
 
 // Jacobi update from the copy r of u (jacobi.c:61-100); the stencil field
@@ -134,10 +155,17 @@ struct OpJacobi2 {
     static constexpr int NRED = 1;
     const double* __restrict__ u0;
     double* __restrict__ u;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> b = ldv<V>(u0, i);
+        In4<V> in{};
+        in.a = ldv<V>(u0, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double* acc) const
+    {
+        const DV<V>& b = in.a;
         DV<V> uo;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -155,7 +183,9 @@ struct OpJacobi2 {
 
 // cg_init passes 1-3 in one sweep over the whole array (cg.c:31-66):
 // p = r = 0, u = e*rho everywhere; w = rho or 1/rho on [1,n-1); kx, ky on
-// [hd,n-1) from the same w values recomputed from rho.
+// [hd,n-1) from the same w values recomputed from rho.  A neighbour outside
+// [1,n-1) (only with halo_depth 1) is not written by this pass, so its w is
+// read from memory as the reference reads it.
 __global__ void __launch_bounds__(kBlock)
 k_cg_init_fields(TlGrid2 g, int coef, double rx, double ry, const double* __restrict__ density,
                  const double* __restrict__ energy, double* __restrict__ u, double* __restrict__ p,
@@ -173,8 +203,10 @@ k_cg_init_fields(TlGrid2 g, int coef, double rx, double ry, const double* __rest
             const double wc = coef == 1 ? rho : 1.0 / rho;
             w[c] = wc;
             if (jj >= g.hd && kk >= g.hd) {
-                const double wl = coef == 1 ? density[c - 1] : 1.0 / density[c - 1];
-                const double wd = coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x];
+                const double wl = kk >= 2 ? (coef == 1 ? density[c - 1] : 1.0 / density[c - 1])
+                                          : w[c - 1];
+                const double wd = jj >= 2 ? (coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x])
+                                          : w[c - g.x];
                 kx[c] = rx * (wl + wc) / (2.0 * wl * wc);
                 ky[c] = ry * (wd + wc) / (2.0 * wd * wc);
             }
@@ -391,15 +423,21 @@ int launch_stencil(const TlGrid2& g, int V, const double* a, const double* kx,
                    const double* ky, const Op& op, const char* what)
 {
     if (g.x - 2 * g.hd <= 0 || g.y - 2 * g.hd <= 0) return 0;
+    if (tl_cdiv(tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V), kWaves) > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: rows too long (%d)", what, g.x);
     const Tile2 t = make_tile(g, V);
     const unsigned nb = (unsigned)(t.nbx * t.nby);
+    const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_stencil2d<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
                            kx, ky, op, tl_reduce_slots());
     else
         hipLaunchKernelGGL((k_stencil2d<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
                            kx, ky, op, tl_reduce_slots());
-    return tl_check_launch(what);
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
+    return 0;
 }
 
 template <class Op>

@@ -39,6 +39,14 @@ __device__ __forceinline__ double apply_a(const Nb3<V>& n, int v)
                            - (n.kzu.v[v] * n.up.v[v] + n.kzc.v[v] * n.dn.v[v]);
 }
 
+// Reads a lane needs to compute one plane besides the planes in registers.
+template <int V>
+struct RowIn3 {
+    DV<V> up, kzu, kxc, kyc, kyn, no, so;
+    In4<V> in;
+    double el, er, ek;   // wave-edge neighbours: a[c0-1], a[c0+V], kx[c0+V]
+};
+
 struct Tile3 {
     int ns;    // strips along x
     int nbj;   // blocks along y (kWaves rows each)
@@ -77,59 +85,71 @@ k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __re
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
 
     if (j < g.y - g.hd) {   // uniform across the wave
+        // Plane k needs the field at k-1, k, k+1 and at rows j-1, j+1 of
+        // plane k, kz at k and k+1, kx and ky at k (ky also at row j+1), and
+        // the op's inputs; plane k+1's reads are issued before k is computed.
         const long jrow = (long)j * X + c0;
+        const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
+        auto load_plane = [&](RowIn3<V>& rw, int k) {
+            const long b = (long)k * XY + jrow;
+            rw.el = rw.er = rw.ek = 0.0;
+            if (ld_ok) {
+                rw.up = ldv<V>(a, b + XY);
+                rw.kzu = ldv<V>(kz, b + XY);
+                rw.kxc = ldv<V>(kx, b);
+                rw.kyc = ldv<V>(ky, b);
+                rw.kyn = ldv<V>(ky, b + X);
+                rw.no = ldv<V>(a, b + X);
+                rw.so = ldv<V>(a, b - X);
+                rw.in = op.template load<V>(b);
+                if (lane == 0) rw.el = a[b - 1];
+            } else {
+                rw.up = rw.kzu = rw.kxc = rw.kyc = rw.kyn = rw.no = rw.so = zdv<V>();
+                rw.in = In4<V>{};
+            }
+            if (edge_r) {
+                rw.er = a[b + V];
+                rw.ek = kx[b + V];
+            }
+        };
         DV<V> dn = zdv<V>(), cc = zdv<V>(), kzc = zdv<V>();
         if (ld_ok) {
             dn = ldv<V>(a, (long)(k0 - 1) * XY + jrow);
             cc = ldv<V>(a, (long)k0 * XY + jrow);
             kzc = ldv<V>(kz, (long)k0 * XY + jrow);
         }
+        RowIn3<V> nxt;
+        load_plane(nxt, k0);
         for (int k = k0; k < k1; ++k) {
-            const long b = (long)k * XY + jrow;       // this lane's first cell
-            DV<V> up = zdv<V>(), kzu = zdv<V>(), kxc = zdv<V>(), kyc = zdv<V>(), kyn = zdv<V>();
-            DV<V> no = zdv<V>(), so = zdv<V>();
-            double el = 0.0, er = 0.0, ek = 0.0;
-            if (ld_ok) {
-                up = ldv<V>(a, b + XY);
-                kzu = ldv<V>(kz, b + XY);
-                kxc = ldv<V>(kx, b);
-                kyc = ldv<V>(ky, b);
-                kyn = ldv<V>(ky, b + X);
-                no = ldv<V>(a, b + X);
-                so = ldv<V>(a, b - X);
-            }
-            if (lane == 0 && ld_ok) el = a[b - 1];
-            if (lane == TL_WAVE - 1 && c0 + V < g.x) {
-                er = a[b + V];
-                ek = kx[b + V];
-            }
+            const RowIn3<V> cur = nxt;
+            if (k + 1 < k1) load_plane(nxt, k + 1);
             Nb3<V> n;
             n.c = cc;
-            n.up = up;
+            n.up = cur.up;
             n.dn = dn;
-            n.no = no;
-            n.so = so;
-            n.kxc = kxc;
-            n.kyc = kyc;
-            n.kyn = kyn;
+            n.no = cur.no;
+            n.so = cur.so;
+            n.kxc = cur.kxc;
+            n.kyc = cur.kyc;
+            n.kyn = cur.kyn;
             n.kzc = kzc;
-            n.kzu = kzu;
-            const double al = tl_lane_prev(cc.v[V - 1], el);
-            const double ar = tl_lane_next(cc.v[0], er);
-            const double kr = tl_lane_next(kxc.v[0], ek);
+            n.kzu = cur.kzu;
+            const double al = tl_lane_prev(cc.v[V - 1], cur.el);
+            const double ar = tl_lane_next(cc.v[0], cur.er);
+            const double kr = tl_lane_next(cur.kxc.v[0], cur.ek);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 n.l.v[v] = v == 0 ? al : cc.v[v - 1];
                 n.r.v[v] = v == V - 1 ? ar : cc.v[v + 1];
-                n.kxr.v[v] = v == V - 1 ? kr : kxc.v[v + 1];
+                n.kxr.v[v] = v == V - 1 ? kr : cur.kxc.v[v + 1];
             }
-            if (nval > 0) op.template cells<V>(b, nval, n, acc);
+            if (nval > 0) op.template apply<V>((long)k * XY + jrow, nval, n, cur.in, acc);
             dn = cc;
-            cc = up;
-            kzc = kzu;
+            cc = cur.up;
+            kzc = cur.kzu;
         }
     }
-    if constexpr (Op::NRED > 0) tl_reduce_finish<Op::NRED, kBlock>(acc, rd);
+    if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
 }
 
 // 3D Jacobi update (jacobi_solver.c:72-116).
@@ -137,10 +157,17 @@ struct OpJacobi3 {
     static constexpr int NRED = 1;
     const double* __restrict__ u0;
     double* __restrict__ u;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> b = ldv<V>(u0, i);
+        In4<V> in{};
+        in.a = ldv<V>(u0, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double* acc) const
+    {
+        const DV<V>& b = in.a;
         DV<V> uo;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -187,9 +214,14 @@ k_cg_init_fields3(TlGrid3 g, int coef, double rx, double ry, double rz,
             const double wc = coef == 1 ? rho : 1.0 / rho;
             w[c] = wc;
             if (ii >= g.hd && jj >= g.hd && kk >= g.hd) {
-                const double wl = coef == 1 ? density[c - 1] : 1.0 / density[c - 1];
-                const double wd = coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x];
-                const double wb = coef == 1 ? density[c - xy] : 1.0 / density[c - xy];
+                // A neighbour outside [1,n-1) (halo_depth 1 only) keeps the w
+                // in memory, as the reference reads it.
+                const double wl = kk >= 2 ? (coef == 1 ? density[c - 1] : 1.0 / density[c - 1])
+                                          : w[c - 1];
+                const double wd = jj >= 2 ? (coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x])
+                                          : w[c - g.x];
+                const double wb = ii >= 2 ? (coef == 1 ? density[c - xy] : 1.0 / density[c - xy])
+                                          : w[c - xy];
                 kx[c] = rx * (wl + wc) / (2.0 * wl * wc);
                 ky[c] = ry * (wd + wc) / (2.0 * wd * wc);
                 kz[c] = rz * (wb + wc) / (2.0 * wb * wc);
@@ -449,15 +481,22 @@ int launch_stencil(const TlGrid3& g, int V, const double* a, const double* kx, c
                    const double* kz, const Op& op, const char* what)
 {
     if (interior_empty(g)) return 0;
+    if ((long)tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V) * tl_cdiv(g.y - 2 * g.hd, kWaves) >
+        TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: planes too large (%d x %d)", what, g.x, g.y);
     const Tile3 t = make_tile(g, V);
     const unsigned nb = (unsigned)(t.ns * t.nbj * t.nbk);
+    const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_stencil3d<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
                            kx, ky, kz, op, tl_reduce_slots());
     else
         hipLaunchKernelGGL((k_stencil3d<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
                            kx, ky, kz, op, tl_reduce_slots());
-    return tl_check_launch(what);
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
+    return 0;
 }
 
 template <class Op>

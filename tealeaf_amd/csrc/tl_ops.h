@@ -1,9 +1,13 @@
 // Per-cell operations shared by the 2D and 3D kernels.
 //
-// Vector helpers move V consecutive doubles of one row (V=2: one 16-byte
-// access).  Pointwise ops act on interior rows through k_interior; stencil
-// ops receive a neighbourhood `N` (Nb2 or Nb3) and call apply_a(n, v),
-// which each dimension defines with the reference's association.
+// Every op is split in two phases so the kernels can software-pipeline:
+//   load<V>(i)            issues the op's own reads for the V cells at i;
+//   apply<V>(i, nval, ...) computes and stores (only the first nval cells).
+// A kernel issues the loads of its next row (or segment) before it applies
+// the current one, which keeps two rows of reads in flight per wavefront.
+// Pointwise ops act on interior rows through k_interior; stencil ops also
+// receive a neighbourhood `N` (Nb2 or Nb3) and call apply_a(n, v), which each
+// dimension defines with the reference's association.
 #pragma once
 
 #include "tl_common.h"
@@ -54,6 +58,12 @@ __device__ __forceinline__ DV<V> zdv()
     return r;
 }
 
+// Up to four op-private input vectors.
+template <int V>
+struct In4 {
+    DV<V> a, b, c, d;
+};
+
 // Interior rows of a 2D or 3D chunk.  Row r (0-based over the interior) is
 // at offset (hd + r % ny)*x + (zoff + r / ny)*xy; 2D uses zoff = xy = 0.
 struct TlRows {
@@ -63,8 +73,24 @@ struct TlRows {
     long nrows;
 };
 
-// Pointwise work over interior rows, block-strided row segments of
-// kBlock*V columns.
+struct Seg {
+    long i;     // index of the lane's first cell
+    int nval;   // interior cells it owns (0..V)
+};
+
+template <int V>
+__device__ __forceinline__ Seg seg_of(const TlRows& g, int segs_per_row, long s)
+{
+    const long r = s / segs_per_row;
+    const long base = (long)(g.hd + r % g.ny) * g.x + (long)(g.zoff + r / g.ny) * g.xy;
+    const int c0 = g.hd + (int)(s % segs_per_row) * (kBlock * V) + threadIdx.x * V;
+    int nval = g.x - g.hd - c0;
+    nval = nval < 0 ? 0 : (nval > V ? V : nval);
+    return Seg{base + c0, nval};
+}
+
+// Pointwise work over interior rows: block-strided row segments of kBlock*V
+// columns, two segments in flight per thread.
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
 k_interior(TlRows g, int segs_per_row, long nseg, Op op, TlReduce rd)
@@ -73,16 +99,18 @@ k_interior(TlRows g, int segs_per_row, long nseg, Op op, TlReduce rd)
     double acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
-    const int xend = g.x - g.hd;
-    for (long s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const long r = s / segs_per_row;
-        const long base = (long)(g.hd + r % g.ny) * g.x + (long)(g.zoff + r / g.ny) * g.xy;
-        const int c0 = g.hd + (int)(s % segs_per_row) * (kBlock * V) + threadIdx.x * V;
-        int nval = xend - c0;
-        nval = nval < 0 ? 0 : (nval > V ? V : nval);
-        if (nval > 0) op.template cells<V>(base + c0, nval, acc);
+    const long stride = gridDim.x;
+    for (long s = blockIdx.x; s < nseg; s += 2 * stride) {
+        const Seg a = seg_of<V>(g, segs_per_row, s);
+        const bool has_b = s + stride < nseg;
+        const Seg b = has_b ? seg_of<V>(g, segs_per_row, s + stride) : Seg{0, 0};
+        In4<V> ia{}, ib{};
+        if (a.nval > 0) ia = op.template load<V>(a.i);
+        if (b.nval > 0) ib = op.template load<V>(b.i);
+        if (a.nval > 0) op.template apply<V>(a.i, a.nval, ia, acc);
+        if (b.nval > 0) op.template apply<V>(b.i, b.nval, ib, acc);
     }
-    if constexpr (Op::NRED > 0) tl_reduce_finish<Op::NRED, kBlock>(acc, rd);
+    if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
 }
 
 // ------------------------------------------------------------ pointwise ops
@@ -95,13 +123,18 @@ struct OpCalcUr {   // u += alpha p ; r -= alpha w ; rrn += r.r  (2d/c_kernels/o
     const double* __restrict__ w;
     double alpha;
     template <int V>
-    __device__ void cells(long i, int nval, double* acc) const
+    __device__ In4<V> load(long i) const
     {
-        DV<V> uu = ldv<V>(u, i), pp = ldv<V>(p, i), rr = ldv<V>(r, i), ww = ldv<V>(w, i);
+        return In4<V>{ldv<V>(u, i), ldv<V>(p, i), ldv<V>(r, i), ldv<V>(w, i)};
+    }
+    template <int V>
+    __device__ void apply(long i, int nval, const In4<V>& in, double* acc) const
+    {
+        DV<V> uu = in.a, rr = in.c;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            uu.v[v] += alpha * pp.v[v];
-            rr.v[v] -= alpha * ww.v[v];
+            uu.v[v] += alpha * in.b.v[v];
+            rr.v[v] -= alpha * in.d.v[v];
             if (v < nval) acc[0] += rr.v[v] * rr.v[v];
         }
         stv<V>(u, i, uu, nval);
@@ -115,12 +148,19 @@ struct OpCalcP {    // p = beta p + r  (2d/c_kernels/omp3/cg.c:146-164)
     const double* __restrict__ r;
     double beta;
     template <int V>
-    __device__ void cells(long i, int nval, double*) const
+    __device__ In4<V> load(long i) const
     {
-        DV<V> pp = ldv<V>(p, i);
-        const DV<V> rr = ldv<V>(r, i);
+        In4<V> in{};
+        in.a = ldv<V>(p, i);
+        in.b = ldv<V>(r, i);
+        return in;
+    }
+    template <int V>
+    __device__ void apply(long i, int nval, const In4<V>& in, double*) const
+    {
+        DV<V> pp;
 #pragma unroll
-        for (int v = 0; v < V; ++v) pp.v[v] = beta * pp.v[v] + rr.v[v];
+        for (int v = 0; v < V; ++v) pp.v[v] = beta * in.a.v[v] + in.b.v[v];
         stv<V>(p, i, pp, nval);
     }
 };
@@ -138,18 +178,24 @@ struct OpSimple {
     const double* __restrict__ other;
     double s;
     template <int V>
-    __device__ void cells(long i, int nval, double*) const
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> a = ldv<V>(src, i);
-        DV<V> o = zdv<V>(), d = zdv<V>();
-        if (mode == 1) o = ldv<V>(dst, i);
-        if (mode == 3) d = ldv<V>(other, i);
+        In4<V> in{};
+        in.a = ldv<V>(src, i);
+        if (mode == 1) in.b = ldv<V>(dst, i);
+        if (mode == 3) in.c = ldv<V>(other, i);
+        return in;
+    }
+    template <int V>
+    __device__ void apply(long i, int nval, const In4<V>& in, double*) const
+    {
+        DV<V> o;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            if (mode == 0) o.v[v] = a.v[v];
-            else if (mode == 1) o.v[v] = o.v[v] + a.v[v];
-            else if (mode == 2) o.v[v] = a.v[v] / s;
-            else o.v[v] = a.v[v] / d.v[v];
+            if (mode == 0) o.v[v] = in.a.v[v];
+            else if (mode == 1) o.v[v] = in.b.v[v] + in.a.v[v];
+            else if (mode == 2) o.v[v] = in.a.v[v] / s;
+            else o.v[v] = in.a.v[v] / in.c.v[v];
         }
         stv<V>(dst, i, o, nval);
     }
@@ -159,12 +205,18 @@ struct OpNorm {     // norm += b.b  (solver_methods.c:51-71)
     static constexpr int NRED = 1;
     const double* __restrict__ b;
     template <int V>
-    __device__ void cells(long i, int nval, double* acc) const
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> bb = ldv<V>(b, i);
+        In4<V> in{};
+        in.a = ldv<V>(b, i);
+        return in;
+    }
+    template <int V>
+    __device__ void apply(long, int nval, const In4<V>& in, double* acc) const
+    {
 #pragma unroll
         for (int v = 0; v < V; ++v)
-            if (v < nval) acc[0] += bb.v[v] * bb.v[v];
+            if (v < nval) acc[0] += in.a.v[v] * in.a.v[v];
     }
 };
 
@@ -175,18 +227,21 @@ struct OpFieldSummary {   // field_summary.c:9-45
     const double* __restrict__ energy0;
     const double* __restrict__ u;
     template <int V>
-    __device__ void cells(long i, int nval, double* acc) const
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> vv = ldv<V>(volume, i), dd = ldv<V>(density, i);
-        const DV<V> ee = ldv<V>(energy0, i), uu = ldv<V>(u, i);
+        return In4<V>{ldv<V>(volume, i), ldv<V>(density, i), ldv<V>(energy0, i), ldv<V>(u, i)};
+    }
+    template <int V>
+    __device__ void apply(long, int nval, const In4<V>& in, double* acc) const
+    {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             if (v < nval) {
-                const double cm = vv.v[v] * dd.v[v];
-                acc[0] += vv.v[v];
+                const double cm = in.a.v[v] * in.b.v[v];
+                acc[0] += in.a.v[v];
                 acc[1] += cm;
-                acc[2] += cm * ee.v[v];
-                acc[3] += cm * uu.v[v];
+                acc[2] += cm * in.c.v[v];
+                acc[3] += cm * in.d.v[v];
             }
         }
     }
@@ -198,8 +253,13 @@ struct OpFieldSummary {   // field_summary.c:9-45
 struct OpCalcW {   // w = A p ; pw += w.p  (cg.c:88-113)
     static constexpr int NRED = 1;
     double* __restrict__ w;
+    template <int V>
+    __device__ In4<V> load(long) const
+    {
+        return In4<V>{};
+    }
     template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    __device__ void apply(long i, int nval, const N& n, const In4<V>&, double* acc) const
     {
         DV<V> o;
 #pragma unroll
@@ -216,8 +276,13 @@ struct OpCgInit {  // w = A u ; r = u - w ; p = r ; rro += r.p  (cg.c:71-84)
     double* __restrict__ w;
     double* __restrict__ r;
     double* __restrict__ p;
+    template <int V>
+    __device__ In4<V> load(long) const
+    {
+        return In4<V>{};
+    }
     template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double* acc) const
+    __device__ void apply(long i, int nval, const N& n, const In4<V>&, double* acc) const
     {
         DV<V> wo, ro;
 #pragma unroll
@@ -239,15 +304,21 @@ struct OpChebyInit {   // w = A u ; r = u0 - w ; p = r/theta  (cheby.c:27-54)
     double* __restrict__ r;
     double* __restrict__ p;
     double theta;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double*) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> b = ldv<V>(u0, i);
+        In4<V> in{};
+        in.a = ldv<V>(u0, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double*) const
+    {
         DV<V> wo, ro, po;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             wo.v[v] = apply_a(n, v);
-            ro.v[v] = b.v[v] - wo.v[v];
+            ro.v[v] = in.a.v[v] - wo.v[v];
             po.v[v] = ro.v[v] / theta;
         }
         stv<V>(w, i, wo, nval);
@@ -266,17 +337,23 @@ struct OpChebyIter {
     double* __restrict__ w;
     double* __restrict__ u_out;
     double alpha, beta;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double*) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> b = ldv<V>(u0, i);
-        const DV<V> pi = ldv<V>(p, i);
+        In4<V> in{};
+        in.a = ldv<V>(u0, i);
+        in.b = ldv<V>(p, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double*) const
+    {
         DV<V> wo, ro, po, uo;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             wo.v[v] = apply_a(n, v);
-            ro.v[v] = b.v[v] - wo.v[v];
-            po.v[v] = alpha * pi.v[v] + beta * ro.v[v];
+            ro.v[v] = in.a.v[v] - wo.v[v];
+            po.v[v] = alpha * in.b.v[v] + beta * ro.v[v];
             uo.v[v] = n.c.v[v] + po.v[v];
         }
         if (w) stv<V>(w, i, wo, nval);
@@ -294,16 +371,22 @@ struct OpPpcgInner {
     double* __restrict__ u;
     double* __restrict__ sd_out;
     double alpha, beta;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double*) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> ri = ldv<V>(r, i);
-        const DV<V> ui = ldv<V>(u, i);
+        In4<V> in{};
+        in.a = ldv<V>(r, i);
+        in.b = ldv<V>(u, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double*) const
+    {
         DV<V> ro, uo, so;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            ro.v[v] = ri.v[v] - apply_a(n, v);
-            uo.v[v] = ui.v[v] + n.c.v[v];
+            ro.v[v] = in.a.v[v] - apply_a(n, v);
+            uo.v[v] = in.b.v[v] + n.c.v[v];
             so.v[v] = alpha * n.c.v[v] + beta * ro.v[v];
         }
         stv<V>(r, i, ro, nval);
@@ -316,13 +399,19 @@ struct OpResidual {   // r = u0 - A u  (solver_methods.c:28-48)
     static constexpr int NRED = 0;
     const double* __restrict__ u0;
     double* __restrict__ r;
-    template <int V, class N>
-    __device__ void cells(long i, int nval, const N& n, double*) const
+    template <int V>
+    __device__ In4<V> load(long i) const
     {
-        const DV<V> b = ldv<V>(u0, i);
+        In4<V> in{};
+        in.a = ldv<V>(u0, i);
+        return in;
+    }
+    template <int V, class N>
+    __device__ void apply(long i, int nval, const N& n, const In4<V>& in, double*) const
+    {
         DV<V> ro;
 #pragma unroll
-        for (int v = 0; v < V; ++v) ro.v[v] = b.v[v] - apply_a(n, v);
+        for (int v = 0; v < V; ++v) ro.v[v] = in.a.v[v] - apply_a(n, v);
         stv<V>(r, i, ro, nval);
     }
 };
@@ -355,14 +444,20 @@ static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const c
     if (nx <= 0 || g.nrows <= 0) return 0;
     const int spr = (int)tl_cdiv(nx, kBlock * V);
     const long nseg = (long)spr * g.nrows;
-    const unsigned nb = (unsigned)(nseg < 4096 ? nseg : 4096);
+    // ~16 waves per CU, each thread owning two segments at a time.
+    const long want = (nseg + 1) / 2;
+    const unsigned nb = (unsigned)(want < 4096 ? want : 4096);
+    const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_interior<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
                            nseg, op, tl_reduce_slots());
     else
         hipLaunchKernelGGL((k_interior<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
                            nseg, op, tl_reduce_slots());
-    return tl_check_launch(what);
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
+    return 0;
 }
 
 // Add the published reduction value to *dst (the reference's += contract).

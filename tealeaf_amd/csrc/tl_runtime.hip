@@ -12,6 +12,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 namespace {
 
 struct Runtime {
@@ -24,7 +26,17 @@ struct Runtime {
     size_t staging_len = 0;
 };
 
+// Event pairs around every launch of one entry point's main kernel, for
+// measuring average kernel duration on the suite's own stream.
+struct KernelTimer {
+    char name[128] = "";
+    int on = 0;
+    std::vector<hipEvent_t> ev;     // 2 per timed launch
+    size_t used = 0;                // launches recorded
+};
+
 Runtime g_rt;
+KernelTimer g_timer;
 thread_local char g_err[512] = "";
 
 }  // namespace
@@ -67,8 +79,6 @@ extern "C" int tlhip_init(int device)
     g_rt.own_stream = 1;
     HIP_TRY(hipMalloc(&g_rt.red.partials, sizeof(double) * TL_MAX_RED * TL_MAX_RED_BLOCKS),
             "hipMalloc(partials)");
-    HIP_TRY(hipMalloc(&g_rt.red.ticket, 256), "hipMalloc(ticket)");
-    HIP_TRY(hipMemset(g_rt.red.ticket, 0, 256), "hipMemset(ticket)");
     HIP_TRY(hipMalloc(&g_rt.red.result, sizeof(double) * TL_MAX_RED), "hipMalloc(result)");
     HIP_TRY(hipHostMalloc(&g_rt.red.mailbox, 256, hipHostMallocMapped | hipHostMallocCoherent),
             "hipHostMalloc(mailbox)");
@@ -128,7 +138,7 @@ extern "C" int tlhip_finalise(void)
     if (!g_rt.initialised) return 0;
     // Teardown: report the first failure but release everything regardless.
     hipError_t first = hipStreamSynchronize(g_rt.stream);
-    hipError_t e[6] = {hipFree(g_rt.red.partials), hipFree(g_rt.red.ticket),
+    hipError_t e[6] = {hipFree(g_rt.red.partials), hipSuccess,
                        hipFree(g_rt.red.result), hipHostFree(g_rt.red.mailbox),
                        g_rt.staging ? hipFree(g_rt.staging) : hipSuccess,
                        g_rt.own_stream ? hipStreamDestroy(g_rt.stream) : hipSuccess};
@@ -187,6 +197,68 @@ extern "C" int tlhip_copy_device(void* dst, const void* src, size_t bytes)
     return 0;
 }
 
+// ------------------------------------------------------------ kernel timer
+int tl_timer_begin(const char* what)
+{
+    if (!g_timer.on || strcmp(what, g_timer.name) != 0) return -1;
+    const size_t k = g_timer.used;
+    while (g_timer.ev.size() < 2 * (k + 1)) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        g_timer.ev.push_back(e);
+    }
+    if (hipEventRecord(g_timer.ev[2 * k], g_rt.stream) != hipSuccess) return -1;
+    return (int)k;
+}
+
+void tl_timer_end(int k)
+{
+    if (k < 0) return;
+    if (hipEventRecord(g_timer.ev[2 * k + 1], g_rt.stream) == hipSuccess) g_timer.used = k + 1;
+}
+
+extern "C" int tlhip_kernel_timer_start(const char* entry_point)
+{
+    TL_TRY(tl_ensure_init());
+    if (!entry_point || strlen(entry_point) >= sizeof(g_timer.name))
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_start: bad name");
+    strcpy(g_timer.name, entry_point);
+    g_timer.on = 1;
+    g_timer.used = 0;
+    return 0;
+}
+
+extern "C" int tlhip_kernel_timer_read(double* total_ms, long* launches)
+{
+    TL_TRY(tl_ensure_init());
+    if (!total_ms || !launches) return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_read: null argument");
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "timer sync");
+    double sum = 0.0;
+    for (size_t k = 0; k < g_timer.used; ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, g_timer.ev[2 * k], g_timer.ev[2 * k + 1]), "hipEventElapsedTime");
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (long)g_timer.used;
+    g_timer.on = 0;
+    g_timer.used = 0;
+    return 0;
+}
+
+// A plain streaming copy kernel on the suite's stream: the reference point
+// for scaling HBM byte counters (read n*8 bytes, write n*8 bytes).
+extern "C" int tlhip_stream_copy(double* dst, const double* src, size_t n)
+{
+    TL_TRY(tl_ensure_init());
+    if (!dst || !src) return tl_fail(TLHIP_ERR_ARG, "tlhip_stream_copy: null argument");
+    const int t = tl_timer_begin("tlhip_stream_copy");
+    hipLaunchKernelGGL(tl::k_copy_all, dim3(tl::flat_blocks((long)n)), dim3(tl::kBlock), 0,
+                       g_rt.stream, (long)n, src, dst);
+    tl_timer_end(t);
+    return tl_check_launch("tlhip_stream_copy");
+}
+
 // Device staging area for halo pack/unpack (grown on demand, never shrunk).
 int tl_staging(size_t doubles, double** out)
 {
@@ -201,6 +273,50 @@ int tl_staging(size_t doubles, double** out)
     }
     *out = g_rt.staging;
     return 0;
+}
+
+// Grid total of the per-block partials, summed in a fixed order: thread t
+// takes slots t, t+256, ... in turn, then a fixed tree over the block.
+template <int NV>
+__global__ void __launch_bounds__(256) k_finish_reduce(unsigned nblocks, TlReduce rd)
+{
+    constexpr int NW = 256 / TL_WAVE;
+    __shared__ double s_part[NW][NV];
+    const int tid = threadIdx.x, lane = tid & (TL_WAVE - 1), wave = tid / TL_WAVE;
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (unsigned b = tid; b < nblocks; b += 256) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += rd.partials[(size_t)b * NV + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double s = tl_wave_sum(acc[i]);
+        if (lane == 0) s_part[wave][i] = s;
+    }
+    __syncthreads();
+    if (tid < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s += s_part[w][tid];
+        rd.result[tid] = s;
+        __hip_atomic_store(&rd.mailbox[tid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+int tl_finish_reduce(int nv, unsigned nblocks, const char* what)
+{
+    if (nblocks > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
+                       TL_MAX_RED_BLOCKS);
+    if (nv == 1)
+        hipLaunchKernelGGL(k_finish_reduce<1>, dim3(1), dim3(256), 0, g_rt.stream, nblocks, g_rt.red);
+    else if (nv == 4)
+        hipLaunchKernelGGL(k_finish_reduce<4>, dim3(1), dim3(256), 0, g_rt.stream, nblocks, g_rt.red);
+    else
+        return tl_fail(TLHIP_ERR_ARG, "%s: unsupported reduction width %d", what, nv);
+    return tl_check_launch(what);
 }
 
 namespace tl {

@@ -35,5 +35,5 @@ def hip():
 
 @pytest.fixture(scope="session")
 def oracle():
-    from tests.oracle_lib import load_oracle
+    from oracle.pyoracle import load_oracle
     return load_oracle()

@@ -11,7 +11,7 @@ Run from the repository root:  python -m tests.golden.make_golden
 import numpy as np
 
 from tests import kernel_cases as kc
-from tests.oracle_lib import load_ref
+from oracle.pyoracle import load_ref
 
 
 def main():

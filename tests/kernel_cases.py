@@ -249,7 +249,7 @@ def make_cases(dims: int, nx: int, ny: int, nz: int = 1, hd: int = 2, seed: int 
     xi, yi, zi = nx, ny, nz
     for face in range(nfaces):
         for pack in (True, False):
-            depth = 2 if face % 2 else 1
+            depth = min(2, hd) if face % 2 else 1    # the reference needs depth <= halo_depth
             if face < 2:
                 blen = depth * yi * (zi if d3 else 1)
             elif face < 4:

@@ -42,29 +42,9 @@ def reference_check_passes(temp: float, expected: float) -> bool:
 
 
 def oracle_run(text: str, dims: int | None = None, inject=None) -> dict:
-    from tests.oracle_lib import load_oracle
-    O = load_oracle()
+    from oracle.pyoracle import run_problem
     s, states = tdeck.parse_deck(text, dims or (3 if "z_cells" in text else 2))
-    cs, cst = tdeck.c_settings(s), tdeck.c_states(states)
-    P = O.or_problem_create(ctypes.addressof(cs), ctypes.addressof(cst), len(states))
-    assert P, "oracle problem creation failed"
-    try:
-        if inject is not None:
-            import numpy as np
-            dens, en = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in inject)
-            assert dens.size == en.size == O.or_problem_field_len(P)
-            O.or_problem_inject(P, dens.ctypes.data, en.ctypes.data)
-        steps = []
-        for tt in range(s.end_step):
-            rep = tdeck.CStepReport()
-            rc = O.or_problem_step(P, tt, ctypes.addressof(rep))
-            assert rc == 0, f"oracle step {tt} failed ({rc})"
-            steps.append(rep.as_dict())
-        out = (ctypes.c_double * 4)()
-        O.or_problem_field_summary(P, out)
-        return {"steps": steps, "summary": list(out), "settings": s}
-    finally:
-        O.or_problem_destroy(P)
+    return run_problem(s, states, inject=inject)
 
 
 _ITER = re.compile(r"^(CG|Cheby|PPCG|Jacobi):\s+(\d+) iterations", re.M)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from tests import kernel_cases as kc
-from tests.oracle_lib import load_oracle
+from oracle.pyoracle import load_oracle
 
 pytestmark = pytest.mark.gpu
 

@@ -17,12 +17,14 @@ import pytest
 
 from tealeaf_amd import deck as tdeck
 from tests import kernel_cases as kc
-from tests.oracle_lib import load_oracle, load_ref
+from oracle.pyoracle import load_oracle, load_ref
 from tests.problem_helpers import (known_answer, oracle_run, printed_iters, ref_run,
                                    reference_check_passes, rel)
 
 CASES = (kc.make_cases(2, 10, 7, seed=11) + kc.make_cases(2, 9, 6, seed=12, coefficient=2)
+         + kc.make_cases(2, 11, 6, hd=1, seed=15) + kc.make_cases(2, 8, 9, hd=3, seed=16)
          + kc.make_cases(3, 6, 5, 4, seed=13) + kc.make_cases(3, 5, 6, 3, seed=14, coefficient=2)
+         + kc.make_cases(3, 5, 4, 6, hd=1, seed=17)
          + [kc.state_case(2, 12, 10), kc.state_case(3, 8, 7, 6)])
 
 
