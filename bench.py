@@ -10,7 +10,7 @@ all three; the three chunks stay resident in HBM throughout.
 value = solver iterations per second over the timed steps, counting
 iterations as the reference host prints them (CG + Chebyshev + PPCG lines).
 The JSON line also carries the per-solver rates, the roofline of the
-dominant kernel (the fused PPCG inner sweep, timed live with HIP events on
+dominant kernel (the temporally blocked PPCG inner loop, timed live with HIP events on
 the suite's stream), and the reference omp3 CPU path timed on this host on a
 bounded sample of the same deck.
 
@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
 # read once + each array written once; see DESIGN.md "Kernels").
 BYTES_PER_CELL = {
     "tlhip2d_ppcg_inner_iteration": 64,   # read sd, kx, ky, r, u; write r, u, sd'
+    "tlhip2d_ppcg_inner_iterations": 64,  # the same arrays once for S fused inner steps
     "tlhip2d_cg_calc_w": 32,              # read p, kx, ky; write w
     "tlhip2d_cg_calc_ur": 48,             # read u, p, r, w; write u, r
     "tlhip2d_cg_calc_p": 24,              # read p, r; write p
     "tlhip2d_cheby_iterate": 64,          # read u, u0, p, kx, ky; write u', p, r
 }
-DOMINANT = "tlhip2d_ppcg_inner_iteration"
 
 
 def parse_args():
@@ -52,6 +52,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--solvers", default="cg,cheby,ppcg")
+    ap.add_argument("--ppcg-fusion", type=int, default=5,
+                    help="PPCG inner steps per fused launch (1 = one launch per step)")
     ap.add_argument("--cpu-iters", type=int, default=300,
                     help="CG iterations of the bounded CPU reference sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -68,6 +70,7 @@ def gpu_run(args):
 
     L = lib()
     L.tlhip_init(0)
+    L.tlhip_set_ppcg_fusion(args.ppcg_fusion)          # raises on a bad value
     names = {"cg": tdeck.CG, "cheby": tdeck.CHEBY, "ppcg": tdeck.PPCG}
     solvers = [s for s in args.solvers.split(",") if s]
     total_steps = args.warmup + args.steps
@@ -88,7 +91,7 @@ def gpu_run(args):
     total_iters = 0
     kernel = {}
     for s in solvers:
-        timed = DOMINANT if s == "ppcg" else ("tlhip2d_cg_calc_w" if s == "cg" else "tlhip2d_cheby_iterate")
+        timed = dominant(args) if s == "ppcg" else ("tlhip2d_cg_calc_w" if s == "cg" else "tlhip2d_cheby_iterate")
         L.tlhip_synchronize()
         torch.cuda.synchronize()
         L.tlhip_kernel_timer_start(timed.encode())
@@ -117,6 +120,10 @@ def gpu_run(args):
     for p in probs.values():
         p.close()
     return {"by_solver": by, "kernels": kernel, "time": total_time, "iters": total_iters}
+
+
+def dominant(args) -> str:
+    return "tlhip2d_ppcg_inner_iterations" if args.ppcg_fusion > 1 else "tlhip2d_ppcg_inner_iteration"
 
 
 def cpu_baseline(cells: int, iters: int):
@@ -150,14 +157,14 @@ def cpu_baseline(cells: int, iters: int):
                       f"OMP_NUM_THREADS={cores}"}
 
 
-def load_traffic():
+def load_traffic(kernel: str):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel)."""
-    path = ROOT / "profiles" / "ppcg_inner_4096_traffic.json"
+    path = ROOT / "profiles" / "ppcg_4096_traffic.json"
     if not path.exists():
         return None
     try:
-        return json.loads(path.read_text()).get("bytes_per_launch")
+        return json.loads(path.read_text())["kernels"][kernel]["bytes_per_launch"]
     except Exception:
         return None
 
@@ -171,14 +178,20 @@ def main():
     res = gpu_run(args)
     value = res["iters"] / res["time"]
     n_int = args.cells * args.cells
-    k = res["kernels"].get(DOMINANT)
+    dom = dominant(args)
+    k = res["kernels"].get(dom)
     roofline = None
     if k and k["launches"]:
         achieved = k["gbs"]
-        roofline = {"kernel": DOMINANT, "bound": "hbm", "achieved": achieved,
+        sweeps = res["by_solver"]["ppcg"]["sweeps"]["ppcg_inner"]
+        per_launch = sweeps / k["launches"]
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": load_traffic(), "algorithmic_bytes": BYTES_PER_CELL[DOMINANT] * n_int,
-                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"]}
+                    "traffic": load_traffic(dom), "algorithmic_bytes": BYTES_PER_CELL[dom] * n_int,
+                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"],
+                    "inner_steps_per_launch": per_launch,
+                    # the rate one-step-per-launch sweeps would need to match it
+                    "step_equivalent_gbs": achieved * per_launch}
     line = {
         "metric": "solver iters/s + HBM GB/s, 2D 4096²; FP32-refined vs FP64 time-to-1e-10",
         "value": value,
@@ -195,7 +208,7 @@ def main():
         "config": {"workload": f"2D {args.cells}x{args.cells} layered-rectangle implicit solve, "
                                "FP64, eps 1e-15, solved by CG, Chebyshev (presteps 30) and PPCG "
                                "(10 inner, presteps 30); one step = one timestep of each",
-                   "cells": n_int, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
+                   "cells": n_int, "ppcg_fusion": args.ppcg_fusion, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
         "roofline": roofline,
         "cpu_baseline": None if (args.no_cpu or args.profile_run) else cpu_baseline(args.cells, args.cpu_iters),
         "by_solver": res["by_solver"],

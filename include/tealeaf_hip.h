@@ -154,6 +154,22 @@ int tlhip2d_ppcg_init(int x, int y, int halo_depth, double theta,
 int tlhip2d_ppcg_inner_iteration(int x, int y, int halo_depth, double alpha,
         double beta, double* u, double* r, const double* kx, const double* ky,
         const double* sd_in, double* sd_out);
+/* Temporally blocked PPCG inner loop, 1 <= count <= 5 steps in one pass.
+ * No single counterpart in the kernel interface: it replaces `count` turns
+ * of the loop in 2d/drivers/ppcg_driver.c:165-183 (a depth-1 halo update of
+ * sd, then run_ppcg_inner_iteration with alphas[i], betas[i]) for a chunk
+ * whose four faces are external (neighbours[4] all -1).  With init_from_r,
+ * sd starts as r/theta (run_ppcg_init, 2d/kernel_interface.h:58) and sd_in
+ * is not read.  r, u and sd are read from *_in and written to *_out; the
+ * interiors equal, bit for bit, the reference's in-place results and the
+ * halos of r and u are carried over.  Needs even x, even halo_depth >= 2. */
+int tlhip2d_ppcg_inner_iterations(int x, int y, int halo_depth, int count,
+        const double* alphas, const double* betas, double theta,
+        int init_from_r, const int* neighbours, const double* sd_in,
+        double* sd_out, const double* r_in, double* r_out, const double* u_in,
+        double* u_out, const double* kx, const double* ky);
+/* Tuning: output rows per tile of the fused PPCG kernel (<= 0: default). */
+int tlhip_set_fused_tile_rows(int rows);
 /* 2d/kernel_interface.h:64 run_copy_u */
 int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);
 /* 2d/kernel_interface.h:66 run_calculate_residual */
@@ -297,6 +313,10 @@ typedef struct {
 } tlhip_step_report;
 
 typedef struct tlhip_problem tlhip_problem;
+
+/* Steps per fused PPCG launch used by the driver (1..5; 0 or 1 = one step
+ * per launch through tlhip2d_ppcg_inner_iteration).  Default 5. */
+int tlhip_set_ppcg_fusion(int steps);
 
 /* Allocate a chunk on the device and initialise it as
  * 2d/initialise.c:11-31 does (set_chunk_data, set_chunk_state, halo prime,

@@ -91,7 +91,11 @@ class _Lib:
             raise TealeafError(f"{name} failed ({rc}): {self.last_error()}")
 
     def __getattr__(self, name: str):
-        if name.startswith("tlhip"):
+        """int entry points raise TealeafError on a non-zero code; the rest
+        (tlhip_last_error) are returned as they are."""
+        if name.startswith("tlhip") and name in self.decls:
+            if self.decls[name][0] != "int":
+                return getattr(self.dll, name)
             def checked(*args, _n=name):
                 return self.call(_n, *args)
             return checked

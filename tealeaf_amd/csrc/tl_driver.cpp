@@ -32,6 +32,8 @@ namespace {
 
 enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 
+int g_ppcg_fusion = 5;   // inner steps per fused launch (2D, single chunk)
+
 double* dalloc(long n, int* rc)
 {
     void* p = nullptr;
@@ -98,7 +100,7 @@ struct tlhip_problem {
     double *density0 = nullptr, *density = nullptr, *energy0 = nullptr, *energy = nullptr;
     double *u = nullptr, *u0 = nullptr, *p = nullptr, *r = nullptr, *mi = nullptr, *w = nullptr;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr, *sd = nullptr;
-    double *u_alt = nullptr, *sd_alt = nullptr;
+    double *u_alt = nullptr, *sd_alt = nullptr, *r_alt = nullptr;
     double *volume = nullptr, *x_area = nullptr, *y_area = nullptr, *z_area = nullptr;
     double *cell_x = nullptr, *cell_y = nullptr, *cell_z = nullptr;
     double *vertex_x = nullptr, *vertex_y = nullptr, *vertex_z = nullptr;
@@ -330,6 +332,13 @@ struct tlhip_problem {
     }
 
     // -------------------------------------------------------- PPCG
+    // The temporally blocked inner loop needs the 16-byte row layout and
+    // all faces external (always true for this single-chunk driver).
+    bool fused_ppcg() const
+    {
+        return dims == 2 && g_ppcg_fusion > 1 && hd >= 2 && hd % 2 == 0 && x % 2 == 0 && r_alt;
+    }
+
     int ppcg_main_step(double* rro, double* error)
     {
         double pw = 0.0;
@@ -338,22 +347,41 @@ struct tlhip_problem {
         double rrn = 0.0;
         TRY(calc_ur(alpha, &rrn));
 
-        if (dims == 2) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
-        else TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
-        reset_fields();
-        fields[F_SD] = true;
-        for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
-            TRY(halo_update(1));
-            double* so = sd_alt;
-            if (dims == 2)
-                TRY(tlhip2d_ppcg_inner_iteration(x, y, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
-                                                 kx, ky, sd, so));
-            else
-                TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp], cheby_betas[pp], u,
-                                                 r, kx, ky, kz, sd, so));
-            sd_alt = sd;
-            sd = so;
-            rep->n_ppcg_inner++;
+        if (fused_ppcg()) {
+            // ppcg_init and the inner loop, several steps per pass; the
+            // depth-1 reflections of sd happen inside the kernel.
+            int done = 0;
+            while (done < s.ppcg_inner_steps) {
+                const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
+                TRY(tlhip2d_ppcg_inner_iterations(x, y, hd, cnt, &cheby_alphas[done],
+                                                  &cheby_betas[done], theta, done == 0,
+                                                  neighbours, done == 0 ? nullptr : sd, sd_alt,
+                                                  r, r_alt, u, u_alt, kx, ky));
+                std::swap(sd, sd_alt);
+                std::swap(r, r_alt);
+                std::swap(u, u_alt);
+                done += cnt;
+                rep->n_ppcg_inner += cnt;
+            }
+            if (s.ppcg_inner_steps == 0) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
+        } else {
+            if (dims == 2) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
+            else TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
+            reset_fields();
+            fields[F_SD] = true;
+            for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
+                TRY(halo_update(1));
+                double* so = sd_alt;
+                if (dims == 2)
+                    TRY(tlhip2d_ppcg_inner_iteration(x, y, hd, cheby_alphas[pp], cheby_betas[pp],
+                                                     u, r, kx, ky, sd, so));
+                else
+                    TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp],
+                                                     cheby_betas[pp], u, r, kx, ky, kz, sd, so));
+                sd_alt = sd;
+                sd = so;
+                rep->n_ppcg_inner++;
+            }
         }
         reset_fields();
         fields[F_P] = true;
@@ -484,12 +512,19 @@ struct tlhip_problem {
     void release()
     {
         double* all[] = {density0, density, energy0, energy, u, u0, p, r, mi, w, kx, ky, kz,
-                         sd, u_alt, sd_alt, volume, x_area, y_area, z_area, cell_x, cell_y,
-                         cell_z, vertex_x, vertex_y, vertex_z};
+                         sd, u_alt, sd_alt, r_alt, volume, x_area, y_area, z_area, cell_x,
+                         cell_y, cell_z, vertex_x, vertex_y, vertex_z};
         for (double* a : all)
             if (a) (void)tlhip_free(a);
     }
 };
+
+extern "C" int tlhip_set_ppcg_fusion(int steps)
+{
+    if (steps < 0 || steps > 5) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_ppcg_fusion: 0..5");
+    g_ppcg_fusion = steps;
+    return 0;
+}
 
 extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* states,
                                     int num_states, tlhip_problem** out)
@@ -523,6 +558,8 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
     if (P->dims == 3) {
         P->kz = dalloc(n, &rc);
         P->z_area = dalloc(n, &rc);
+    } else if (s->solver == 3) {
+        P->r_alt = dalloc(n, &rc);      // partner of r for the fused PPCG loop
     }
     P->cell_x = dalloc(P->x + 1, &rc);
     P->cell_y = dalloc(P->y + 1, &rc);

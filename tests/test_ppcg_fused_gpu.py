@@ -1,0 +1,153 @@
+"""Temporally blocked PPCG inner loop (2D) against S separate reference steps.
+
+tlhip2d_ppcg_inner_iterations runs `count` inner steps of
+2d/drivers/ppcg_driver.c:165-183 in one pass.  Its r, u and sd must equal,
+bit for bit, those of the oracle doing what the reference does per step:
+a depth-1 reflective halo update of sd (2d/c_kernels/omp3/local_halos.c)
+then ppcg_inner_iteration (2d/c_kernels/omp3/ppcg.c:28-61), optionally
+preceded by ppcg_init (ppcg.c:8-25).  The halo ring of r and u passes
+through unchanged.  The cases cover partial x windows and y tiles, tiles
+shorter than the step count, grids narrower than one window and halo depth 4.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import ptr
+from tealeaf_amd import deck as tdeck
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, hd, tile_rows): nx even; windows are 128 - 2*SP wide (SP = S
+# rounded up to even), tiles `tile_rows` high (0 = default).
+GRIDS = [(32, 32, 2, 0), (250, 70, 2, 0), (300, 129, 2, 8), (6, 5, 2, 3),
+         (124, 3, 2, 0), (118, 41, 4, 2), (460, 17, 2, 1)]
+
+
+def _oracle_steps(O, x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
+    sd, r, u = sd.copy(), r.copy(), u.copy()
+    if from_r:
+        O.or2d_ppcg_init(x, y, hd, theta, ptr(r), ptr(sd))
+    nb = np.full(4, -1, dtype=np.int32)
+    flags = np.zeros(6, dtype=np.bool_)
+    flags[5] = True                                   # sd only
+    dummy = np.zeros(x * y)
+    for a, b in zip(alphas, betas):
+        O.or2d_local_halos(x, y, 1, hd, ptr(nb), ptr(flags), ptr(dummy), ptr(dummy), ptr(dummy),
+                           ptr(dummy), ptr(dummy), ptr(sd))
+        O.or2d_ppcg_inner_iteration(x, y, hd, a, b, ptr(u), ptr(r), ptr(kx), ptr(ky), ptr(sd))
+    return sd, r, u
+
+
+def _interior(a, x, y, hd):
+    return a.reshape(y, x)[hd:y - hd, hd:x - hd]
+
+
+def _ring(a, x, y, hd):
+    m = np.ones((y, x), dtype=bool)
+    m[hd:y - hd, hd:x - hd] = False
+    return a.reshape(y, x)[m]
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in GRIDS])
+@pytest.mark.parametrize("count", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
+def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r):
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, hd, ty = grid
+    x, y = nx + 2 * hd, ny + 2 * hd
+    rng = np.random.default_rng(1000 * nx + 10 * count + from_r)
+    n = x * y
+    sd = rng.uniform(-1, 1, n)
+    r = rng.uniform(-1, 1, n)
+    u = rng.uniform(-1, 1, n)
+    kx = rng.uniform(0.05, 0.3, n)
+    ky = rng.uniform(0.05, 0.3, n)
+    alphas = rng.uniform(0.1, 0.9, count)
+    betas = rng.uniform(0.1, 0.9, count)
+    theta = 1.7
+    want_sd, want_r, want_u = _oracle_steps(oracle, x, y, hd, alphas, betas, theta, from_r, sd,
+                                            r, u, kx, ky)
+
+    d = {k: DeviceArray.from_host(v) for k, v in
+         dict(sd=sd, r=r, u=u, kx=kx, ky=ky).items()}
+    out = {k: DeviceArray.from_host(np.full(n, np.nan)) for k in ("sd", "r", "u")}
+    nb = np.full(4, -1, dtype=np.int32)
+    hip.tlhip_set_fused_tile_rows(ty if ty else -1)
+    try:
+        hip.tlhip2d_ppcg_inner_iterations(
+            x, y, hd, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
+            nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
+            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr)
+    finally:
+        hip.tlhip_set_fused_tile_rows(-1)
+    hip.tlhip_synchronize()
+    got = {k: v.download() for k, v in out.items()}
+    for name, want, src in (("r", want_r, r), ("u", want_u, u), ("sd", want_sd, sd)):
+        g = got[name]
+        np.testing.assert_array_equal(_interior(g, x, y, hd), _interior(want, x, y, hd),
+                                      err_msg=f"{name} interior")
+        if name != "sd" or not from_r:
+            np.testing.assert_array_equal(_ring(g, x, y, hd), _ring(src, x, y, hd),
+                                          err_msg=f"{name} halo ring")
+    for v in list(d.values()) + list(out.values()):
+        v.free()
+
+
+def test_fused_rejects_bad_layout(hip):
+    from tealeaf_amd.device import DeviceArray
+    raw = hip.dll
+    nb = np.full(4, -1, dtype=np.int32)
+    a = np.ones(2)
+    x, y = 15, 14                        # odd row length
+    bufs = [DeviceArray.from_host(np.zeros(x * y)) for _ in range(8)]
+    p = [b.ptr for b in bufs]
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 1,
+                                           nb.ctypes.data, None, *p[:7])
+    assert rc != 0 and "even" in hip.last_error()
+    x = 16                               # in-place output
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
+                                           nb.ctypes.data, p[0], p[0], p[1], p[2], p[3], p[4],
+                                           p[5], p[6])
+    assert rc != 0
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 6, a.ctypes.data, a.ctypes.data, 1.0, 0,
+                                           nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
+                                           p[6], p[7])
+    assert rc != 0                       # count above 5
+    nb[1] = 3                            # an internal face
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
+                                           nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
+                                           p[6], p[7])
+    assert rc != 0
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("inner", [1, 7, 10, 12])
+def test_driver_fusion_is_bit_identical(hip, inner):
+    """The native driver with the fused inner loop gives the same bits as
+    one launch per inner step, for inner-step counts that split unevenly."""
+    from tealeaf_amd.problem import Problem
+    text = tdeck.layered_2d(90, 2, solver=tdeck.PPCG, presteps=15, inner=inner, max_iters=2000)
+    res = {}
+    for fusion in (1, 5, 3):
+        hip.tlhip_set_ppcg_fusion(fusion)
+        try:
+            with Problem.from_deck(text, 2) as prob:
+                steps = prob.run()
+                res[fusion] = (steps, prob.get("u"), prob.get("energy"), prob.field_summary())
+        finally:
+            hip.tlhip_set_ppcg_fusion(5)
+    base = res[1]
+    for f in (3, 5):
+        steps, u, e, summ = res[f]
+        assert [s["n_ppcg_inner"] for s in steps] == [s["n_ppcg_inner"] for s in base[0]]
+        assert [s["n_ppcg_outer"] for s in steps] == [s["n_ppcg_outer"] for s in base[0]]
+        np.testing.assert_array_equal(u, base[1])
+        np.testing.assert_array_equal(e, base[2])
+        assert summ == base[3]
+
+
+def test_set_ppcg_fusion_range(hip):
+    assert hip.dll.tlhip_set_ppcg_fusion(6) != 0
+    assert hip.dll.tlhip_set_ppcg_fusion(-1) != 0
+    assert hip.dll.tlhip_set_ppcg_fusion(5) == 0
