@@ -57,6 +57,9 @@ def parse_args():
     ap.add_argument("--cpu-iters", type=int, default=300,
                     help="CG iterations of the bounded CPU reference sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--refine-cells", type=int, default=8192,
+                    help="grid of the FP32-refined vs FP64 time-to-1e-10 comparison (0 = skip)")
+    ap.add_argument("--refine-steps", type=int, default=2)
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for counter collection: no CPU baseline, no JSON extras")
     return ap.parse_args()
@@ -122,6 +125,67 @@ def gpu_run(args):
     return {"by_solver": by, "kernels": kernel, "time": total_time, "iters": total_iters}
 
 
+def refine_run(args):
+    """FP32-refined vs FP64 PPCG, each timestep solved to ||u0 - A u|| <=
+    1e-10 ||u0||: the seeded block-density workload (64x64-cell blocks,
+    log-uniform density over a 1e3 or 1e6 contrast, energy U(0.1, 25)),
+    dt 0.004 on [0,10]^2, PPCG with 10 inner steps entered through the
+    reference's error switch (20 CG iterations for the eigenvalue
+    estimate)."""
+    import numpy as np
+    import torch
+    from tealeaf_amd import blocks
+    from tealeaf_amd import deck as tdeck
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.problem import Problem
+
+    L = lib()
+    n = args.refine_cells
+    text = tdeck.layered_2d(n, args.refine_steps, solver=tdeck.PPCG, eps=1e-15, max_iters=20000,
+                            inner=10, presteps=30, epslim=1e30)
+    s, states = tdeck.parse_deck(text, 2)
+    out = {}
+    for contrast, seed in ((1e3, 1001), (1e6, 1002)):
+        dens, en = blocks.block_fields((n, n), 64, seed=seed, rho=(1.0, contrast),
+                                       energy=(0.1, 25.0), halo_depth=s.halo_depth)
+        res = {}
+        u_final = {}
+        for prec in (0, 1):
+            prob = Problem(s, states)
+            prob.inject(dens, en)
+            prob.set_solve(prec, 1e-10)
+            times, reps = [], []
+            for tt in range(args.refine_steps):
+                L.tlhip_synchronize()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                reps.append(prob.step(tt))
+                L.tlhip_synchronize()
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+                print(f"[bench] refine contrast {contrast:g} {'fp32' if prec else 'fp64'} step {tt}: "
+                      f"{times[-1]:.3f} s, {reps[-1]['n_cg_steps']} CG + {reps[-1]['n_ppcg_outer']} PPCG, "
+                      f"rel residual {reps[-1]['rel_residual']:.2e}", file=sys.stderr, flush=True)
+            u_final[prec] = prob.get("u")
+            prob.close()
+            res["fp32_refined" if prec else "fp64"] = {
+                "seconds_per_step": times, "mean_s": sum(times) / len(times),
+                "cg_iters": [r["n_cg_steps"] for r in reps],
+                "ppcg_outer": [r["n_ppcg_outer"] for r in reps],
+                "true_residual_evals": [r["n_true_residuals"] for r in reps],
+                "rel_residual": [r["rel_residual"] for r in reps]}
+        hd = s.halo_depth
+        a = u_final[1].reshape(s.shape())[hd:-hd, hd:-hd]
+        b = u_final[0].reshape(s.shape())[hd:-hd, hd:-hd]
+        res["rel_error_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        res["time_ratio"] = res["fp32_refined"]["mean_s"] / res["fp64"]["mean_s"]
+        out[f"contrast_{contrast:g}"] = res
+    return {"workload": f"2D {n}x{n}, 64x64-cell blocks, log-uniform density [1, C], energy "
+                        f"U(0.1,25), dt 0.004, PPCG 10 inner via the error switch, "
+                        f"{args.refine_steps} steps each solved to relative residual 1e-10",
+            "target_ratio": 0.6, **out}
+
+
 def dominant(args) -> str:
     return "tlhip2d_ppcg_inner_iterations" if args.ppcg_fusion > 1 else "tlhip2d_ppcg_inner_iteration"
 
@@ -176,6 +240,7 @@ def main():
               "is a design, not a build (DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     res = gpu_run(args)
+    refine = refine_run(args) if args.refine_cells > 0 else None
     value = res["iters"] / res["time"]
     n_int = args.cells * args.cells
     dom = dominant(args)
@@ -211,6 +276,7 @@ def main():
                    "cells": n_int, "ppcg_fusion": args.ppcg_fusion, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
         "roofline": roofline,
         "cpu_baseline": None if (args.no_cpu or args.profile_run) else cpu_baseline(args.cells, args.cpu_iters),
+        "refine": refine,
         "by_solver": res["by_solver"],
         "kernels": res["kernels"],
     }

@@ -162,14 +162,27 @@ int tlhip2d_ppcg_inner_iteration(int x, int y, int halo_depth, double alpha,
  * sd starts as r/theta (run_ppcg_init, 2d/kernel_interface.h:58) and sd_in
  * is not read.  r, u and sd are read from *_in and written to *_out; the
  * interiors equal, bit for bit, the reference's in-place results and the
- * halos of r and u are carried over.  Needs even x, even halo_depth >= 2. */
+ * halos of r and u are carried over (sd's too unless init_from_r).  Needs
+ * an even x and 16-byte aligned fields. */
 int tlhip2d_ppcg_inner_iterations(int x, int y, int halo_depth, int count,
         const double* alphas, const double* betas, double theta,
         int init_from_r, const int* neighbours, const double* sd_in,
         double* sd_out, const double* r_in, double* r_out, const double* u_in,
         double* u_out, const double* kx, const double* ky);
-/* Tuning: output rows per tile of the fused PPCG kernel (<= 0: default). */
-int tlhip_set_fused_tile_rows(int rows);
+/* The same loop on FP32 fields (the inner sweeps of the mixed-precision
+ * solve): rows of `pitch` floats (a multiple of 4, >= x), interior
+ * [halo_depth, x-halo_depth) x [halo_depth, y-halo_depth).  Coefficients
+ * and theta are rounded to FP32; every operation is the FP32 one of the
+ * reference's expression, in the same order. */
+int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int halo_depth, int pitch,
+        int count, const double* alphas, const double* betas, double theta,
+        int init_from_r, const int* neighbours, const float* sd_in,
+        float* sd_out, const float* r_in, float* r_out, const float* u_in,
+        float* u_out, const float* kx, const float* ky);
+/* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
+ * 96 for FP64 and 48 for FP32) and next-row prefetch (0 off, 1 on, < 0:
+ * default on). */
+int tlhip_set_fused_tuning(int tile_rows, int prefetch);
 /* 2d/kernel_interface.h:64 run_copy_u */
 int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);
 /* 2d/kernel_interface.h:66 run_calculate_residual */
@@ -310,6 +323,11 @@ typedef struct {
     double eigmin, eigmax;
     int summary_done;
     double vol, mass, ie, temp;
+    /* Not printed by the reference: sqrt(||u0 - A u||^2 / ||u0||^2) in FP64
+     * at the last true-residual evaluation (relative-residual stop or mixed
+     * precision; 0 otherwise) and how many such evaluations the solve made. */
+    double rel_residual;
+    int n_true_residuals;
 } tlhip_step_report;
 
 typedef struct tlhip_problem tlhip_problem;
@@ -318,6 +336,19 @@ typedef struct tlhip_problem tlhip_problem;
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 5. */
 int tlhip_set_ppcg_fusion(int steps);
 
+/* How tlhip_problem_step solves (no counterpart in the reference, which
+ * always solves in FP64 and stops on tl_eps):
+ *   precision 0  FP64 throughout, the reference's arithmetic (default);
+ *   precision 1  FP32 CG / PPCG sweeps (FP32 kx, ky, p, w, r, sd and the
+ *                correction to u) with FP64-accumulated dot products, inside
+ *                an FP64 loop that adds the correction to u and replaces r by
+ *                the true residual u0 - A u whenever ||r|| has fallen 10x;
+ *                2D, CG or PPCG only;
+ *   rel_residual > 0  stop once ||u0 - A u||_2 <= rel_residual ||u0||_2,
+ *                confirmed by the FP64 residual, instead of the eps test.
+ * With precision 1 and rel_residual 0 the eps test applies to the true
+ * ||u0 - A u||^2. */
+int tlhip_problem_set_solve(tlhip_problem* p, int precision, double rel_residual);
 /* Allocate a chunk on the device and initialise it as
  * 2d/initialise.c:11-31 does (set_chunk_data, set_chunk_state, halo prime,
  * store_energy). */

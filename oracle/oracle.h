@@ -80,6 +80,11 @@ typedef struct {
     double eigmin, eigmax;
     int summary_done;
     double vol, mass, ie, temp;
+    /* Not printed by the reference: sqrt(||u0 - A u||^2 / ||u0||^2) in FP64
+     * at the last true-residual evaluation (relative-residual stop or mixed
+     * precision; 0 otherwise) and how many such evaluations the solve made. */
+    double rel_residual;
+    int n_true_residuals;
 } tl_step_report;
 
 /* ---------------- 2D kernels (2d/c_kernels/omp3) ---------------- */
@@ -218,6 +223,10 @@ int or_problem_inject(or_problem* p, const double* density,
         const double* energy0);
 /* One timestep, as 2d/diffuse.c:22-80 (3D: 3d/diffuse.c). tt is 0-based. */
 int or_problem_step(or_problem* p, int tt, tl_step_report* rep);
+/* Extension (not in the reference): stop a solve once ||u0 - A u||_2 <=
+ * rel_tol ||u0||_2, the recurrence residual being confirmed by the true one
+ * (u's depth-1 halo refreshed first); 0 restores the eps test. */
+int or_problem_set_rel_tol(or_problem* p, double rel_tol);
 void or_problem_field_summary(or_problem* p, double out[4]);
 /* Copy a field (full array incl. halo) out.  id: 0 density, 1 energy0,
  * 2 energy, 3 u, 4 u0, 5 p, 6 r, 7 w, 8 kx, 9 ky, 10 kz, 11 sd. */

@@ -29,6 +29,8 @@ struct or_problem {
     int neighbours[6];
     bool fields[6];
     tl_step_report* rep;
+    double rel_tol;   /* > 0: relative true-residual stop (not in the reference) */
+    double bb;        /* ||u0||^2 of the current solve when rel_tol > 0 */
 };
 
 /* ---------- eigenvalues (2d/drivers/eigenvalue_driver.c) ---------- */
@@ -199,6 +201,8 @@ static void k_2norm(or_problem* P, const double* buf, double* norm)
 
 /* ---------- CG (2d/drivers/cg_driver.c) ---------- */
 
+static void start_solve(or_problem* P);
+
 static int cg_init_driver(or_problem* P, double rx, double ry, double rz,
         double* rro)
 {
@@ -217,6 +221,7 @@ static int cg_init_driver(or_problem* P, double rx, double ry, double rz,
     P->fields[F_P] = true;
     halo_update(P, 1);
     k_copy_u(P);
+    start_solve(P);
     return 0;
 }
 
@@ -236,12 +241,43 @@ static void cg_main_step(or_problem* P, int tt, double* rro, double* error)
     P->rep->n_cg_steps++;
 }
 
+/* ||u0 - A u||^2 with u's depth-1 halo refreshed first; w is scratch. */
+static double true_residual(or_problem* P)
+{
+    bool keep[6];
+    memcpy(keep, P->fields, sizeof(keep));
+    reset_fields(P);
+    P->fields[F_U] = true;
+    k_local_halos(P, 1);
+    memcpy(P->fields, keep, sizeof(keep));
+    if (P->s.dims == 2) or2d_calculate_residual(P->x, P->y, P->hd, P->u, P->u0, P->w, P->kx, P->ky);
+    else or3d_calculate_residual(P->x, P->y, P->z, P->hd, P->u, P->u0, P->w, P->kx, P->ky, P->kz);
+    double rr = 0.0;
+    k_2norm(P, P->w, &rr);
+    P->rep->n_true_residuals++;
+    P->rep->rel_residual = P->bb > 0.0 ? sqrt(rr / P->bb) : 0.0;
+    return rr;
+}
+
 /* Convergence test shared by every solver loop: 3D compares sqrt(error)
- * (3d/drivers/cg_solver_driver.c:24-25), 2D the raw value. */
+ * (3d/drivers/cg_solver_driver.c:24-25), 2D the raw value.  With rel_tol
+ * set (an extension, not in the reference): error = ||r||^2 at or below
+ * rel_tol^2 ||u0||^2, confirmed by the true residual. */
 static int converged(or_problem* P, double* error)
 {
+    if (P->rel_tol > 0.0) {
+        const double target = P->rel_tol * P->rel_tol * P->bb;
+        if (!(*error <= target)) return 0;
+        return true_residual(P) <= target;
+    }
     if (P->s.dims == 3) *error = sqrt(*error);
     return fabs(*error) < P->s.eps;
+}
+
+static void start_solve(or_problem* P)
+{
+    P->bb = 0.0;
+    if (P->rel_tol > 0.0) k_2norm(P, P->u0, &P->bb);
 }
 
 static int cg_driver(or_problem* P, double rx, double ry, double rz,
@@ -417,6 +453,7 @@ static int jacobi_driver(or_problem* P, double rx, double ry, double rz,
                 rx, ry, rz, P->density, P->energy, P->u0, P->u, P->kx, P->ky, P->kz);
     if (rc) return rc;
     k_copy_u(P);
+    start_solve(P);
     reset_fields(P);
     P->fields[F_U] = true;
     int tt;
@@ -512,6 +549,13 @@ or_problem* or_problem_create(const tl_settings* s, const tl_state* states,
     }
     prime(P);
     return P;
+}
+
+int or_problem_set_rel_tol(or_problem* P, double rel_tol)
+{
+    if (!(rel_tol >= 0.0)) return -1;
+    P->rel_tol = rel_tol;
+    return 0;
 }
 
 int or_problem_inject(or_problem* P, const double* density,

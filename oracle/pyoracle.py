@@ -68,8 +68,9 @@ def ptr(a) -> int:
     return 0 if a is None else a.ctypes.data
 
 
-def run_problem(settings, states, steps=None, inject=None):
-    """The oracle driver on a parsed deck: per-step reports + final summary."""
+def run_problem(settings, states, steps=None, inject=None, rel_tol=0.0, fields=()):
+    """The oracle driver on a parsed deck: per-step reports + final summary
+    (+ the named fields, by oracle field id, after the last step)."""
     from tealeaf_amd import deck as tdeck
     O = load_oracle()
     cs, cst = tdeck.c_settings(settings), tdeck.c_states(states)
@@ -77,6 +78,8 @@ def run_problem(settings, states, steps=None, inject=None):
     if not P:
         raise RuntimeError("oracle problem creation failed")
     try:
+        if rel_tol:
+            O.or_problem_set_rel_tol(P, float(rel_tol))
         if inject is not None:
             import numpy as np
             dens, en = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in inject)
@@ -92,6 +95,14 @@ def run_problem(settings, states, steps=None, inject=None):
             reps.append(rep.as_dict())
         out = (ctypes.c_double * 4)()
         O.or_problem_field_summary(P, ctypes.addressof(out))
-        return {"steps": reps, "summary": list(out), "settings": settings}
+        got = {}
+        if fields:
+            import numpy as np
+            n = O.or_problem_field_len(P)
+            for name, fid in fields:
+                a = np.empty(n)
+                O.or_problem_get_field(P, fid, a.ctypes.data)
+                got[name] = a
+        return {"steps": reps, "summary": list(out), "settings": settings, "fields": got}
     finally:
         O.or_problem_destroy(P)

@@ -16,9 +16,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "tealeaf_hip.h"
+#include "tl_mixed.h"
 
 int tl_fail(int code, const char* fmt, ...);
 
@@ -110,6 +112,15 @@ struct tlhip_problem {
     bool fields[6] = {};
     tlhip_step_report* rep = nullptr;
 
+    // How the solve runs (tlhip_problem_set_solve).
+    int precision = 0;       // 0: FP64; 1: FP32 sweeps + FP64 residual replacement
+    double rel_tol = 0.0;    // > 0: stop on ||u0 - A u|| <= rel_tol ||u0||
+    double bb = 0.0;         // ||u0||^2 of the current solve
+    // FP32 fields of the mixed-precision solve, row pitch P.
+    int P = 0;
+    float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *w32 = nullptr, *r32 = nullptr,
+          *d32 = nullptr, *sd32 = nullptr, *sd32b = nullptr, *r32b = nullptr, *d32b = nullptr;
+
     // -------------------------------------------------------- kernels
     int local_halos(int depth)
     {
@@ -181,7 +192,8 @@ struct tlhip_problem {
         reset_fields();
         fields[F_U] = fields[F_P] = true;
         TRY(halo_update(1));
-        return copy_u();
+        TRY(copy_u());
+        return start_solve();
     }
 
     int cg_main_step(int tt, double* rro, double* error)
@@ -208,6 +220,51 @@ struct tlhip_problem {
         return std::fabs(*error) < s.eps;
     }
 
+    // ||u0 - A u||^2 in FP64, with u's depth-1 halo refreshed first (PPCG
+    // leaves it stale); w serves as scratch.
+    int true_residual(double* rr)
+    {
+        bool keep[6];
+        std::memcpy(keep, fields, sizeof(fields));
+        reset_fields();
+        fields[F_U] = true;
+        int rc = local_halos(1);
+        std::memcpy(fields, keep, sizeof(fields));
+        TRY(rc);
+        if (dims == 2) TRY(tlhip2d_calculate_residual(x, y, hd, u, u0, w, kx, ky));
+        else TRY(tlhip3d_calculate_residual(x, y, z, hd, u, u0, w, kx, ky, kz));
+        *rr = 0.0;
+        TRY(norm2(w, rr));
+        rep->n_true_residuals++;
+        rep->rel_residual = bb > 0.0 ? std::sqrt(*rr / bb) : 0.0;
+        return 0;
+    }
+
+    // The stopping test of the FP64 solvers: the reference's, or with
+    // rel_tol set, a recurrence residual ||r||^2 = error at or below
+    // rel_tol^2 ||u0||^2 that the true FP64 residual confirms.
+    int stop(double* error, bool* done)
+    {
+        if (rel_tol <= 0.0) {
+            *done = converged(error);
+            return 0;
+        }
+        *done = false;
+        const double target = rel_tol * rel_tol * bb;
+        if (!(*error <= target)) return 0;
+        double rr;
+        TRY(true_residual(&rr));
+        *done = rr <= target;
+        return 0;
+    }
+
+    int start_solve()
+    {
+        bb = 0.0;
+        if (rel_tol > 0.0 || precision == 1) TRY(norm2(u0, &bb));
+        return 0;
+    }
+
     int cg_driver(double rx, double ry, double rz, double* error)
     {
         double rro;
@@ -216,7 +273,9 @@ struct tlhip_problem {
         for (tt = 0; tt < s.max_iters; ++tt) {
             TRY(cg_main_step(tt, &rro, error));
             TRY(halo_update(1));
-            if (converged(error)) break;
+            bool done;
+            TRY(stop(error, &done));
+            if (done) break;
         }
         rep->printed_cg_iters = tt;
         return 0;
@@ -323,7 +382,9 @@ struct tlhip_problem {
                 TRY(cheby_step(ncheby, ncheby >= est && (tt + 1) % 10 == 0, error));
             }
             TRY(halo_update(1));
-            if (converged(error)) break;
+            bool done;
+            TRY(stop(error, &done));
+            if (done) break;
         }
         rep->printed_cg_iters = tt - ncheby + 1;
         rep->printed_inner_iters = ncheby;
@@ -416,10 +477,141 @@ struct tlhip_problem {
                 TRY(ppcg_main_step(&rro, error));
             }
             TRY(halo_update(1));
-            if (converged(error)) break;
+            bool done;
+            TRY(stop(error, &done));
+            if (done) break;
         }
         rep->printed_cg_iters = tt - nppcg + 1;
         rep->printed_inner_iters = nppcg;
+        return 0;
+    }
+
+    // -------------------------------------------------------- mixed precision
+    // CG / PPCG with FP32 sweeps (kx, ky, p, w, r, sd and d, the correction
+    // to u since the last FP64 update) and FP64-accumulated dots, inside an
+    // FP64 residual-replacement loop ("reliable updates", as in Sleijpen &
+    // van der Vorst, Computing 56 (1996)): whenever the FP32 recurrence
+    // residual has fallen kDrop below the last true one, or claims
+    // convergence, u += d in FP64 and r is replaced by u0 - A u computed in
+    // FP64.  The CG/PPCG control flow, the switch rule, the Lanczos
+    // eigenvalue estimate and the Chebyshev coefficients are the reference's
+    // (cg_driver.c, ppcg_driver.c), in FP64 on the host.  Convergence is
+    // only ever declared on a true FP64 residual.
+    static constexpr double kDrop = 0.1;
+
+    int mixed_alloc()
+    {
+        if (kx32) return 0;
+        if (dims != 2) return tl_fail(TLHIP_ERR_ARG, "mixed precision: 2D only");
+        P = (x + 3) / 4 * 4;
+        const size_t bytes = sizeof(float) * (size_t)P * (size_t)y;
+        float** all[] = {&kx32, &ky32, &p32, &w32, &r32, &d32, &sd32, &sd32b, &r32b, &d32b};
+        for (float** f : all) TRY(tlhip_alloc(bytes, (void**)f));
+        return 0;
+    }
+
+    // u += d (FP64); r32 = u0 - A u computed in FP64; *rr = its squared norm.
+    int replace_residual(double* rr)
+    {
+        TRY(tl::mix::promote(x, y, hd, P, u, d32));
+        *rr = 0.0;
+        TRY(tl::mix::residual64(x, y, hd, P, u, u0, kx, ky, r32, rr));
+        rep->n_true_residuals++;
+        rep->rel_residual = bb > 0.0 ? std::sqrt(*rr / bb) : 0.0;
+        return 0;
+    }
+
+    int mixed_cg_step(int tt, double* rro, double* rrn)
+    {
+        double pw = 0.0;
+        TRY(tl::mix::calc_w32(x, y, hd, P, &pw, p32, w32, kx32, ky32));
+        const double alpha = *rro / pw;
+        cg_alphas[tt] = alpha;
+        TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, rrn, d32, p32, r32, w32));
+        const double beta = *rrn / *rro;
+        cg_betas[tt] = beta;
+        TRY(tl::mix::calc_p32(x, y, hd, P, beta, false, p32, r32));
+        *rro = *rrn;
+        rep->n_cg_steps++;
+        return 0;
+    }
+
+    int mixed_ppcg_step(double* rro, double* rrn)
+    {
+        double pw = 0.0;
+        TRY(tl::mix::calc_w32(x, y, hd, P, &pw, p32, w32, kx32, ky32));
+        const double alpha = *rro / pw;
+        double unused = 0.0;
+        TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
+        const int per = std::max(1, g_ppcg_fusion);
+        int done = 0;
+        while (done < s.ppcg_inner_steps) {
+            const int cnt = std::min(per, s.ppcg_inner_steps - done);
+            TRY(tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, cnt, &cheby_alphas[done],
+                                                  &cheby_betas[done], theta, done == 0,
+                                                  neighbours, done == 0 ? nullptr : sd32, sd32b,
+                                                  r32, r32b, d32, d32b, kx32, ky32));
+            std::swap(sd32, sd32b);
+            std::swap(r32, r32b);
+            std::swap(d32, d32b);
+            done += cnt;
+            rep->n_ppcg_inner += cnt;
+        }
+        *rrn = 0.0;
+        TRY(tl::mix::norm32(x, y, hd, P, r32, rrn));
+        const double beta = *rrn / *rro;
+        TRY(tl::mix::calc_p32(x, y, hd, P, beta, false, p32, r32));
+        *rro = *rrn;
+        rep->n_ppcg_outer++;
+        return 0;
+    }
+
+    int mixed_driver(double rx, double ry, double* error)
+    {
+        if (s.solver != 1 && s.solver != 3)
+            return tl_fail(TLHIP_ERR_ARG, "mixed precision: CG or PPCG only (solver %d)", s.solver);
+        TRY(mixed_alloc());
+        double rro;
+        TRY(cg_init_driver(rx, ry, 0.0, &rro));   // FP64 kx, ky, u, r = u0 - A u, p = r
+        TRY(tl::mix::demote(kx, x, y, kx32, P));
+        TRY(tl::mix::demote(ky, x, y, ky32, P));
+        TRY(tl::mix::demote(r, x, y, r32, P));
+        TRY(tl::mix::calc_p32(x, y, hd, P, 0.0, true, p32, r32));
+        TRY(tl::mix::zero32(d32, (long)P * y));
+        const double target = rel_tol > 0.0 ? rel_tol * rel_tol * bb : s.eps;
+        double rr_true = rro, last_true = rro;
+        *error = rro;
+        int nppcg = 0, tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            double rrn = 0.0;
+            if (s.solver == 3 && switch_now(nppcg, tt, *error)) {
+                if (++nppcg == 1) {
+                    TRY(eigenvalues(tt));
+                    cheby_coefs(s.ppcg_inner_steps);
+                    // ppcg_init_driver recomputes the residual (ppcg_driver.c:72-77)
+                    TRY(replace_residual(&rr_true));
+                    last_true = rr_true;
+                }
+                TRY(mixed_ppcg_step(&rro, &rrn));
+            } else {
+                TRY(mixed_cg_step(tt, &rro, &rrn));
+            }
+            *error = rrn;
+            if (rrn <= target || rrn < kDrop * kDrop * last_true) {
+                TRY(replace_residual(&rr_true));
+                last_true = rr_true;
+                rro = rr_true;        // continue the recurrence from the true residual
+                *error = rr_true;
+                if (rr_true <= target) break;
+            }
+        }
+        if (tt == s.max_iters) TRY(replace_residual(&rr_true));   // fold the last correction
+        if (s.solver == 1) {
+            rep->printed_cg_iters = tt;                  // as cg_driver.c prints
+        } else {
+            rep->printed_cg_iters = tt - nppcg + 1;      // as ppcg_driver.c prints
+            rep->printed_inner_iters = nppcg;
+        }
         return 0;
     }
 
@@ -432,6 +624,7 @@ struct tlhip_problem {
             TRY(tlhip3d_jacobi_solver_init(x, y, z, hd, s.coefficient, rx, ry, rz, density, energy,
                                            u0, u, kx, ky, kz));
         TRY(copy_u());
+        TRY(start_solve());
         reset_fields();
         fields[F_U] = true;
         int tt;
@@ -445,7 +638,9 @@ struct tlhip_problem {
                 TRY(norm2(r, error));
             }
             TRY(halo_update(1));
-            if (converged(error)) break;
+            bool done;
+            TRY(stop(error, &done));
+            if (done) break;
         }
         rep->printed_cg_iters = tt;
         return 0;
@@ -472,7 +667,9 @@ struct tlhip_problem {
         fields[F_ENERGY1] = fields[F_DENSITY] = true;
         TRY(halo_update(2));
         double error = 1e+10;
-        switch (s.solver) {
+        if (precision == 1) {
+            TRY(mixed_driver(rx, ry, &error));
+        } else switch (s.solver) {
         case 0: TRY(jacobi_driver(rx, ry, rz, &error)); break;
         case 1: TRY(cg_driver(rx, ry, rz, &error)); break;
         case 2: TRY(cheby_driver(rx, ry, rz, &error)); break;
@@ -516,6 +713,9 @@ struct tlhip_problem {
                          cell_y, cell_z, vertex_x, vertex_y, vertex_z};
         for (double* a : all)
             if (a) (void)tlhip_free(a);
+        float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b};
+        for (float* a : all32)
+            if (a) (void)tlhip_free(a);
     }
 };
 
@@ -523,6 +723,25 @@ extern "C" int tlhip_set_ppcg_fusion(int steps)
 {
     if (steps < 0 || steps > 5) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_ppcg_fusion: 0..5");
     g_ppcg_fusion = steps;
+    return 0;
+}
+
+extern "C" int tlhip_problem_set_solve(tlhip_problem* P, int precision, double rel_residual)
+{
+    if (!P) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: null problem");
+    if (precision != 0 && precision != 1)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: precision must be 0 or 1");
+    if (!(rel_residual >= 0.0))
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: rel_residual must be >= 0");
+    if (precision == 1) {
+        if (P->dims != 2)
+            return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: mixed precision is 2D only");
+        if (P->s.solver != 1 && P->s.solver != 3)
+            return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: mixed precision needs CG or PPCG");
+        TRY(P->mixed_alloc());
+    }
+    P->precision = precision;
+    P->rel_tol = rel_residual;
     return 0;
 }
 

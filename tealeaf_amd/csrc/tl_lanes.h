@@ -1,0 +1,114 @@
+// Lane-vector helpers for the row-marching kernels on pitched 2D arrays.
+//
+// A lane owns N adjacent cells of a row moved as one 16-byte vector (2 FP64
+// or 4 FP32 cells), so a wave covers 64*N columns with dwordx4 accesses.
+// Windows start on a multiple of N; cells of a lane outside the window's
+// output range are masked per cell.
+#pragma once
+
+#include "tl_common.h"
+
+namespace tl {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct Lane16;
+template <> struct Lane16<double> { static constexpr int N = 2; using V = dvec2; };
+template <> struct Lane16<float> { static constexpr int N = 4; using V = fvec4; };
+
+// The N cells of one lane.
+template <class T>
+struct Cells {
+    T v[Lane16<T>::N];
+};
+
+template <class T>
+__device__ __forceinline__ Cells<T> ldc(const T* __restrict__ p, long i)
+{
+    return __builtin_bit_cast(Cells<T>, *reinterpret_cast<const typename Lane16<T>::V*>(p + i));
+}
+
+template <class T>
+__device__ __forceinline__ void stc(T* __restrict__ p, long i, const Cells<T>& c)
+{
+    *reinterpret_cast<typename Lane16<T>::V*>(p + i) = __builtin_bit_cast(typename Lane16<T>::V, c);
+}
+
+template <class T>
+__device__ __forceinline__ Cells<T> zeros()
+{
+    Cells<T> c;
+#pragma unroll
+    for (int i = 0; i < Lane16<T>::N; ++i) c.v[i] = T(0);
+    return c;
+}
+
+// Row pitch X (a multiple of N), interior columns [hd, hd+nx), rows [hd, hd+ny).
+struct PitchGrid {
+    int X, hd, nx, ny;
+};
+
+// Strips of 64*N-column windows whose centre `w` columns are written.
+struct StripTile {
+    int w;     // output columns per wave (a multiple of N)
+    int cw0;   // first column of strip 0's window (a multiple of N; may be < 0)
+    int ns;    // strips along x
+    int nbx;   // blocks along x (kWaves strips each)
+    int nby;   // tiles along y
+    int ty;    // output rows per tile
+};
+
+// Strip geometry for a window that must extend `margin` columns beyond its
+// output columns on both sides.
+template <class T>
+inline StripTile strip_tile(const PitchGrid& g, int margin, int ty, int waves_per_block)
+{
+    constexpr int N = Lane16<T>::N;
+    StripTile t;
+    const int off = margin + (((g.hd - margin) % N) + N) % N;
+    t.w = (TL_WAVE * N - off - margin) / N * N;
+    t.cw0 = g.hd - off;
+    t.ns = (int)tl_cdiv(g.nx, t.w);
+    t.nbx = (int)tl_cdiv(t.ns, waves_per_block);
+    t.ty = ty;
+    t.nby = (int)tl_cdiv(g.ny, t.ty);
+    return t;
+}
+
+// blockIdx -> tile so that each XCD (blocks are dealt round-robin to the 8
+// XCDs) works on a contiguous run of tiles and shares their rows in its L2.
+__device__ __forceinline__ int xcd_tile_of(int bid, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, k = bid & 7, i = bid >> 3;
+    return k * q + (k < r ? k : r) + i;
+}
+
+// A * a for the N cells of a lane (2d/shared.h:50-54, same association), with
+// x neighbours outside the interior mirroring the cell (a depth-1 reflective
+// halo).  up/dn are the rows above/below, already mirrored by the caller.
+template <class T>
+__device__ __forceinline__ Cells<T> apply_a_lane(const Cells<T>& cen, const Cells<T>& up,
+                                                 const Cells<T>& dn, const Cells<T>& kxc,
+                                                 const Cells<T>& kyu, const Cells<T>& kyc, int c,
+                                                 int xin0, int xin1)
+{
+    constexpr int N = Lane16<T>::N;
+    const T lft_edge = tl_lane_prev(cen.v[N - 1], cen.v[0]);
+    const T rgt_edge = tl_lane_next(cen.v[0], cen.v[N - 1]);
+    const T kx_edge = tl_lane_next(kxc.v[0], kxc.v[N - 1]);
+    Cells<T> a;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T lft = i == 0 ? lft_edge : cen.v[i - 1];
+        T rgt = i == N - 1 ? rgt_edge : cen.v[i + 1];
+        if (c + i == xin0) lft = cen.v[i];
+        if (c + i == xin1 - 1) rgt = cen.v[i];
+        const T kxr = i == N - 1 ? kx_edge : kxc.v[i + 1];
+        const T d = (T(1) + (kxr + kxc.v[i])) + (kyu.v[i] + kyc.v[i]);
+        a.v[i] = d * cen.v[i] - (kxr * rgt + kxc.v[i] * lft) - (kyu.v[i] * up.v[i] + kyc.v[i] * dn.v[i]);
+    }
+    return a;
+}
+
+}  // namespace tl

@@ -1,0 +1,31 @@
+// Internal interface of the mixed-precision 2D kernels (FP32 sweeps, FP64
+// dots and updates) used by the native driver.  FP32 arrays have row pitch
+// P (a multiple of 4, >= x); FP64 arrays have row pitch x.  Scalars come
+// back synchronously, like the reference's reductions.
+#pragma once
+
+namespace tl {
+namespace mix {
+
+// a[0..n) = 0
+int zero32(float* a, long n);
+// dst[row*P+col] = (float)src[row*x+col] over the whole chunk; pad columns 0.
+int demote(const double* src, int x, int y, float* dst, int P);
+// w = A p; *pw = sum w*p (FP64)
+int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,
+             const float* ky);
+// d += alpha p; r -= alpha w; *rrn = sum r*r (FP64)
+int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,
+              float* r, const float* w);
+// p = beta p + r, or p = r when init
+int calc_p32(int x, int y, int hd, int P, double beta, bool init, float* p, const float* r);
+// *out = sum a*a (FP64)
+int norm32(int x, int y, int hd, int P, const float* a, double* out);
+// u += d (FP64); d = 0
+int promote(int x, int y, int hd, int P, double* u, float* d);
+// r32 = (float)(u0 - A u) computed in FP64; *rr = sum (u0 - A u)^2 (FP64)
+int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
+               const double* ky, float* r32, double* rr);
+
+}  // namespace mix
+}  // namespace tl

@@ -1,0 +1,372 @@
+// FP32 kernels of the mixed-precision 2D solve (FP32 Krylov sweeps inside an
+// FP64 residual-replacement loop).
+//
+// The reference solves in FP64 throughout (2d/drivers/cg_driver.c,
+// ppcg_driver.c).  Here the CG and PPCG vector sweeps run on FP32 copies of
+// kx, ky, p, w, r, sd and of the correction d accumulated since the last
+// FP64 update, halving their HBM traffic; every dot product is accumulated
+// in FP64; the solution u, the right-hand side u0 and the true residual
+// u0 - A u stay FP64.
+//
+// FP32 arrays have a row pitch P >= x that is a multiple of 4, so that a lane
+// moves four cells with one 16-byte access; interior cells are [hd, x-hd) x
+// [hd, y-hd) as in the FP64 arrays.  No kernel here reads an FP32 halo: a
+// stencil neighbour outside the interior takes the cell's own value, which
+// is what the reference's depth-1 reflective halo update provides on a
+// single chunk (2d/c_kernels/omp3/local_halos.c).
+#include "tl_lanes.h"
+#include "tl_mixed.h"
+#include "tl_ops.h"
+#include "tealeaf_hip.h"
+
+using namespace tl;
+
+namespace {
+
+using C4 = Cells<float>;
+
+// ---------------------------------------------------------------- pointwise
+// Thread q of a row covers the 4-cell group [4q, 4q+4) of the pitched row;
+// groups that meet the interior are visited, cells outside it are masked.
+struct Groups {
+    int hd, x, P;      // FP32 pitch P
+    int q0, nq;        // first group, groups per row
+    int y0, ny;        // interior rows
+    __host__ __device__ long total() const { return (long)nq * ny; }
+};
+
+Groups groups_of(int x, int y, int hd, int P)
+{
+    Groups G;
+    G.hd = hd;
+    G.x = x;
+    G.P = P;
+    G.q0 = hd / 4;
+    G.nq = (x - hd + 3) / 4 - G.q0;
+    G.y0 = hd;
+    G.ny = y - 2 * hd;
+    return G;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_groups(Groups G, Op op, TlReduce rd)
+{
+    double acc[Op::NRED > 0 ? Op::NRED : 1] = {};
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < G.total();
+         t += (long)gridDim.x * kBlock) {
+        const int row = G.y0 + (int)(t / G.nq);
+        const int c = 4 * (G.q0 + (int)(t % G.nq));
+        unsigned mask = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (c + i >= G.hd && c + i < G.x - G.hd) mask |= 1u << i;
+        op.apply(row, c, mask, acc);
+    }
+    if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
+}
+
+// d += alpha p; r -= alpha w; rrn += r*r   (cg.c:116-143 in FP32, FP64 sum)
+struct OpUr32 {
+    static constexpr int NRED = 1;
+    int P;
+    float alpha;
+    float* d;
+    const float* p;
+    float* r;
+    const float* w;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        const long i0 = (long)row * P + c;
+        C4 dv = ldc(d, i0), pv = ldc(p, i0), rv = ldc(r, i0), wv = ldc(w, i0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!(mask >> i & 1)) continue;
+            dv.v[i] = dv.v[i] + alpha * pv.v[i];
+            rv.v[i] = rv.v[i] - alpha * wv.v[i];
+            acc[0] += (double)rv.v[i] * (double)rv.v[i];
+        }
+        if (mask == 15u) {
+            stc(d, i0, dv);
+            stc(r, i0, rv);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (mask >> i & 1) {
+                    d[i0 + i] = dv.v[i];
+                    r[i0 + i] = rv.v[i];
+                }
+        }
+    }
+};
+
+// p = beta p + r (cg.c:146-164), or p = r when init
+struct OpP32 {
+    static constexpr int NRED = 0;
+    int P;
+    float beta;
+    int init;
+    float* p;
+    const float* r;
+    __device__ void apply(int row, int c, unsigned mask, double*) const
+    {
+        const long i0 = (long)row * P + c;
+        C4 pv = ldc(p, i0);
+        const C4 rv = ldc(r, i0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pv.v[i] = init ? rv.v[i] : beta * pv.v[i] + rv.v[i];
+        if (mask == 15u) {
+            stc(p, i0, pv);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (mask >> i & 1) p[i0 + i] = pv.v[i];
+        }
+    }
+};
+
+// sum of squares, FP64
+struct OpNorm32 {
+    static constexpr int NRED = 1;
+    int P;
+    const float* a;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        const C4 v = ldc(a, (long)row * P + c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (mask >> i & 1) acc[0] += (double)v.v[i] * (double)v.v[i];
+    }
+};
+
+// u += d (FP64 update of the solution); d = 0
+struct OpPromote {
+    static constexpr int NRED = 0;
+    int P, x;
+    double* u;
+    float* d;
+    __device__ void apply(int row, int c, unsigned mask, double*) const
+    {
+        const long i0 = (long)row * P + c, j0 = (long)row * x + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (mask >> i & 1) u[j0 + i] = u[j0 + i] + (double)d[i0 + i];
+        if (mask == 15u) {
+            stc(d, i0, zeros<float>());
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (mask >> i & 1) d[i0 + i] = 0.0f;
+        }
+    }
+};
+
+// r = u0 - A u in FP64 (solver_methods.c:28-48, with u's x/y neighbours
+// outside the interior mirrored), stored as FP32; rr += r*r in FP64.
+struct OpResidual64 {
+    static constexpr int NRED = 1;
+    int P, x, y, hd;
+    const double* u;
+    const double* u0;
+    const double* kx;
+    const double* ky;
+    float* r32;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        const int xin0 = hd, xin1 = x - hd, yin0 = hd, yin1 = y - hd;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!(mask >> i & 1)) continue;
+            const int k = c + i;
+            const long j = (long)row * x + k;
+            const double cen = u[j];
+            const double lft = k == xin0 ? cen : u[j - 1];
+            const double rgt = k == xin1 - 1 ? cen : u[j + 1];
+            const double dn = row == yin0 ? cen : u[j - x];
+            const double up = row == yin1 - 1 ? cen : u[j + x];
+            const double kxc = kx[j], kxr = kx[j + 1], kyc = ky[j], kyu = ky[j + x];
+            const double d = (1.0 + (kxr + kxc)) + (kyu + kyc);
+            const double a = d * cen - (kxr * rgt + kxc * lft) - (kyu * up + kyc * dn);
+            const double rv = u0[j] - a;
+            r32[(long)row * P + k] = (float)rv;
+            acc[0] += rv * rv;
+        }
+    }
+};
+
+template <class Op>
+int launch_groups(const Groups& G, const Op& op, const char* what, double* out)
+{
+    if (G.nq <= 0 || G.ny <= 0) {
+        if (out) *out = 0.0;
+        return 0;
+    }
+    const unsigned nb = flat_blocks(G.total());
+    const int slot = tl_timer_begin(what);
+    hipLaunchKernelGGL((k_groups<Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), G, op,
+                       tl_reduce_slots());
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    if constexpr (Op::NRED > 0) {
+        TL_TRY(tl_finish_reduce(Op::NRED, nb, what));
+        return tl_fetch(1, out, what);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- demote
+__global__ void __launch_bounds__(kBlock)
+k_demote(const double* __restrict__ src, int x, int y, float* __restrict__ dst, int P)
+{
+    const long n = (long)P * y;
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < n; t += (long)gridDim.x * kBlock) {
+        const int row = (int)(t / P), col = (int)(t % P);
+        dst[t] = col < x ? (float)src[(long)row * x + col] : 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero32(float* __restrict__ a, long n)
+{
+    for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < n; t += (long)gridDim.x * kBlock)
+        a[t] = 0.0f;
+}
+
+// ---------------------------------------------------------------- w = A p
+// One wave marches up the rows of its strip holding the rows below, at and
+// above the current one in registers (p, ky) and prefetching the next row.
+__global__ void __launch_bounds__(kBlock)
+k_calc_w32(PitchGrid g, StripTile t, const float* __restrict__ p, float* __restrict__ w,
+           const float* __restrict__ kx, const float* __restrict__ ky, TlReduce rd)
+{
+    constexpr int N = 4;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
+    const int bx = tile % t.nbx, by = tile / t.nbx;
+    const int sx = bx * kWaves + wave;
+    double acc[1] = {0.0};
+    if (sx < t.ns) {
+        const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
+        const int o0 = xin0 + sx * t.w;
+        const int o1 = min(o0 + t.w, xin1);
+        const int c = t.cw0 + sx * t.w + N * lane;
+        const bool ld_ok = c >= 0 && c + N <= g.X;
+        const bool all_out = c >= o0 && c + N <= o1;
+        const bool any_out = c + N > o0 && c < o1;
+        const int j0 = yin0 + by * t.ty;
+        const int j1 = min(j0 + t.ty, yin1);
+        const long X = g.X;
+        auto ld = [&](const float* __restrict__ f, int row) -> C4 {
+            return ld_ok ? ldc(f, (long)row * X + c) : zeros<float>();
+        };
+        // rows j0-1 .. j0+1 exist: hd >= 1 and j0 < yin1
+        C4 dn = ld(p, j0 - 1), cen = ld(p, j0), up = ld(p, j0 + 1);
+        C4 kxc = ld(kx, j0), kyc = ld(ky, j0), kyu = ld(ky, j0 + 1);
+        for (int j = j0; j < j1; ++j) {
+            C4 up2 = zeros<float>(), kyu2 = zeros<float>(), kxn = zeros<float>();
+            if (j + 1 < j1) {            // rows j+2 <= yin1 <= y-1
+                up2 = ld(p, j + 2);
+                kyu2 = ld(ky, j + 2);
+                kxn = ld(kx, j + 1);
+            }
+            const C4 upm = j == yin1 - 1 ? cen : up;
+            const C4 dnm = j == yin0 ? cen : dn;
+            const C4 av = apply_a_lane(cen, upm, dnm, kxc, kyu, kyc, c, xin0, xin1);
+            if (any_out) {
+                const long i0 = (long)j * X + c;
+                if (all_out) {
+                    stc(w, i0, av);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) acc[0] += (double)av.v[i] * (double)cen.v[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (c + i >= o0 && c + i < o1) {
+                            w[i0 + i] = av.v[i];
+                            acc[0] += (double)av.v[i] * (double)cen.v[i];
+                        }
+                }
+            }
+            dn = cen;
+            cen = up;
+            up = up2;
+            kyc = kyu;
+            kyu = kyu2;
+            kxc = kxn;
+        }
+    }
+    tl_reduce_partial<1, kBlock>(acc, rd);
+}
+
+int g_w32_ty = 32;
+
+}  // namespace
+
+namespace tl {
+namespace mix {
+
+int zero32(float* a, long n)
+{
+    TL_TRY(tl_ensure_init());
+    hipLaunchKernelGGL(k_zero32, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), a, n);
+    return tl_check_launch("tlhip_mix_zero32");
+}
+
+int demote(const double* src, int x, int y, float* dst, int P)
+{
+    const char* what = "tlhip_mix_demote";
+    TL_TRY(tl_ensure_init());
+    hipLaunchKernelGGL(k_demote, dim3(flat_blocks((long)P * y)), dim3(kBlock), 0, tl_stream(), src,
+                       x, y, dst, P);
+    return tl_check_launch(what);
+}
+
+int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,
+             const float* ky)
+{
+    const char* what = "tlhip2d_mix_calc_w32";
+    const PitchGrid g{P, hd, x - 2 * hd, y - 2 * hd};
+    const StripTile t = strip_tile<float>(g, 1, g_w32_ty, kWaves);
+    const unsigned nb = (unsigned)(t.nbx * t.nby);
+    if (nb > TL_MAX_RED_BLOCKS) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    const int slot = tl_timer_begin(what);
+    hipLaunchKernelGGL(k_calc_w32, dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, p, w, kx, ky,
+                       tl_reduce_slots());
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return tl_fetch(1, pw, what);
+}
+
+int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,
+              float* r, const float* w)
+{
+    const OpUr32 op{P, (float)alpha, d, p, r, w};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_calc_ur32", rrn);
+}
+
+int calc_p32(int x, int y, int hd, int P, double beta, bool init, float* p, const float* r)
+{
+    const OpP32 op{P, (float)beta, init ? 1 : 0, p, r};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_calc_p32", nullptr);
+}
+
+int norm32(int x, int y, int hd, int P, const float* a, double* out)
+{
+    const OpNorm32 op{P, a};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_norm32", out);
+}
+
+int promote(int x, int y, int hd, int P, double* u, float* d)
+{
+    const OpPromote op{P, x, u, d};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_promote", nullptr);
+}
+
+int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
+               const double* ky, float* r32, double* rr)
+{
+    const OpResidual64 op{P, x, y, hd, u, u0, kx, ky, r32};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_residual64", rr);
+}
+
+}  // namespace mix
+}  // namespace tl

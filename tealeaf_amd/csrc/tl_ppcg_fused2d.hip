@@ -1,5 +1,7 @@
 // Temporally blocked PPCG inner loop (2D): S consecutive inner steps of
-// 2d/drivers/ppcg_driver.c:165-183 in one pass over memory.
+// 2d/drivers/ppcg_driver.c:165-183 in one pass over memory, in FP64 (the
+// reference's arithmetic) or FP32 (the inner sweeps of the mixed-precision
+// solve).
 //
 // The reference runs, per inner step pp, a depth-1 halo update of sd
 // followed by ppcg_inner_iteration (2d/c_kernels/omp3/ppcg.c:28-61):
@@ -8,20 +10,22 @@
 // outside the interior simply takes the cell's own value.  Every time level
 // k of a cell depends only on level k-1 within one cell of it, so a wave can
 // carry S levels at once:
+//  * each lane holds N adjacent cells of a row (one 16-byte vector: 2 FP64
+//    or 4 FP32 cells); the wave covers a window of 64*N columns;
 //  * it marches up its rows keeping, for each level, the last three rows of
 //    sd and the last row of r and u in registers; level k computes row
 //    (j - k + 1) while level 1 computes row j;
-//  * its 128-column window overlaps the neighbouring windows by SP >= S
-//    columns on each side and it writes only the centre 128 - 2*SP, where
-//    all S levels are exact; the march starts S-1 rows below its tile for
-//    the same reason in y;
+//  * windows overlap their neighbours by at least S columns on each side and
+//    a wave writes only its centre columns, where all S levels are exact;
+//    the march starts S-1 rows below its tile for the same reason in y;
 //  * r, u and sd are read from one buffer and written to another, because
 //    the overlapped windows of other waves still need the level-0 values.
 // Each level applies the reference's per-cell expression in the reference's
-// order, so the result is bit-for-bit that of S separate steps.  HBM traffic
-// for S steps is that of one: read sd, kx, ky, r, u; write sd, r, u.
+// order, so in FP64 the result is bit-for-bit that of S separate steps.  HBM
+// traffic for S steps is that of one: read sd, kx, ky, r, u; write sd, r, u.
 // Optionally sd at level 0 is ppcg_init's r/theta (ppcg.c:8-25), computed on
 // the fly from r.
+#include "tl_lanes.h"
 #include "tl_ops.h"
 #include "tealeaf_hip.h"
 
@@ -31,88 +35,71 @@ namespace {
 
 constexpr int kMaxFused = 5;
 
-template <int S>
+template <class T, int S>
 struct PpcgCoefs {
-    double alpha[S], beta[S];
+    T alpha[S], beta[S];
 };
 
-struct FusedTile {
-    int w;     // output columns per wave
-    int ns;    // strips along x
-    int nbx;   // blocks along x
-    int nby;   // tiles along y
-    int ty;    // output rows per tile
-};
-
-__device__ __forceinline__ int xcd_tile(int bid, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, k = bid & 7, i = bid >> 3;
-    return k * q + (k < r ? k : r) + i;
-}
-
-__device__ __forceinline__ double2 ld2(const double* __restrict__ p, long i)
-{
-    return *reinterpret_cast<const double2*>(p + i);
-}
-
-template <int S, bool FROM_R>
+template <class T, int S, bool FROM_R, bool PF>
 __global__ void __launch_bounds__(kBlock)
-k_ppcg_fused2d(TlGrid2 g, FusedTile t, PpcgCoefs<S> cf, double theta,
-               const double* __restrict__ sd_in, double* __restrict__ sd_out,
-               const double* __restrict__ r_in, double* __restrict__ r_out,
-               const double* __restrict__ u_in, double* __restrict__ u_out,
-               const double* __restrict__ kx, const double* __restrict__ ky)
+k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
+               const T* __restrict__ sd_in, T* __restrict__ sd_out,
+               const T* __restrict__ r_in, T* __restrict__ r_out,
+               const T* __restrict__ u_in, T* __restrict__ u_out,
+               const T* __restrict__ kx, const T* __restrict__ ky)
 {
-    constexpr int SP = S + (S & 1);
+    constexpr int N = Lane16<T>::N;
+    using C = Cells<T>;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
-    const int tile = xcd_tile(blockIdx.x, t.nbx * t.nby);
+    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
     const int bx = tile % t.nbx, by = tile / t.nbx;
     const int sx = bx * kWaves + wave;
     if (sx >= t.ns) return;                      // uniform per wave; no block sync below
 
-    const int xin0 = g.hd, xin1 = g.x - g.hd;   // interior columns
-    const int yin0 = g.hd, yin1 = g.y - g.hd;   // interior rows
+    const int xin0 = g.hd, xin1 = g.hd + g.nx;  // interior columns
+    const int yin0 = g.hd, yin1 = g.hd + g.ny;  // interior rows
     const int o0 = xin0 + sx * t.w;
     const int o1 = min(o0 + t.w, xin1);
-    const int c = o0 - SP + 2 * lane;           // this lane's columns c, c+1
-    const bool ld_ok = c >= 0 && c + 2 <= g.x;
-    const bool out_col = c >= o0 && c < o1;     // both cells or neither
-    // x mirror: a neighbour outside the interior is the cell itself.
-    const bool mir_l0 = c == xin0, mir_l1 = c + 1 == xin0;
-    const bool mir_r0 = c == xin1 - 1, mir_r1 = c + 1 == xin1 - 1;
+    const int c = t.cw0 + sx * t.w + N * lane;  // this lane's first column
+    const bool ld_ok = c >= 0 && c + N <= g.X;
+    const bool all_out = c >= o0 && c + N <= o1;
+    const bool any_out = c + N > o0 && c < o1;
 
     const int j0 = yin0 + by * t.ty;
     const int j1 = min(j0 + t.ty, yin1);
     const int rs = j0 - S + 1;                  // first row of level 1
     const int niter = j1 - j0 + 2 * S - 2;
-    const long X = g.x;
+    const long X = g.X;
 
-    auto ld_field = [&](const double* __restrict__ f, int row, int lo, int hi) -> double2 {
-        if (ld_ok && row >= lo && row < hi) return ld2(f, (long)row * X + c);
-        return make_double2(0.0, 0.0);
+    auto ld_field = [&](const T* __restrict__ f, int row, int lo, int hi) -> C {
+        if (ld_ok && row >= lo && row < hi) return ldc(f, (long)row * X + c);
+        return zeros<T>();
+    };
+    auto over_theta = [&](C a) -> C {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] / theta;
+        return a;
     };
 
     // Windows (index 0 oldest).  sdw[k][*]: rows of level k for k < S.
-    double2 sdw[S][3];
-    double2 rprev[S], uprev[S];     // level k's r, u from the previous iteration
-    double2 kxw[S], kyw[S + 1];     // kx rows rho1 - m; ky rows rho1 + 1 - m
+    C sdw[S][3];
+    C rprev[S], uprev[S];     // level k's r, u from the previous iteration
+    C kxw[S], kyw[S + 1];     // kx rows rho1 - m; ky rows rho1 + 1 - m
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-        sdw[k][0] = sdw[k][1] = sdw[k][2] = make_double2(0.0, 0.0);
-        rprev[k] = uprev[k] = make_double2(0.0, 0.0);
-        kxw[k] = make_double2(0.0, 0.0);
+        sdw[k][0] = sdw[k][1] = sdw[k][2] = zeros<T>();
+        rprev[k] = uprev[k] = zeros<T>();
+        kxw[k] = zeros<T>();
     }
 #pragma unroll
-    for (int m = 0; m <= S; ++m) kyw[m] = make_double2(0.0, 0.0);
+    for (int m = 0; m <= S; ++m) kyw[m] = zeros<T>();
 
     // Prologue: level-0 sd rows rs-1, rs; kx rows below rs; ky rows up to rs.
-    double2 rnext = make_double2(0.0, 0.0);     // r_in row rho1 (FROM_R carries it)
+    C rcarry = zeros<T>();    // FROM_R: r_in row rho1, loaded one iteration ago
     if constexpr (FROM_R) {
-        const double2 a = ld_field(r_in, rs - 1, yin0, yin1);
-        const double2 b = ld_field(r_in, rs, yin0, yin1);
-        sdw[0][1] = make_double2(a.x / theta, a.y / theta);
-        sdw[0][2] = make_double2(b.x / theta, b.y / theta);
-        rnext = b;
+        sdw[0][1] = over_theta(ld_field(r_in, rs - 1, yin0, yin1));
+        rcarry = ld_field(r_in, rs, yin0, yin1);
+        sdw[0][2] = over_theta(rcarry);
     } else {
         sdw[0][1] = ld_field(sd_in, rs - 1, yin0, yin1);
         sdw[0][2] = ld_field(sd_in, rs, yin0, yin1);
@@ -122,76 +109,98 @@ k_ppcg_fused2d(TlGrid2 g, FusedTile t, PpcgCoefs<S> cf, double theta,
 #pragma unroll
     for (int m = 0; m < S; ++m) kyw[m] = ld_field(ky, rs - m, yin0, yin1 + 1);
 
+    // Level-0 inputs of the iteration whose level 1 computes row rho1:
+    // a = sd (or r for FROM_R) at rho1+1, b = r at rho1 (unless FROM_R),
+    // u at rho1, kx at rho1, ky at rho1+1.
+    struct In {
+        C a, b, u, kx, ky;
+    };
+    auto load_in = [&](int rho1) -> In {
+        In v;
+        v.a = ld_field(FROM_R ? r_in : sd_in, rho1 + 1, yin0, yin1);
+        if constexpr (!FROM_R) v.b = ld_field(r_in, rho1, yin0, yin1);
+        v.u = ld_field(u_in, rho1, yin0, yin1);
+        v.kx = ld_field(kx, rho1, yin0, yin1);
+        v.ky = ld_field(ky, rho1 + 1, yin0, yin1 + 1);
+        return v;
+    };
+    In cur;
+    if constexpr (PF) cur = load_in(rs);
+
     for (int it = 0; it < niter; ++it) {
         const int rho1 = rs + it;
-        // level-0 inputs for this iteration
-        double2 sd_new, r0;
-        if constexpr (FROM_R) {
-            r0 = rnext;
-            rnext = ld_field(r_in, rho1 + 1, yin0, yin1);
-            sd_new = make_double2(rnext.x / theta, rnext.y / theta);
+        // With PF the next iteration's loads are issued before this
+        // iteration's arithmetic, which then hides their latency.
+        In nxt;
+        if constexpr (PF) {
+            if (it + 1 < niter) nxt = load_in(rho1 + 1);
+            else nxt = cur;
         } else {
-            r0 = ld_field(r_in, rho1, yin0, yin1);
-            sd_new = ld_field(sd_in, rho1 + 1, yin0, yin1);
+            cur = load_in(rho1);
         }
-        const double2 u0 = ld_field(u_in, rho1, yin0, yin1);
-        const double2 kx_new = ld_field(kx, rho1, yin0, yin1);
-        const double2 ky_new = ld_field(ky, rho1 + 1, yin0, yin1 + 1);
+        C sd_new, r0;
+        if constexpr (FROM_R) {
+            r0 = rcarry;
+            rcarry = cur.a;
+            sd_new = over_theta(cur.a);
+        } else {
+            r0 = cur.b;
+            sd_new = cur.a;
+        }
+        const C u0 = cur.u;
         sdw[0][0] = sdw[0][1];
         sdw[0][1] = sdw[0][2];
         sdw[0][2] = sd_new;
 #pragma unroll
         for (int m = S - 1; m > 0; --m) kxw[m] = kxw[m - 1];
-        kxw[0] = kx_new;
+        kxw[0] = cur.kx;
 #pragma unroll
         for (int m = S; m > 0; --m) kyw[m] = kyw[m - 1];
-        kyw[0] = ky_new;
+        kyw[0] = cur.ky;
+        if constexpr (PF) cur = nxt;
 
-        double2 rcur[S], ucur[S];
+        C rcur[S], ucur[S];
 #pragma unroll
         for (int k = 1; k <= S; ++k) {
             const int rho = rho1 - (k - 1);
-            const double2 cen = sdw[k - 1][1];
-            double2 upv = sdw[k - 1][2], dnv = sdw[k - 1][0];
-            if (rho == yin0) dnv = cen;                 // bottom mirror
-            if (rho == yin1 - 1) upv = cen;             // top mirror
-            // x neighbours: lane-1's second cell, lane+1's first cell
-            double lft0 = tl_lane_prev(cen.y, cen.x);
-            double rgt1 = tl_lane_next(cen.x, cen.y);
-            double lft1 = cen.x, rgt0 = cen.y;
-            if (mir_l0) lft0 = cen.x;
-            if (mir_l1) lft1 = cen.y;
-            if (mir_r0) rgt0 = cen.x;
-            if (mir_r1) rgt1 = cen.y;
-            const double2 kxc = kxw[k - 1];
-            const double kxr0 = kxc.y;
-            const double kxr1 = tl_lane_next(kxc.x, kxc.y);
-            const double2 kyu = kyw[k - 1], kyc = kyw[k];
-            const double2 rin = k == 1 ? r0 : rprev[k - 1];
-            const double2 uin = k == 1 ? u0 : uprev[k - 1];
+            const C cen = sdw[k - 1][1];
+            // y neighbours; outside the interior they mirror the cell
+            const C upv = rho == yin1 - 1 ? cen : sdw[k - 1][2];
+            const C dnv = rho == yin0 ? cen : sdw[k - 1][0];
+            const C kxc = kxw[k - 1];
+            const C kyu = kyw[k - 1], kyc = kyw[k];
+            const C rin = k == 1 ? r0 : rprev[k - 1];
+            const C uin = k == 1 ? u0 : uprev[k - 1];
             // SMVP (2d/shared.h:50-54) and the ppcg.c:46-59 updates, same order
-            const double d0 = (1.0 + (kxr0 + kxc.x)) + (kyu.x + kyc.x);
-            const double d1 = (1.0 + (kxr1 + kxc.y)) + (kyu.y + kyc.y);
-            const double a0 = d0 * cen.x - (kxr0 * rgt0 + kxc.x * lft0) - (kyu.x * upv.x + kyc.x * dnv.x);
-            const double a1 = d1 * cen.y - (kxr1 * rgt1 + kxc.y * lft1) - (kyu.y * upv.y + kyc.y * dnv.y);
-            double2 rn, un, sn;
-            rn.x = rin.x - a0;
-            rn.y = rin.y - a1;
-            un.x = uin.x + cen.x;
-            un.y = uin.y + cen.y;
-            sn.x = cf.alpha[k - 1] * cen.x + cf.beta[k - 1] * rn.x;
-            sn.y = cf.alpha[k - 1] * cen.y + cf.beta[k - 1] * rn.y;
+            const C av = apply_a_lane(cen, upv, dnv, kxc, kyu, kyc, c, xin0, xin1);
+            C rn, un, sn;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                rn.v[i] = rin.v[i] - av.v[i];
+                un.v[i] = uin.v[i] + cen.v[i];
+                sn.v[i] = cf.alpha[k - 1] * cen.v[i] + cf.beta[k - 1] * rn.v[i];
+            }
             rcur[k - 1] = rn;
             ucur[k - 1] = un;
             if (k < S) {
                 sdw[k][0] = sdw[k][1];
                 sdw[k][1] = sdw[k][2];
                 sdw[k][2] = sn;
-            } else if (out_col && rho >= j0 && rho < j1) {
-                const long i = (long)rho * X + c;
-                *reinterpret_cast<double2*>(r_out + i) = rn;
-                *reinterpret_cast<double2*>(u_out + i) = un;
-                *reinterpret_cast<double2*>(sd_out + i) = sn;
+            } else if (any_out && rho >= j0 && rho < j1) {
+                const long i0 = (long)rho * X + c;
+                if (all_out) {
+                    stc(r_out, i0, rn);
+                    stc(u_out, i0, un);
+                    stc(sd_out, i0, sn);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (c + i >= o0 && c + i < o1) {
+                            r_out[i0 + i] = rn.v[i];
+                            u_out[i0 + i] = un.v[i];
+                            sd_out[i0 + i] = sn.v[i];
+                        }
+                }
             }
         }
 #pragma unroll
@@ -202,104 +211,107 @@ k_ppcg_fused2d(TlGrid2 g, FusedTile t, PpcgCoefs<S> cf, double theta,
     }
 }
 
-// Halo ring of up to three fields, src -> dst.
+// Everything outside the interior of up to three fields, src -> dst.
+template <class T>
 struct Ring3 {
-    const double* src[3];
-    double* dst[3];
+    const T* src[3];
+    T* dst[3];
     int n;
 };
 
-__global__ void __launch_bounds__(kBlock) k_copy_rings2d(TlGrid2 g, Ring3 rg)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_copy_rings2d(PitchGrid g, int rows, Ring3<T> rg)
 {
-    const long nrow = 2L * g.hd * g.x;
-    const long ncol = 2L * g.hd * (g.y - 2 * g.hd);
+    // rows [0,hd) and [hd+ny,rows) whole; rows [hd,hd+ny) outside [hd,hd+nx)
+    const int side = g.X - g.nx;
+    const long nrow = (long)(rows - g.ny) * g.X;
+    const long ncol = (long)side * g.ny;
     const long per = nrow + ncol;
     for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < per * rg.n;
          t += (long)gridDim.x * kBlock) {
         const int f = (int)(t / per);
         const long q = t % per;
-        long c;
+        long cell;
         if (q < nrow) {
-            const int rr = (int)(q / g.x), kk = (int)(q % g.x);
-            const int jj = rr < g.hd ? rr : g.y - 2 * g.hd + rr;
-            c = (long)jj * g.x + kk;
+            const int rr = (int)(q / g.X), kk = (int)(q % g.X);
+            const int jj = rr < g.hd ? rr : g.ny + rr;
+            cell = (long)jj * g.X + kk;
         } else {
-            const long u = q - nrow;
-            const int jj = g.hd + (int)(u / (2 * g.hd));
-            const int p = (int)(u % (2 * g.hd));
-            const int kk = p < g.hd ? p : g.x - 2 * g.hd + p;
-            c = (long)jj * g.x + kk;
+            const long v = q - nrow;
+            const int jj = g.hd + (int)(v / side);
+            const int p = (int)(v % side);
+            const int kk = p < g.hd ? p : g.nx + p;
+            cell = (long)jj * g.X + kk;
         }
-        rg.dst[f][c] = rg.src[f][c];
+        rg.dst[f][cell] = rg.src[f][cell];
     }
 }
 
-int g_fused_ty = -1;
+int g_fused_ty = -1;   // output rows per tile (<= 0: default)
+int g_fused_pf = -1;   // row prefetch (< 0: default)
 
-template <int S, bool FROM_R>
-int launch_fused(const TlGrid2& g, const double* al, const double* be, double theta,
-                 const double* sd_in, double* sd_out, const double* r_in, double* r_out,
-                 const double* u_in, double* u_out, const double* kx, const double* ky,
-                 const char* what)
+template <class T, int S, bool FROM_R>
+int launch_fused(const PitchGrid& g, const double* al, const double* be, double theta,
+                 const T* sd_in, T* sd_out, const T* r_in, T* r_out, const T* u_in, T* u_out,
+                 const T* kx, const T* ky, const char* what)
 {
-    constexpr int SP = S + (S & 1);
-    FusedTile t;
-    const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd;
-    t.w = TL_WAVE * 2 - 2 * SP;
-    t.ns = (int)tl_cdiv(nx, t.w);
-    t.nbx = (int)tl_cdiv(t.ns, kWaves);
-    t.ty = g_fused_ty > 0 ? g_fused_ty : 32;
-    t.nby = (int)tl_cdiv(ny, t.ty);
-    PpcgCoefs<S> cf;
+    // Windows reach S columns beyond their output columns on both sides.
+    // Default tile heights from a sweep at 4096^2 (S = 5, prefetch on).
+    const int ty = g_fused_ty > 0 ? g_fused_ty : (sizeof(T) == 8 ? 96 : 48);
+    const StripTile t = strip_tile<T>(g, S, ty, kWaves);
+    if ((long)t.nbx * t.nby > (1L << 30)) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    PpcgCoefs<T, S> cf;
     for (int k = 0; k < S; ++k) {
-        cf.alpha[k] = al[k];
-        cf.beta[k] = be[k];
+        cf.alpha[k] = (T)al[k];
+        cf.beta[k] = (T)be[k];
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
     const int slot = tl_timer_begin(what);
-    hipLaunchKernelGGL((k_ppcg_fused2d<S, FROM_R>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t,
-                       cf, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky);
+    if (g_fused_pf != 0)
+        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, FROM_R, true>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, cf, (T)theta, sd_in, sd_out, r_in, r_out, u_in,
+                           u_out, kx, ky);
+    else
+        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, FROM_R, false>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, cf, (T)theta, sd_in, sd_out, r_in, r_out, u_in,
+                           u_out, kx, ky);
     tl_timer_end(slot);
     return tl_check_launch(what);
 }
 
-template <bool FROM_R>
-int dispatch_fused(int count, const TlGrid2& g, const double* al, const double* be, double theta,
-                   const double* sd_in, double* sd_out, const double* r_in, double* r_out,
-                   const double* u_in, double* u_out, const double* kx, const double* ky,
-                   const char* what)
+template <class T, bool FROM_R>
+int dispatch_fused(int count, const PitchGrid& g, const double* al, const double* be,
+                   double theta, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
+                   const T* u_in, T* u_out, const T* kx, const T* ky, const char* what)
 {
+#define TL_FUSED_CASE(S)                                                                       \
+    case S:                                                                                    \
+        return launch_fused<T, S, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, \
+                                          u_out, kx, ky, what);
     switch (count) {
-    case 1: return launch_fused<1, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, what);
-    case 2: return launch_fused<2, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, what);
-    case 3: return launch_fused<3, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, what);
-    case 4: return launch_fused<4, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, what);
-    case 5: return launch_fused<5, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, what);
+        TL_FUSED_CASE(1)
+        TL_FUSED_CASE(2)
+        TL_FUSED_CASE(3)
+        TL_FUSED_CASE(4)
+        TL_FUSED_CASE(5)
     default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", what, kMaxFused);
     }
+#undef TL_FUSED_CASE
 }
 
-}  // namespace
-
-extern "C" int tlhip_set_fused_tile_rows(int rows)
+template <class T>
+int inner_iterations(const char* what, int x, int y, int hd, int pitch, int count,
+                     const double* alphas, const double* betas, double theta, int init_from_r,
+                     const int* neighbours, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
+                     const T* u_in, T* u_out, const T* kx, const T* ky)
 {
-    g_fused_ty = rows;
-    return 0;
-}
-
-extern "C" int tlhip2d_ppcg_inner_iterations(int x, int y, int hd, int count,
-                                             const double* alphas, const double* betas,
-                                             double theta, int init_from_r,
-                                             const int* neighbours, const double* sd_in,
-                                             double* sd_out, const double* r_in, double* r_out,
-                                             const double* u_in, double* u_out,
-                                             const double* kx, const double* ky)
-{
-    const char* what = "tlhip2d_ppcg_inner_iterations";
+    constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
-    const TlGrid2 g{x, y, hd};
-    if (hd < 2 || hd % 2 || x % 2 || x < 2 * hd + 1 || y < 2 * hd + 1 || (long)x * y > (1L << 31) - 1)
-        return tl_fail(TLHIP_ERR_ARG, "%s: needs even x, even halo_depth >= 2 (x=%d hd=%d)", what, x, hd);
+    if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || pitch < x || pitch % N ||
+        (long)pitch * y > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG,
+                       "%s: needs a row pitch that is a multiple of %d and >= x (x=%d pitch=%d "
+                       "y=%d hd=%d)", what, N, x, pitch, y, hd);
     if (count < 1 || count > kMaxFused)
         return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d (got %d)", what, kMaxFused, count);
     TL_REQUIRE(alphas && betas && neighbours, what, "null coefficients/neighbours");
@@ -311,8 +323,9 @@ extern "C" int tlhip2d_ppcg_inner_iterations(int x, int y, int hd, int count,
         TL_REQUIRE(neighbours[f] == -1, what, "every face must be external (single chunk)");
     TL_REQUIRE(all_aligned(sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky), what,
                "fields must be 16-byte aligned");
-    if (!init_from_r && theta == 0.0) theta = 1.0;
-    Ring3 rg{};
+    if (init_from_r && theta == 0.0) return tl_fail(TLHIP_ERR_ARG, "%s: theta is zero", what);
+    const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
+    Ring3<T> rg{};
     rg.src[0] = r_in;  rg.dst[0] = r_out;
     rg.src[1] = u_in;  rg.dst[1] = u_out;
     rg.n = 2;
@@ -321,12 +334,48 @@ extern "C" int tlhip2d_ppcg_inner_iterations(int x, int y, int hd, int count,
         rg.dst[2] = sd_out;
         rg.n = 3;
     }
-    const long ring = (2L * hd * x + 2L * hd * (y - 2 * hd)) * rg.n;
-    hipLaunchKernelGGL(k_copy_rings2d, dim3(flat_blocks(ring)), dim3(kBlock), 0, tl_stream(), g, rg);
+    const long ring = ((long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny) * rg.n;
+    hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0, tl_stream(), g,
+                       y, rg);
     TL_TRY(tl_check_launch(what));
     if (init_from_r)
-        return dispatch_fused<true>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out,
+        return dispatch_fused<T, true>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out,
+                                       u_in, u_out, kx, ky, what);
+    return dispatch_fused<T, false>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out,
                                     u_in, u_out, kx, ky, what);
-    return dispatch_fused<false>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out, u_in,
-                                 u_out, kx, ky, what);
+}
+
+}  // namespace
+
+extern "C" int tlhip_set_fused_tuning(int tile_rows, int prefetch)
+{
+    g_fused_ty = tile_rows;
+    g_fused_pf = prefetch;
+    return 0;
+}
+
+extern "C" int tlhip2d_ppcg_inner_iterations(int x, int y, int hd, int count,
+                                             const double* alphas, const double* betas,
+                                             double theta, int init_from_r,
+                                             const int* neighbours, const double* sd_in,
+                                             double* sd_out, const double* r_in, double* r_out,
+                                             const double* u_in, double* u_out,
+                                             const double* kx, const double* ky)
+{
+    return inner_iterations<double>("tlhip2d_ppcg_inner_iterations", x, y, hd, x, count, alphas,
+                                    betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
+                                    r_out, u_in, u_out, kx, ky);
+}
+
+extern "C" int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int hd, int pitch, int count,
+                                                 const double* alphas, const double* betas,
+                                                 double theta, int init_from_r,
+                                                 const int* neighbours, const float* sd_in,
+                                                 float* sd_out, const float* r_in, float* r_out,
+                                                 const float* u_in, float* u_out,
+                                                 const float* kx, const float* ky)
+{
+    return inner_iterations<float>("tlhip2d_ppcg_inner_iterations_f32", x, y, hd, pitch, count,
+                                   alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
+                                   r_in, r_out, u_in, u_out, kx, ky);
 }

@@ -131,6 +131,7 @@ class CStepReport(ctypes.Structure):
         ("summary_done", ctypes.c_int),
         ("vol", ctypes.c_double), ("mass", ctypes.c_double), ("ie", ctypes.c_double),
         ("temp", ctypes.c_double),
+        ("rel_residual", ctypes.c_double), ("n_true_residuals", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -298,14 +299,15 @@ use_c_kernels
 check_result
 eps = {eps}
 max_iters = {max_iters}
-initial_timestep = 0.004
+initial_timestep = {dt}
 end_step = {steps}
 end_time = 100.0
 halo_depth = 2
 num_chunks_per_rank = 1
 ppcg_inner_steps = {inner}
-epslim = 0.0001
+epslim = {epslim}
 presteps = {presteps}
+{errswitch}
 *endtea
 """
 
@@ -341,10 +343,16 @@ _SOLVER_KEY = {CG: "use_cg", CHEBY: "use_chebyshev", PPCG: "use_ppcg", JACOBI: "
 
 
 def layered_2d(n: int, steps: int, solver: int = CG, eps: float = 1e-15,
-               max_iters: int = 10000, inner: int = 10, presteps: int = 30) -> str:
-    """The reference's layered-rectangle deck (2d/tea.in) at n x n cells."""
+               max_iters: int = 10000, inner: int = 10, presteps: int = 30,
+               dt: float = 0.004, epslim: float | None = None) -> str:
+    """The reference's layered-rectangle deck (2d/tea.in) at n x n cells.
+    With `epslim` the deck selects the reference's error switch
+    (tl_use_errswitch): Chebyshev/PPCG start once the CG error is below
+    epslim and more than 20 CG iterations have run."""
     return LAYERED_2D.format(n=n, steps=steps, solver=_SOLVER_KEY[solver], eps=eps,
-                             max_iters=max_iters, inner=inner, presteps=presteps)
+                             max_iters=max_iters, inner=inner, presteps=presteps, dt=dt,
+                             epslim=0.0001 if epslim is None else epslim,
+                             errswitch="" if epslim is None else "errswitch")
 
 
 def cuboid_3d(n: int, steps: int, solver: int = CG, eps: float = 1e-15,

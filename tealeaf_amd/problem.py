@@ -46,6 +46,13 @@ class Problem:
             raise ValueError(f"expected {self.n} cells, got {d.size} / {e.size}")
         lib().tlhip_problem_inject(self.handle, d.ctypes.data, e.ctypes.data)
 
+    def set_solve(self, precision: int = 0, rel_residual: float = 0.0) -> None:
+        """precision 0: FP64 (the reference's arithmetic); 1: FP32 CG/PPCG
+        sweeps inside an FP64 residual-replacement loop (2D).  rel_residual
+        > 0 stops on ||u0 - A u|| <= rel_residual ||u0|| (FP64-confirmed)
+        instead of the deck's eps."""
+        lib().tlhip_problem_set_solve(self.handle, int(precision), float(rel_residual))
+
     def step(self, tt: int) -> dict:
         rep = tdeck.CStepReport()
         lib().tlhip_problem_step(self.handle, int(tt), ctypes.addressof(rep))

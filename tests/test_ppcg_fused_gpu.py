@@ -7,7 +7,12 @@ a depth-1 reflective halo update of sd (2d/c_kernels/omp3/local_halos.c)
 then ppcg_inner_iteration (2d/c_kernels/omp3/ppcg.c:28-61), optionally
 preceded by ppcg_init (ppcg.c:8-25).  The halo ring of r and u passes
 through unchanged.  The cases cover partial x windows and y tiles, tiles
-shorter than the step count, grids narrower than one window and halo depth 4.
+shorter than the step count, grids narrower than one window, odd halo
+depths (lanes that straddle the interior edge) and both loop schedules.
+
+The FP32 variant (the mixed-precision solve's inner sweeps) is held to the
+same per-step arithmetic done in numpy float32: IEEE single precision, no
+fused multiply-add, same association -- so it too must match bit for bit.
 """
 import numpy as np
 import pytest
@@ -20,7 +25,7 @@ pytestmark = pytest.mark.gpu
 # (nx, ny, hd, tile_rows): nx even; windows are 128 - 2*SP wide (SP = S
 # rounded up to even), tiles `tile_rows` high (0 = default).
 GRIDS = [(32, 32, 2, 0), (250, 70, 2, 0), (300, 129, 2, 8), (6, 5, 2, 3),
-         (124, 3, 2, 0), (118, 41, 4, 2), (460, 17, 2, 1)]
+         (124, 3, 2, 0), (118, 41, 4, 2), (460, 17, 2, 1), (64, 20, 1, 0), (122, 33, 3, 4)]
 
 
 def _oracle_steps(O, x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
@@ -51,7 +56,8 @@ def _ring(a, x, y, hd):
 @pytest.mark.parametrize("grid", GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in GRIDS])
 @pytest.mark.parametrize("count", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
-def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r):
+@pytest.mark.parametrize("pf", [1, 0], ids=["pf", "nopf"])
+def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r, pf):
     from tealeaf_amd.device import DeviceArray
     nx, ny, hd, ty = grid
     x, y = nx + 2 * hd, ny + 2 * hd
@@ -72,14 +78,14 @@ def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r):
          dict(sd=sd, r=r, u=u, kx=kx, ky=ky).items()}
     out = {k: DeviceArray.from_host(np.full(n, np.nan)) for k in ("sd", "r", "u")}
     nb = np.full(4, -1, dtype=np.int32)
-    hip.tlhip_set_fused_tile_rows(ty if ty else -1)
+    hip.tlhip_set_fused_tuning(ty if ty else -1, pf)
     try:
         hip.tlhip2d_ppcg_inner_iterations(
             x, y, hd, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
             nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
             out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr)
     finally:
-        hip.tlhip_set_fused_tile_rows(-1)
+        hip.tlhip_set_fused_tuning(-1, -1)
     hip.tlhip_synchronize()
     got = {k: v.download() for k, v in out.items()}
     for name, want, src in (("r", want_r, r), ("u", want_u, u), ("sd", want_sd, sd)):
@@ -103,7 +109,7 @@ def test_fused_rejects_bad_layout(hip):
     p = [b.ptr for b in bufs]
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 1,
                                            nb.ctypes.data, None, *p[:7])
-    assert rc != 0 and "even" in hip.last_error()
+    assert rc != 0 and "pitch" in hip.last_error()
     x = 16                               # in-place output
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[0], p[1], p[2], p[3], p[4],
@@ -151,3 +157,78 @@ def test_set_ppcg_fusion_range(hip):
     assert hip.dll.tlhip_set_ppcg_fusion(6) != 0
     assert hip.dll.tlhip_set_ppcg_fusion(-1) != 0
     assert hip.dll.tlhip_set_ppcg_fusion(5) == 0
+
+
+def _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
+    """The reference's per-step expressions in numpy float32 on (rows, pitch) arrays."""
+    f = np.float32
+    sd, r, u = sd.copy(), r.copy(), u.copy()
+    J, K = slice(hd, y - hd), slice(hd, x - hd)
+    if from_r:
+        sd[J, K] = r[J, K] / f(theta)
+    for a, b in zip(alphas, betas):
+        a, b = f(a), f(b)
+        s = sd.copy()                                     # depth-1 reflective halo
+        s[J, hd - 1] = s[J, hd]
+        s[J, x - hd] = s[J, x - hd - 1]
+        s[hd - 1, K] = s[hd, K]
+        s[y - hd, K] = s[y - hd - 1, K]
+        cen = s[J, K]
+        lft, rgt = s[J, hd - 1:x - hd - 1], s[J, hd + 1:x - hd + 1]
+        dn, up = s[hd - 1:y - hd - 1, K], s[hd + 1:y - hd + 1, K]
+        kxc, kxr = kx[J, K], kx[J, hd + 1:x - hd + 1]
+        kyc, kyu = ky[J, K], ky[hd + 1:y - hd + 1, K]
+        d = (f(1) + (kxr + kxc)) + (kyu + kyc)
+        av = d * cen - (kxr * rgt + kxc * lft) - (kyu * up + kyc * dn)
+        r[J, K] = r[J, K] - av
+        u[J, K] = u[J, K] + cen
+        sd[J, K] = a * cen + b * r[J, K]
+    return sd, r, u
+
+
+F32_GRIDS = [(32, 30, 2, 0), (250, 70, 2, 0), (510, 37, 2, 4), (6, 5, 2, 0), (61, 9, 1, 0),
+             (1000, 21, 3, 8), (248, 17, 2, 2)]
+
+
+@pytest.mark.parametrize("grid", F32_GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in F32_GRIDS])
+@pytest.mark.parametrize("count", [1, 3, 5])
+@pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
+@pytest.mark.parametrize("pad", [0, 4])
+def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, hd, ty = grid
+    x, y = nx + 2 * hd, ny + 2 * hd
+    pitch = (x + 3) // 4 * 4 + pad
+    rng = np.random.default_rng(7000 + nx + 10 * count + pad)
+    shape = (y, pitch)
+    f32 = lambda lo, hi: rng.uniform(lo, hi, shape).astype(np.float32)  # noqa: E731
+    sd, r, u = f32(-1, 1), f32(-1, 1), f32(-1, 1)
+    kx, ky = f32(0.05, 0.3), f32(0.05, 0.3)
+    alphas = rng.uniform(0.1, 0.9, count)
+    betas = rng.uniform(0.1, 0.9, count)
+    theta = 1.7
+    want = dict(zip(("sd", "r", "u"), _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r,
+                                                  u, kx, ky)))
+    src = dict(sd=sd, r=r, u=u)
+    d = {k: DeviceArray.from_host(v) for k, v in dict(sd=sd, r=r, u=u, kx=kx, ky=ky).items()}
+    out = {k: DeviceArray.from_host(np.full(shape, np.nan, dtype=np.float32)) for k in ("sd", "r", "u")}
+    nb = np.full(4, -1, dtype=np.int32)
+    hip.tlhip_set_fused_tuning(ty if ty else -1, -1)
+    try:
+        hip.tlhip2d_ppcg_inner_iterations_f32(
+            x, y, hd, pitch, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
+            nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
+            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr)
+    finally:
+        hip.tlhip_set_fused_tuning(-1, -1)
+    hip.tlhip_synchronize()
+    J, K = slice(hd, y - hd), slice(hd, x - hd)
+    ring = np.ones(shape, dtype=bool)
+    ring[J, K] = False
+    for name in ("r", "u", "sd"):
+        g = out[name].download().reshape(shape)
+        np.testing.assert_array_equal(g[J, K], want[name][J, K], err_msg=f"{name} interior")
+        if name != "sd" or not from_r:
+            np.testing.assert_array_equal(g[ring], src[name][ring], err_msg=f"{name} halo ring")
+    for v in list(d.values()) + list(out.values()):
+        v.free()
