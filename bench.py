@@ -60,8 +60,11 @@ def parse_args():
     ap.add_argument("--refine-cells", type=int, default=8192,
                     help="grid of the FP32-refined vs FP64 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--refine-steps", type=int, default=2)
+    ap.add_argument("--refine-only", action="store_true",
+                    help="run only the FP32-refined vs FP64 comparison and print its JSON")
     ap.add_argument("--profile-run", action="store_true",
-                    help="short run for counter collection: no CPU baseline, no JSON extras")
+                    help="counter-collection run: a few launches of each hot kernel at --cells "
+                         "and of a copy kernel of known size, then exit")
     return ap.parse_args()
 
 
@@ -186,6 +189,55 @@ def refine_run(args):
             "target_ratio": 0.6, **out}
 
 
+def profile_run(args):
+    """Three launches each of the hot kernels on --cells^2 synthetic fields,
+    and of a 1 GiB device copy (8 B read + 8 B written per element) whose
+    FETCH_SIZE / WRITE_SIZE calibrate the counters; for rocprofv3 --pmc."""
+    import ctypes as C
+    import numpy as np
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.device import DeviceArray
+
+    L = lib()
+    L.tlhip_init(0)
+    n, hd = args.cells, 2
+    x = y = n + 2 * hd
+    N, P = x * y, (x + 3) // 4 * 4
+    rng = np.random.default_rng(0)
+    f64 = {k: DeviceArray.from_host(rng.uniform(-1, 1, N)) for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
+    for k in ("kx", "ky"):
+        f64[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, N))
+    f32 = {k: DeviceArray.from_host(rng.uniform(-1, 1, P * y).astype(np.float32))
+           for k in ("u", "u2", "r", "r2", "sd", "sd2")}
+    for k in ("kx", "ky"):
+        f32[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, P * y).astype(np.float32))
+    ncopy = (1 << 30) // 8
+    src, dst = DeviceArray(ncopy, np.float64), DeviceArray(ncopy, np.float64)
+    nb = np.full(4, -1, dtype=np.int32)
+    al, be = np.full(5, 0.7), np.full(5, 0.2)
+    pw = C.c_double()
+    for _ in range(3):
+        L.tlhip_stream_copy(dst.ptr, src.ptr, C.c_size_t(ncopy))
+        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
+                                        nb.ctypes.data, f64["sd"].ptr, f64["sd2"].ptr, f64["r"].ptr,
+                                        f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
+                                        f64["ky"].ptr)
+        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 1,
+                                        nb.ctypes.data, None, f64["sd2"].ptr, f64["r"].ptr,
+                                        f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
+                                        f64["ky"].ptr)
+        L.tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
+                                            nb.ctypes.data, f32["sd"].ptr, f32["sd2"].ptr,
+                                            f32["r"].ptr, f32["r2"].ptr, f32["u"].ptr,
+                                            f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr)
+        L.tlhip2d_ppcg_inner_iteration(x, y, hd, 0.7, 0.2, f64["u"].ptr, f64["r"].ptr, f64["kx"].ptr,
+                                       f64["ky"].ptr, f64["sd"].ptr, f64["sd2"].ptr)
+        L.tlhip2d_cg_calc_w(x, y, hd, C.addressof(pw), f64["p"].ptr, f64["w"].ptr, f64["kx"].ptr,
+                            f64["ky"].ptr)
+    L.tlhip_synchronize()
+    print(json.dumps({"profile_run": True, "cells": n, "copy_elements": ncopy}), flush=True)
+
+
 def dominant(args) -> str:
     return "tlhip2d_ppcg_inner_iterations" if args.ppcg_fusion > 1 else "tlhip2d_ppcg_inner_iteration"
 
@@ -221,14 +273,21 @@ def cpu_baseline(cells: int, iters: int):
                       f"OMP_NUM_THREADS={cores}"}
 
 
-def load_traffic(kernel: str):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel)."""
+def load_traffic(kernel: str, init_share: float = 0.0):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size).
+    For the fused PPCG loop, launches that start from r (ppcg_init folded in)
+    and launches that continue from sd are weighted by their share."""
     path = ROOT / "profiles" / "ppcg_4096_traffic.json"
     if not path.exists():
         return None
     try:
-        return json.loads(path.read_text())["kernels"][kernel]["bytes_per_launch"]
+        kernels = json.loads(path.read_text())["kernels"]
+        if kernel == "tlhip2d_ppcg_inner_iterations":
+            a = kernels["tlhip2d_ppcg_inner_iterations (from r)"]["bytes_per_launch"]
+            b = kernels["tlhip2d_ppcg_inner_iterations"]["bytes_per_launch"]
+            return init_share * a + (1.0 - init_share) * b
+        return kernels[kernel]["bytes_per_launch"]
     except Exception:
         return None
 
@@ -239,6 +298,12 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
               "is a design, not a build (DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.profile_run:
+        profile_run(args)
+        return
+    if args.refine_only:
+        print(json.dumps({"refine": refine_run(args)}), flush=True)
+        return
     res = gpu_run(args)
     refine = refine_run(args) if args.refine_cells > 0 else None
     value = res["iters"] / res["time"]
@@ -247,16 +312,26 @@ def main():
     k = res["kernels"].get(dom)
     roofline = None
     if k and k["launches"]:
-        achieved = k["gbs"]
-        sweeps = res["by_solver"]["ppcg"]["sweeps"]["ppcg_inner"]
-        per_launch = sweeps / k["launches"]
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved,
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": load_traffic(dom), "algorithmic_bytes": BYTES_PER_CELL[dom] * n_int,
-                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"],
-                    "inner_steps_per_launch": per_launch,
-                    # the rate one-step-per-launch sweeps would need to match it
-                    "step_equivalent_gbs": achieved * per_launch}
+        sw = res["by_solver"]["ppcg"]["sweeps"]
+        t = k["avg_ms"] * 1e-3
+        if dom == "tlhip2d_ppcg_inner_iterations":
+            # the survey's per-unit figures: 64 B/cell per inner step, plus
+            # 16 B/cell for the ppcg_init each outer iteration's first launch does
+            alg_cell = (64.0 * sw["ppcg_inner"] + 16.0 * sw["ppcg_outer"]) / k["launches"]
+            init_share = sw["ppcg_outer"] / k["launches"]
+        else:
+            alg_cell, init_share = 64.0, 0.0
+        alg = alg_cell * n_int
+        achieved = alg / t / 1e9
+        traffic = load_traffic(dom, init_share)
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                    "algorithmic_bytes": alg, "avg_kernel_ms": k["avg_ms"],
+                    "launches": k["launches"],
+                    "inner_steps_per_launch": sw["ppcg_inner"] / k["launches"],
+                    # what the launch itself moves through HBM (counters), and at what rate
+                    "hbm_gbs": traffic / t / 1e9 if traffic else None,
+                    "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None}
     line = {
         "metric": "solver iters/s + HBM GB/s, 2D 4096²; FP32-refined vs FP64 time-to-1e-10",
         "value": value,
@@ -275,7 +350,7 @@ def main():
                                "(10 inner, presteps 30); one step = one timestep of each",
                    "cells": n_int, "ppcg_fusion": args.ppcg_fusion, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
         "roofline": roofline,
-        "cpu_baseline": None if (args.no_cpu or args.profile_run) else cpu_baseline(args.cells, args.cpu_iters),
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cells, args.cpu_iters),
         "refine": refine,
         "by_solver": res["by_solver"],
         "kernels": res["kernels"],

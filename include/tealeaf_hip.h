@@ -349,6 +349,20 @@ int tlhip_set_ppcg_fusion(int steps);
  * With precision 1 and rel_residual 0 the eps test applies to the true
  * ||u0 - A u||^2. */
 int tlhip_problem_set_solve(tlhip_problem* p, int precision, double rel_residual);
+/* Reverse-mode gradient through `steps` implicit timesteps (no counterpart
+ * in the reference, which only runs forward).  Runs the steps from the
+ * current state as tlhip_problem_step does (FP64 solve; the state
+ * advances; reports[i] filled when reports is not NULL), keeping u after
+ * each solve, then returns
+ *   *loss = sum over the interior of (u_N - u_target)^2 and
+ *   grad_log_density[c] = dL/d ln(density[c]) (0 on the halo),
+ * through the dependence of the conduction coefficients (2d/c_kernels/omp3/
+ * cg.c:49-63, 3d .../cg_solver.c) and of u_0 = energy0 * density on the
+ * density.  Each backward step solves A mu = lambda with the forward
+ * solver and stopping test.  u_target and grad_log_density are host arrays
+ * of the whole chunk; u, u0, r, p and w are overwritten. */
+int tlhip_problem_adjoint(tlhip_problem* p, int steps, const double* u_target,
+        double* loss, double* grad_log_density, tlhip_step_report* reports);
 /* Allocate a chunk on the device and initialise it as
  * 2d/initialise.c:11-31 does (set_chunk_data, set_chunk_state, halo prime,
  * store_energy). */

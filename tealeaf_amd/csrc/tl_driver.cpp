@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "tealeaf_hip.h"
+#include "tl_adjoint.h"
 #include "tl_mixed.h"
 
 int tl_fail(int code, const char* fmt, ...);
@@ -646,6 +647,91 @@ struct tlhip_problem {
         return 0;
     }
 
+    // -------------------------------------------------------- adjoint
+    // CG on A x = b from x = b (A is symmetric, so the adjoint of a solve is
+    // a solve), leaving x in u.  Uses the solver scratch fields u, u0, r, p,
+    // w and the forward solves' stopping test.
+    int adjoint_solve(const double* b, tlhip_step_report* report)
+    {
+        rep = report;
+        const size_t bytes = sizeof(double) * (size_t)n;
+        TRY(tlhip_copy_device(u0, b, bytes));
+        TRY(tlhip_copy_device(u, b, bytes));
+        reset_fields();
+        fields[F_U] = true;
+        TRY(halo_update(1));
+        TRY(residual());
+        TRY(tlhip_copy_device(p, r, bytes));
+        reset_fields();
+        fields[F_U] = fields[F_P] = true;
+        TRY(halo_update(1));
+        double rro = 0.0;
+        TRY(norm2(r, &rro));
+        TRY(start_solve());
+        double error = rro;
+        int tt;
+        for (tt = 0; tt < s.max_iters; ++tt) {
+            TRY(cg_main_step(tt, &rro, &error));
+            TRY(halo_update(1));
+            bool done;
+            TRY(stop(&error, &done));
+            if (done) break;
+        }
+        report->printed_cg_iters = tt;
+        report->error = error;
+        return 0;
+    }
+
+    // Gradient of L = sum_interior (u_N - target)^2 with respect to ln(rho)
+    // per cell, u_N being u after `steps` timesteps from the current state.
+    // The forward steps are ordinary steps (the state advances) and keep u
+    // after each solve; the backward sweep solves A mu = lambda per step and
+    // scatters the face terms onto the cells (tl_adjoint.hip).
+    int adjoint(int steps, const double* target_h, double* loss, double* grad_h,
+                tlhip_step_report* reports)
+    {
+        if (steps < 1) return tl_fail(TLHIP_ERR_ARG, "adjoint: steps must be >= 1");
+        if (precision != 0) return tl_fail(TLHIP_ERR_ARG, "adjoint: needs the FP64 solve");
+        const size_t bytes = sizeof(double) * (size_t)n;
+        int rc = 0;
+        std::vector<double*> ck(steps, nullptr);
+        for (auto& c : ck) c = dalloc(n, &rc);
+        double* target = dalloc(n, &rc);
+        double* lambda = dalloc(n, &rc);
+        double* grad = dalloc(n, &rc);
+        double* b0 = dalloc(n, &rc);
+        auto body = [&]() -> int {
+            TRY(rc);
+            TRY(tlhip_copy_to_device(target, target_h, bytes));
+            for (int t = 0; t < steps; ++t) {
+                tlhip_step_report tmp;
+                TRY(step(t, reports ? &reports[t] : &tmp));
+                if (t == 0) TRY(tlhip_copy_device(b0, u0, bytes));   // e_0 rho
+                TRY(tlhip_copy_device(ck[t], u, bytes));            // u_{t+1}
+            }
+            const TlGrid3 g{x, y, z, hd};
+            TRY(tl::adj::loss_seed(g, ck[steps - 1], target, lambda, loss));
+            TRY(tl::adj::zero(grad, n));
+            const double dt = s.dt_init;
+            const double rx = dt / (dx * dx), ry = dt / (dy * dy);
+            const double rz = dims == 3 ? dt / (dz * dz) : 0.0;
+            for (int k = steps - 1; k >= 0; --k) {
+                tlhip_step_report tmp{};
+                TRY(adjoint_solve(lambda, &tmp));                   // mu_k in u
+                TRY(tl::adj::face_grad(g, s.coefficient, rx, ry, rz, u, ck[k], density, grad));
+                if (k == 0) TRY(tl::adj::rhs_grad(g, u, b0, grad));
+                else TRY(tlhip_copy_device(lambda, u, bytes));
+            }
+            return tlhip_copy_to_host(grad_h, grad, bytes);
+        };
+        const int out = body();
+        for (double* c : ck)
+            if (c) (void)tlhip_free(c);
+        for (double* a : {target, lambda, grad, b0})
+            if (a) (void)tlhip_free(a);
+        return out;
+    }
+
     // -------------------------------------------------------- lifecycle
     // initialise.c:24-30: prime halos of density/energy0/energy, store energy.
     int prime()
@@ -743,6 +829,17 @@ extern "C" int tlhip_problem_set_solve(tlhip_problem* P, int precision, double r
     P->precision = precision;
     P->rel_tol = rel_residual;
     return 0;
+}
+
+extern "C" int tlhip_problem_adjoint(tlhip_problem* P, int steps, const double* u_target,
+                                     double* loss, double* grad_log_density,
+                                     tlhip_step_report* reports)
+{
+    if (!P || !u_target || !loss || !grad_log_density)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_adjoint: null argument");
+    *loss = 0.0;
+    TRY(P->adjoint(steps, u_target, loss, grad_log_density, reports));
+    return tlhip_synchronize();
 }
 
 extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* states,

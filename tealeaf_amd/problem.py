@@ -53,6 +53,19 @@ class Problem:
         instead of the deck's eps."""
         lib().tlhip_problem_set_solve(self.handle, int(precision), float(rel_residual))
 
+    def adjoint(self, steps: int, target: np.ndarray):
+        """Run `steps` timesteps and return (loss, dloss/dln(density), reports)
+        for loss = sum over the interior of (u_N - target)^2."""
+        t = np.ascontiguousarray(target, dtype=np.float64).ravel()
+        if t.size != self.n:
+            raise ValueError(f"expected {self.n} cells, got {t.size}")
+        grad = np.zeros(self.n)
+        loss = ctypes.c_double()
+        reps = (tdeck.CStepReport * steps)()
+        lib().tlhip_problem_adjoint(self.handle, int(steps), t.ctypes.data, ctypes.addressof(loss),
+                                    grad.ctypes.data, ctypes.addressof(reps))
+        return loss.value, grad, [r.as_dict() for r in reps]
+
     def step(self, tt: int) -> dict:
         rep = tdeck.CStepReport()
         lib().tlhip_problem_step(self.handle, int(tt), ctypes.addressof(rep))
