@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
 BYTES_PER_CELL = {
     "tlhip2d_ppcg_inner_iteration": 64,   # read sd, kx, ky, r, u; write r, u, sd'
     "tlhip2d_ppcg_inner_iterations": 64,  # the same arrays once for S fused inner steps
+    "tlhip2d_cg_calc_pw": 48,             # read p, r, kx, ky; write p', w
     "tlhip2d_cg_calc_w": 32,              # read p, kx, ky; write w
     "tlhip2d_cg_calc_ur": 48,             # read u, p, r, w; write u, r
     "tlhip2d_cg_calc_p": 24,              # read p, r; write p
@@ -204,7 +205,8 @@ def profile_run(args):
     x = y = n + 2 * hd
     N, P = x * y, (x + 3) // 4 * 4
     rng = np.random.default_rng(0)
-    f64 = {k: DeviceArray.from_host(rng.uniform(-1, 1, N)) for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
+    f64 = {k: DeviceArray.from_host(rng.uniform(-1, 1, N))
+           for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
     for k in ("kx", "ky"):
         f64[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, N))
     f32 = {k: DeviceArray.from_host(rng.uniform(-1, 1, P * y).astype(np.float32))
@@ -221,15 +223,18 @@ def profile_run(args):
         L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
                                         nb.ctypes.data, f64["sd"].ptr, f64["sd2"].ptr, f64["r"].ptr,
                                         f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
-                                        f64["ky"].ptr)
+                                        f64["ky"].ptr, 0.0, None, None)
         L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 1,
                                         nb.ctypes.data, None, f64["sd2"].ptr, f64["r"].ptr,
                                         f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
-                                        f64["ky"].ptr)
+                                        f64["ky"].ptr, 0.3, f64["p"].ptr, f64["w"].ptr)
+        L.tlhip2d_cg_calc_pw(x, y, hd, 0.5, C.addressof(pw), f64["p"].ptr, f64["sd"].ptr,
+                             f64["r"].ptr, f64["w"].ptr, f64["kx"].ptr, f64["ky"].ptr)
         L.tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
                                             nb.ctypes.data, f32["sd"].ptr, f32["sd2"].ptr,
                                             f32["r"].ptr, f32["r2"].ptr, f32["u"].ptr,
-                                            f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr)
+                                            f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr, 0.0,
+                                            None, None)
         L.tlhip2d_ppcg_inner_iteration(x, y, hd, 0.7, 0.2, f64["u"].ptr, f64["r"].ptr, f64["kx"].ptr,
                                        f64["ky"].ptr, f64["sd"].ptr, f64["sd2"].ptr)
         L.tlhip2d_cg_calc_w(x, y, hd, C.addressof(pw), f64["p"].ptr, f64["w"].ptr, f64["kx"].ptr,
@@ -315,9 +320,10 @@ def main():
         sw = res["by_solver"]["ppcg"]["sweeps"]
         t = k["avg_ms"] * 1e-3
         if dom == "tlhip2d_ppcg_inner_iterations":
-            # the survey's per-unit figures: 64 B/cell per inner step, plus
-            # 16 B/cell for the ppcg_init each outer iteration's first launch does
-            alg_cell = (64.0 * sw["ppcg_inner"] + 16.0 * sw["ppcg_outer"]) / k["launches"]
+            # the survey's per-unit figures: 64 B/cell per inner step, plus the
+            # cg_calc_ur (48 B/cell) and ppcg_init (16 B/cell) that each outer
+            # iteration's first launch does
+            alg_cell = (64.0 * sw["ppcg_inner"] + (48.0 + 16.0) * sw["ppcg_outer"]) / k["launches"]
             init_share = sw["ppcg_outer"] / k["launches"]
         else:
             alg_cell, init_share = 64.0, 0.0

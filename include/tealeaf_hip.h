@@ -160,15 +160,19 @@ int tlhip2d_ppcg_inner_iteration(int x, int y, int halo_depth, double alpha,
  * sd, then run_ppcg_inner_iteration with alphas[i], betas[i]) for a chunk
  * whose four faces are external (neighbours[4] all -1).  With init_from_r,
  * sd starts as r/theta (run_ppcg_init, 2d/kernel_interface.h:58) and sd_in
- * is not read.  r, u and sd are read from *_in and written to *_out; the
- * interiors equal, bit for bit, the reference's in-place results and the
- * halos of r and u are carried over (sd's too unless init_from_r).  Needs
- * an even x and 16-byte aligned fields. */
+ * is not read; if p and w are also given, r and u first take run_cg_calc_ur's
+ * update (u += alpha p, r -= alpha w; 2d/kernel_interface.h:40, which
+ * ppcg_main_step_driver runs before the inner loop, ppcg_driver.c:106-116,
+ * and whose rrn PPCG overwrites unused, ppcg_driver.c:121-130).  r, u and sd are read from *_in and written
+ * to *_out; the interiors equal, bit for bit, the reference's in-place
+ * results and the halos of r and u are carried over (sd's too unless
+ * init_from_r).  Needs an even x and 16-byte aligned fields. */
 int tlhip2d_ppcg_inner_iterations(int x, int y, int halo_depth, int count,
         const double* alphas, const double* betas, double theta,
         int init_from_r, const int* neighbours, const double* sd_in,
         double* sd_out, const double* r_in, double* r_out, const double* u_in,
-        double* u_out, const double* kx, const double* ky);
+        double* u_out, const double* kx, const double* ky, double alpha,
+        const double* p, const double* w);
 /* The same loop on FP32 fields (the inner sweeps of the mixed-precision
  * solve): rows of `pitch` floats (a multiple of 4, >= x), interior
  * [halo_depth, x-halo_depth) x [halo_depth, y-halo_depth).  Coefficients
@@ -178,7 +182,17 @@ int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int halo_depth, int pitch,
         int count, const double* alphas, const double* betas, double theta,
         int init_from_r, const int* neighbours, const float* sd_in,
         float* sd_out, const float* r_in, float* r_out, const float* u_in,
-        float* u_out, const float* kx, const float* ky);
+        float* u_out, const float* kx, const float* ky, double alpha,
+        const float* p, const float* w);
+/* run_cg_calc_p (2d/kernel_interface.h:42), a depth-1 halo update of p and
+ * run_cg_calc_w (:38) in one pass, for a chunk whose faces are external:
+ * p_out = beta p_in + r, w = A p_out, *pw += sum w p_out.  p_out and w equal
+ * the reference's results bit for bit (p_out's halo ring is p_in's).  This
+ * is one CG step's calc_p followed by the next step's calc_w
+ * (2d/drivers/cg_driver.c:115 then :78).  Needs an even x. */
+int tlhip2d_cg_calc_pw(int x, int y, int halo_depth, double beta, double* pw,
+        const double* p_in, double* p_out, const double* r, double* w,
+        const double* kx, const double* ky);
 /* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
  * 96 for FP64 and 48 for FP32) and next-row prefetch (0 off, 1 on, < 0:
  * default on). */

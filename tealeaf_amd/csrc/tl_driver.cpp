@@ -103,7 +103,7 @@ struct tlhip_problem {
     double *density0 = nullptr, *density = nullptr, *energy0 = nullptr, *energy = nullptr;
     double *u = nullptr, *u0 = nullptr, *p = nullptr, *r = nullptr, *mi = nullptr, *w = nullptr;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr, *sd = nullptr;
-    double *u_alt = nullptr, *sd_alt = nullptr, *r_alt = nullptr;
+    double *u_alt = nullptr, *sd_alt = nullptr, *r_alt = nullptr, *p_alt = nullptr;
     double *volume = nullptr, *x_area = nullptr, *y_area = nullptr, *z_area = nullptr;
     double *cell_x = nullptr, *cell_y = nullptr, *cell_z = nullptr;
     double *vertex_x = nullptr, *vertex_y = nullptr, *vertex_z = nullptr;
@@ -120,7 +120,12 @@ struct tlhip_problem {
     // FP32 fields of the mixed-precision solve, row pitch P.
     int P = 0;
     float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *w32 = nullptr, *r32 = nullptr,
-          *d32 = nullptr, *sd32 = nullptr, *sd32b = nullptr, *r32b = nullptr, *d32b = nullptr;
+          *d32 = nullptr, *sd32 = nullptr, *sd32b = nullptr, *r32b = nullptr, *d32b = nullptr,
+          *p32b = nullptr;
+    // A CG/PPCG step's calc_p waits to be fused into the next step's calc_w
+    // (tlhip2d_cg_calc_pw); flush_p() applies it where p is read otherwise.
+    bool pending_p = false;
+    double pending_beta = 0.0;
 
     // -------------------------------------------------------- kernels
     int local_halos(int depth)
@@ -197,17 +202,56 @@ struct tlhip_problem {
         return start_solve();
     }
 
+    // -------------------------------------------------------- deferred calc_p
+    // 2D: the p = beta p + r of one step and the w = A p of the next run as
+    // one pass (p ping-pongs with p_alt); the reference's halo update of p in
+    // between is the kernel's mirror.  The results are the reference's bit
+    // for bit.
+    bool can_defer_p() const { return dims == 2 && x % 2 == 0 && p_alt; }
+
+    int calc_w_pending(double* pw)
+    {
+        if (!pending_p) return calc_w(pw);
+        pending_p = false;
+        TRY(tlhip2d_cg_calc_pw(x, y, hd, pending_beta, pw, p, p_alt, r, w, kx, ky));
+        std::swap(p, p_alt);
+        return 0;
+    }
+
+    int calc_p_deferred(double beta)
+    {
+        if (!can_defer_p()) return calc_p(beta);
+        pending_p = true;
+        pending_beta = beta;
+        return 0;
+    }
+
+    // Apply a deferred calc_p and the depth-1 halo update of p that followed it.
+    int flush_p()
+    {
+        if (!pending_p) return 0;
+        pending_p = false;
+        TRY(calc_p(pending_beta));
+        bool keep[6];
+        std::memcpy(keep, fields, sizeof(fields));
+        reset_fields();
+        fields[F_P] = true;
+        const int rc = local_halos(1);
+        std::memcpy(fields, keep, sizeof(fields));
+        return rc;
+    }
+
     int cg_main_step(int tt, double* rro, double* error)
     {
         double pw = 0.0;
-        TRY(calc_w(&pw));
+        TRY(calc_w_pending(&pw));
         const double alpha = *rro / pw;
         double rrn = 0.0;
         cg_alphas[tt] = alpha;
         TRY(calc_ur(alpha, &rrn));
         const double beta = rrn / *rro;
         cg_betas[tt] = beta;
-        TRY(calc_p(beta));
+        TRY(calc_p_deferred(beta));
         *error = rrn;
         *rro = rrn;
         rep->n_cg_steps++;
@@ -278,6 +322,7 @@ struct tlhip_problem {
             TRY(stop(error, &done));
             if (done) break;
         }
+        TRY(flush_p());
         rep->printed_cg_iters = tt;
         return 0;
     }
@@ -366,6 +411,7 @@ struct tlhip_problem {
             if (!switch_now(ncheby, tt, *error)) {
                 TRY(cg_main_step(tt, &rro, error));
             } else if (++ncheby == 1) {
+                TRY(flush_p());
                 double bb = 0.0;
                 TRY(eigenvalues(tt));
                 cheby_coefs(s.max_iters - tt);
@@ -387,6 +433,7 @@ struct tlhip_problem {
             TRY(stop(error, &done));
             if (done) break;
         }
+        TRY(flush_p());
         rep->printed_cg_iters = tt - ncheby + 1;
         rep->printed_inner_iters = ncheby;
         rep->est_iters = est;
@@ -404,29 +451,34 @@ struct tlhip_problem {
     int ppcg_main_step(double* rro, double* error)
     {
         double pw = 0.0;
-        TRY(calc_w(&pw));
+        TRY(calc_w_pending(&pw));
         const double alpha = *rro / pw;
         double rrn = 0.0;
-        TRY(calc_ur(alpha, &rrn));
 
-        if (fused_ppcg()) {
-            // ppcg_init and the inner loop, several steps per pass; the
-            // depth-1 reflections of sd happen inside the kernel.
+        if (fused_ppcg() && s.ppcg_inner_steps > 0) {
+            // calc_ur, ppcg_init and the inner loop, several steps per pass;
+            // the depth-1 reflections of sd happen inside the kernel.  (The
+            // rrn of calc_ur is not used by PPCG: ppcg_driver.c:121-130.)
             int done = 0;
             while (done < s.ppcg_inner_steps) {
                 const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
+                const bool first = done == 0;
                 TRY(tlhip2d_ppcg_inner_iterations(x, y, hd, cnt, &cheby_alphas[done],
-                                                  &cheby_betas[done], theta, done == 0,
-                                                  neighbours, done == 0 ? nullptr : sd, sd_alt,
-                                                  r, r_alt, u, u_alt, kx, ky));
+                                                  &cheby_betas[done], theta, first, neighbours,
+                                                  first ? nullptr : sd, sd_alt, r, r_alt, u,
+                                                  u_alt, kx, ky, alpha, first ? p : nullptr,
+                                                  first ? w : nullptr));
                 std::swap(sd, sd_alt);
                 std::swap(r, r_alt);
                 std::swap(u, u_alt);
                 done += cnt;
                 rep->n_ppcg_inner += cnt;
             }
-            if (s.ppcg_inner_steps == 0) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
+        } else if (fused_ppcg()) {
+            TRY(calc_ur(alpha, &rrn));
+            TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
         } else {
+            TRY(calc_ur(alpha, &rrn));
             if (dims == 2) TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
             else TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
             reset_fields();
@@ -451,7 +503,7 @@ struct tlhip_problem {
         rrn = 0.0;
         TRY(norm2(r, &rrn));
         const double beta = rrn / *rro;
-        TRY(calc_p(beta));
+        TRY(calc_p_deferred(beta));
         *error = rrn;
         *rro = rrn;
         rep->n_ppcg_outer++;
@@ -468,6 +520,7 @@ struct tlhip_problem {
                 TRY(cg_main_step(tt, &rro, error));
             } else {
                 if (++nppcg == 1) {
+                    TRY(flush_p());             // residual() overwrites r, which calc_p reads
                     TRY(eigenvalues(tt));
                     cheby_coefs(s.ppcg_inner_steps);
                     TRY(residual());
@@ -482,6 +535,7 @@ struct tlhip_problem {
             TRY(stop(error, &done));
             if (done) break;
         }
+        TRY(flush_p());
         rep->printed_cg_iters = tt - nppcg + 1;
         rep->printed_inner_iters = nppcg;
         return 0;
@@ -506,14 +560,34 @@ struct tlhip_problem {
         if (dims != 2) return tl_fail(TLHIP_ERR_ARG, "mixed precision: 2D only");
         P = (x + 3) / 4 * 4;
         const size_t bytes = sizeof(float) * (size_t)P * (size_t)y;
-        float** all[] = {&kx32, &ky32, &p32, &w32, &r32, &d32, &sd32, &sd32b, &r32b, &d32b};
+        float** all[] = {&kx32, &ky32, &p32, &w32, &r32, &d32, &sd32, &sd32b, &r32b, &d32b, &p32b};
         for (float** f : all) TRY(tlhip_alloc(bytes, (void**)f));
         return 0;
     }
 
+    // FP32 calc_w, with a deferred calc_p fused in (as calc_w_pending)
+    int calc_w32_pending(double* pw)
+    {
+        if (!pending_p) return tl::mix::calc_w32(x, y, hd, P, pw, p32, w32, kx32, ky32);
+        pending_p = false;
+        TRY(tl::mix::calc_pw32(x, y, hd, P, pending_beta, pw, p32, p32b, r32, w32, kx32, ky32));
+        std::swap(p32, p32b);
+        return 0;
+    }
+
+    int flush_p32()
+    {
+        if (!pending_p) return 0;
+        pending_p = false;
+        return tl::mix::calc_p32(x, y, hd, P, pending_beta, false, p32, r32);
+    }
+
     // u += d (FP64); r32 = u0 - A u computed in FP64; *rr = its squared norm.
+    // A deferred calc_p is applied first, with the recurrence residual, as
+    // the step that deferred it would have.
     int replace_residual(double* rr)
     {
+        TRY(flush_p32());
         TRY(tl::mix::promote(x, y, hd, P, u, d32));
         *rr = 0.0;
         TRY(tl::mix::residual64(x, y, hd, P, u, u0, kx, ky, r32, rr));
@@ -525,13 +599,14 @@ struct tlhip_problem {
     int mixed_cg_step(int tt, double* rro, double* rrn)
     {
         double pw = 0.0;
-        TRY(tl::mix::calc_w32(x, y, hd, P, &pw, p32, w32, kx32, ky32));
+        TRY(calc_w32_pending(&pw));
         const double alpha = *rro / pw;
         cg_alphas[tt] = alpha;
         TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, rrn, d32, p32, r32, w32));
         const double beta = *rrn / *rro;
         cg_betas[tt] = beta;
-        TRY(tl::mix::calc_p32(x, y, hd, P, beta, false, p32, r32));
+        pending_p = true;
+        pending_beta = beta;
         *rro = *rrn;
         rep->n_cg_steps++;
         return 0;
@@ -540,18 +615,23 @@ struct tlhip_problem {
     int mixed_ppcg_step(double* rro, double* rrn)
     {
         double pw = 0.0;
-        TRY(tl::mix::calc_w32(x, y, hd, P, &pw, p32, w32, kx32, ky32));
+        TRY(calc_w32_pending(&pw));
         const double alpha = *rro / pw;
-        double unused = 0.0;
-        TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
+        if (s.ppcg_inner_steps == 0) {
+            double unused = 0.0;
+            TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
+        }
         const int per = std::max(1, g_ppcg_fusion);
         int done = 0;
         while (done < s.ppcg_inner_steps) {
+            // the first launch also applies calc_ur (u is d here)
             const int cnt = std::min(per, s.ppcg_inner_steps - done);
+            const bool first = done == 0;
             TRY(tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, cnt, &cheby_alphas[done],
-                                                  &cheby_betas[done], theta, done == 0,
-                                                  neighbours, done == 0 ? nullptr : sd32, sd32b,
-                                                  r32, r32b, d32, d32b, kx32, ky32));
+                                                  &cheby_betas[done], theta, first, neighbours,
+                                                  first ? nullptr : sd32, sd32b, r32, r32b, d32,
+                                                  d32b, kx32, ky32, alpha, first ? p32 : nullptr,
+                                                  first ? w32 : nullptr));
             std::swap(sd32, sd32b);
             std::swap(r32, r32b);
             std::swap(d32, d32b);
@@ -561,7 +641,8 @@ struct tlhip_problem {
         *rrn = 0.0;
         TRY(tl::mix::norm32(x, y, hd, P, r32, rrn));
         const double beta = *rrn / *rro;
-        TRY(tl::mix::calc_p32(x, y, hd, P, beta, false, p32, r32));
+        pending_p = true;
+        pending_beta = beta;
         *rro = *rrn;
         rep->n_ppcg_outer++;
         return 0;
@@ -577,6 +658,7 @@ struct tlhip_problem {
         TRY(tl::mix::demote(kx, x, y, kx32, P));
         TRY(tl::mix::demote(ky, x, y, ky32, P));
         TRY(tl::mix::demote(r, x, y, r32, P));
+        pending_p = false;
         TRY(tl::mix::calc_p32(x, y, hd, P, 0.0, true, p32, r32));
         TRY(tl::mix::zero32(d32, (long)P * y));
         const double target = rel_tol > 0.0 ? rel_tol * rel_tol * bb : s.eps;
@@ -607,6 +689,7 @@ struct tlhip_problem {
             }
         }
         if (tt == s.max_iters) TRY(replace_residual(&rr_true));   // fold the last correction
+        TRY(flush_p32());
         if (s.solver == 1) {
             rep->printed_cg_iters = tt;                  // as cg_driver.c prints
         } else {
@@ -677,6 +760,7 @@ struct tlhip_problem {
             TRY(stop(&error, &done));
             if (done) break;
         }
+        TRY(flush_p());
         report->printed_cg_iters = tt;
         report->error = error;
         return 0;
@@ -746,6 +830,7 @@ struct tlhip_problem {
     {
         std::memset(report, 0, sizeof(*report));
         rep = report;
+        pending_p = false;
         const double dt = s.dt_init;
         const double rx = dt / (dx * dx), ry = dt / (dy * dy);
         const double rz = dims == 3 ? dt / (dz * dz) : 0.0;
@@ -795,11 +880,11 @@ struct tlhip_problem {
     void release()
     {
         double* all[] = {density0, density, energy0, energy, u, u0, p, r, mi, w, kx, ky, kz,
-                         sd, u_alt, sd_alt, r_alt, volume, x_area, y_area, z_area, cell_x,
+                         sd, u_alt, sd_alt, r_alt, p_alt, volume, x_area, y_area, z_area, cell_x,
                          cell_y, cell_z, vertex_x, vertex_y, vertex_z};
         for (double* a : all)
             if (a) (void)tlhip_free(a);
-        float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b};
+        float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b, p32b};
         for (float* a : all32)
             if (a) (void)tlhip_free(a);
     }
@@ -874,8 +959,9 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
     if (P->dims == 3) {
         P->kz = dalloc(n, &rc);
         P->z_area = dalloc(n, &rc);
-    } else if (s->solver == 3) {
-        P->r_alt = dalloc(n, &rc);      // partner of r for the fused PPCG loop
+    } else {
+        P->p_alt = dalloc(n, &rc);      // partner of p for the fused calc_p + calc_w
+        if (s->solver == 3) P->r_alt = dalloc(n, &rc);   // partner of r for the fused PPCG loop
     }
     P->cell_x = dalloc(P->x + 1, &rc);
     P->cell_y = dalloc(P->y + 1, &rc);

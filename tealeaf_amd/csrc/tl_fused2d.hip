@@ -24,8 +24,11 @@
 // order, so in FP64 the result is bit-for-bit that of S separate steps.  HBM
 // traffic for S steps is that of one: read sd, kx, ky, r, u; write sd, r, u.
 // Optionally sd at level 0 is ppcg_init's r/theta (ppcg.c:8-25), computed on
-// the fly from r.
+// the fly from r, and r, u at level 0 may first take cg_calc_ur's update
+// u += alpha p, r -= alpha w (cg.c:116-143; the PPCG outer step that
+// precedes the inner loop), also on the fly.
 #include "tl_lanes.h"
+#include "tl_mixed.h"
 #include "tl_ops.h"
 #include "tealeaf_hip.h"
 
@@ -40,14 +43,19 @@ struct PpcgCoefs {
     T alpha[S], beta[S];
 };
 
-template <class T, int S, bool FROM_R, bool PF>
+// Level 0 of the loop: MODE 0 reads sd; 1 forms sd = r/theta (ppcg_init);
+// 2 first applies cg_calc_ur to r and u, then forms sd = r/theta.
+template <class T, int S, int MODE, bool PF>
 __global__ void __launch_bounds__(kBlock)
-k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
+k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ur,
                const T* __restrict__ sd_in, T* __restrict__ sd_out,
                const T* __restrict__ r_in, T* __restrict__ r_out,
                const T* __restrict__ u_in, T* __restrict__ u_out,
-               const T* __restrict__ kx, const T* __restrict__ ky)
+               const T* __restrict__ kx, const T* __restrict__ ky,
+               const T* __restrict__ p_in, const T* __restrict__ w_in)
 {
+    constexpr bool FROM_R = MODE >= 1;
+    constexpr bool UR = MODE == 2;
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
@@ -94,11 +102,28 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
 #pragma unroll
     for (int m = 0; m <= S; ++m) kyw[m] = zeros<T>();
 
+    // r - alpha w and u + alpha p (cg.c:128-129, same association)
+    auto minus_aw = [&](C a, const C& b) -> C {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] - alpha_ur * b.v[i];
+        return a;
+    };
+    auto plus_ap = [&](C a, const C& b) -> C {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] + alpha_ur * b.v[i];
+        return a;
+    };
+    auto r_level0 = [&](int row) -> C {
+        C a = ld_field(r_in, row, yin0, yin1);
+        if constexpr (UR) a = minus_aw(a, ld_field(w_in, row, yin0, yin1));
+        return a;
+    };
+
     // Prologue: level-0 sd rows rs-1, rs; kx rows below rs; ky rows up to rs.
-    C rcarry = zeros<T>();    // FROM_R: r_in row rho1, loaded one iteration ago
+    C rcarry = zeros<T>();    // FROM_R: level-0 r at row rho1, loaded one iteration ago
     if constexpr (FROM_R) {
-        sdw[0][1] = over_theta(ld_field(r_in, rs - 1, yin0, yin1));
-        rcarry = ld_field(r_in, rs, yin0, yin1);
+        sdw[0][1] = over_theta(r_level0(rs - 1));
+        rcarry = r_level0(rs);
         sdw[0][2] = over_theta(rcarry);
     } else {
         sdw[0][1] = ld_field(sd_in, rs - 1, yin0, yin1);
@@ -110,15 +135,21 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
     for (int m = 0; m < S; ++m) kyw[m] = ld_field(ky, rs - m, yin0, yin1 + 1);
 
     // Level-0 inputs of the iteration whose level 1 computes row rho1:
-    // a = sd (or r for FROM_R) at rho1+1, b = r at rho1 (unless FROM_R),
-    // u at rho1, kx at rho1, ky at rho1+1.
+    // a = sd (or r for FROM_R) at rho1+1, b = r at rho1 (or, for UR, w at
+    // rho1+1), u at rho1 (and p at rho1 for UR), kx at rho1, ky at rho1+1.
+    // They are combined only when used, so that prefetched loads stay in
+    // flight across the arithmetic.
     struct In {
-        C a, b, u, kx, ky;
+        C a, b, u, p, kx, ky;
     };
     auto load_in = [&](int rho1) -> In {
         In v;
         v.a = ld_field(FROM_R ? r_in : sd_in, rho1 + 1, yin0, yin1);
         if constexpr (!FROM_R) v.b = ld_field(r_in, rho1, yin0, yin1);
+        if constexpr (UR) {
+            v.b = ld_field(w_in, rho1 + 1, yin0, yin1);
+            v.p = ld_field(p_in, rho1, yin0, yin1);
+        }
         v.u = ld_field(u_in, rho1, yin0, yin1);
         v.kx = ld_field(kx, rho1, yin0, yin1);
         v.ky = ld_field(ky, rho1 + 1, yin0, yin1 + 1);
@@ -138,16 +169,16 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
         } else {
             cur = load_in(rho1);
         }
-        C sd_new, r0;
+        C sd_new, r0, u0 = cur.u;
         if constexpr (FROM_R) {
             r0 = rcarry;
-            rcarry = cur.a;
-            sd_new = over_theta(cur.a);
+            rcarry = UR ? minus_aw(cur.a, cur.b) : cur.a;
+            sd_new = over_theta(rcarry);
         } else {
             r0 = cur.b;
             sd_new = cur.a;
         }
-        const C u0 = cur.u;
+        if constexpr (UR) u0 = plus_ap(u0, cur.p);
         sdw[0][0] = sdw[0][1];
         sdw[0][1] = sdw[0][2];
         sdw[0][2] = sd_new;
@@ -211,6 +242,93 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta,
     }
 }
 
+// w = A p, sum w*p (cg.c:88-113) on a row march with the depth-1 mirror
+// built in; with UPDATE, p is first replaced by beta p + r (cg_calc_p,
+// cg.c:146-164), computed for every row and neighbour the stencil reads
+// and stored to a second buffer.  Both are the reference's expressions in
+// its order, so p_out and w equal cg_calc_p, a depth-1 halo update of p and
+// cg_calc_w bit for bit; the dot product differs only by summation order.
+template <class T, bool UPDATE>
+__global__ void __launch_bounds__(kBlock)
+k_calc_pw(PitchGrid g, StripTile t, T beta, const T* __restrict__ p_in, T* __restrict__ p_out,
+          const T* __restrict__ r, T* __restrict__ w, const T* __restrict__ kx,
+          const T* __restrict__ ky, TlReduce rd)
+{
+    constexpr int N = Lane16<T>::N;
+    using C = Cells<T>;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
+    const int bx = tile % t.nbx, by = tile / t.nbx;
+    const int sx = bx * kWaves + wave;
+    double acc[1] = {0.0};
+    if (sx < t.ns) {                            // every wave reaches the block reduction
+        const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
+        const int o0 = xin0 + sx * t.w;
+        const int o1 = min(o0 + t.w, xin1);
+        const int c = t.cw0 + sx * t.w + N * lane;
+        const bool ld_ok = c >= 0 && c + N <= g.X;
+        const bool all_out = c >= o0 && c + N <= o1;
+        const bool any_out = c + N > o0 && c < o1;
+        const int j0 = yin0 + by * t.ty;
+        const int j1 = min(j0 + t.ty, yin1);
+        const long X = g.X;
+        auto ld = [&](const T* __restrict__ f, int row) -> C {
+            return ld_ok ? ldc(f, (long)row * X + c) : zeros<T>();
+        };
+        auto combine = [&](C a, const C& b) -> C {
+            if constexpr (UPDATE) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) a.v[i] = beta * a.v[i] + b.v[i];
+            }
+            return a;
+        };
+        auto pnew = [&](int row) -> C {
+            if constexpr (UPDATE) return combine(ld(p_in, row), ld(r, row));
+            return ld(p_in, row);
+        };
+        // rows j0-1 .. j0+1 exist: hd >= 1 and j0 < yin1
+        C dn = pnew(j0 - 1), cen = pnew(j0), up = pnew(j0 + 1);
+        C kxc = ld(kx, j0), kyc = ld(ky, j0), kyu = ld(ky, j0 + 1);
+        for (int j = j0; j < j1; ++j) {
+            // raw inputs of the next row, combined after this row's arithmetic
+            C pn = zeros<T>(), rn = zeros<T>(), kyn = zeros<T>(), kxn = zeros<T>();
+            if (j + 1 < j1) {                   // rows j+2 <= yin1 <= y-1
+                pn = ld(p_in, j + 2);
+                if constexpr (UPDATE) rn = ld(r, j + 2);
+                kyn = ld(ky, j + 2);
+                kxn = ld(kx, j + 1);
+            }
+            const C upm = j == yin1 - 1 ? cen : up;
+            const C dnm = j == yin0 ? cen : dn;
+            const C av = apply_a_lane(cen, upm, dnm, kxc, kyu, kyc, c, xin0, xin1);
+            if (any_out) {
+                const long i0 = (long)j * X + c;
+                if (all_out) {
+                    stc(w, i0, av);
+                    if constexpr (UPDATE) stc(p_out, i0, cen);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) acc[0] += (double)av.v[i] * (double)cen.v[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (c + i >= o0 && c + i < o1) {
+                            w[i0 + i] = av.v[i];
+                            if constexpr (UPDATE) p_out[i0 + i] = cen.v[i];
+                            acc[0] += (double)av.v[i] * (double)cen.v[i];
+                        }
+                }
+            }
+            dn = cen;
+            cen = up;
+            up = combine(pn, rn);
+            kyc = kyu;
+            kyu = kyn;
+            kxc = kxn;
+        }
+    }
+    tl_reduce_partial<1, kBlock>(acc, rd);
+}
+
 // Everything outside the interior of up to three fields, src -> dst.
 template <class T>
 struct Ring3 {
@@ -250,60 +368,72 @@ __global__ void __launch_bounds__(kBlock) k_copy_rings2d(PitchGrid g, int rows, 
 int g_fused_ty = -1;   // output rows per tile (<= 0: default)
 int g_fused_pf = -1;   // row prefetch (< 0: default)
 
-template <class T, int S, bool FROM_R>
-int launch_fused(const PitchGrid& g, const double* al, const double* be, double theta,
-                 const T* sd_in, T* sd_out, const T* r_in, T* r_out, const T* u_in, T* u_out,
-                 const T* kx, const T* ky, const char* what)
+template <class T>
+struct FusedArgs {
+    PitchGrid g;
+    const double* al;
+    const double* be;
+    double theta, alpha_ur;
+    const T* sd_in;
+    T* sd_out;
+    const T* r_in;
+    T* r_out;
+    const T* u_in;
+    T* u_out;
+    const T* kx;
+    const T* ky;
+    const T* p;
+    const T* w;
+    const char* what;
+};
+
+template <class T, int S, int MODE>
+int launch_fused(const FusedArgs<T>& a)
 {
     // Windows reach S columns beyond their output columns on both sides.
     // Default tile heights from a sweep at 4096^2 (S = 5, prefetch on).
     const int ty = g_fused_ty > 0 ? g_fused_ty : (sizeof(T) == 8 ? 96 : 48);
-    const StripTile t = strip_tile<T>(g, S, ty, kWaves);
-    if ((long)t.nbx * t.nby > (1L << 30)) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    const StripTile t = strip_tile<T>(a.g, S, ty, kWaves);
+    if ((long)t.nbx * t.nby > (1L << 30))
+        return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", a.what);
     PpcgCoefs<T, S> cf;
     for (int k = 0; k < S; ++k) {
-        cf.alpha[k] = (T)al[k];
-        cf.beta[k] = (T)be[k];
+        cf.alpha[k] = (T)a.al[k];
+        cf.beta[k] = (T)a.be[k];
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
-    const int slot = tl_timer_begin(what);
+    const int slot = tl_timer_begin(a.what);
     if (g_fused_pf != 0)
-        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, FROM_R, true>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), g, t, cf, (T)theta, sd_in, sd_out, r_in, r_out, u_in,
-                           u_out, kx, ky);
+        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, true>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.sd_in, a.sd_out,
+                           a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w);
     else
-        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, FROM_R, false>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), g, t, cf, (T)theta, sd_in, sd_out, r_in, r_out, u_in,
-                           u_out, kx, ky);
+        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, false>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.sd_in, a.sd_out,
+                           a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w);
     tl_timer_end(slot);
-    return tl_check_launch(what);
+    return tl_check_launch(a.what);
 }
 
-template <class T, bool FROM_R>
-int dispatch_fused(int count, const PitchGrid& g, const double* al, const double* be,
-                   double theta, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
-                   const T* u_in, T* u_out, const T* kx, const T* ky, const char* what)
+template <class T, int MODE>
+int dispatch_fused(int count, const FusedArgs<T>& a)
 {
-#define TL_FUSED_CASE(S)                                                                       \
-    case S:                                                                                    \
-        return launch_fused<T, S, FROM_R>(g, al, be, theta, sd_in, sd_out, r_in, r_out, u_in, \
-                                          u_out, kx, ky, what);
     switch (count) {
-        TL_FUSED_CASE(1)
-        TL_FUSED_CASE(2)
-        TL_FUSED_CASE(3)
-        TL_FUSED_CASE(4)
-        TL_FUSED_CASE(5)
-    default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", what, kMaxFused);
+    case 1: return launch_fused<T, 1, MODE>(a);
+    case 2: return launch_fused<T, 2, MODE>(a);
+    case 3: return launch_fused<T, 3, MODE>(a);
+    case 4: return launch_fused<T, 4, MODE>(a);
+    case 5: return launch_fused<T, 5, MODE>(a);
+    default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", a.what, kMaxFused);
     }
-#undef TL_FUSED_CASE
 }
 
 template <class T>
 int inner_iterations(const char* what, int x, int y, int hd, int pitch, int count,
                      const double* alphas, const double* betas, double theta, int init_from_r,
                      const int* neighbours, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
-                     const T* u_in, T* u_out, const T* kx, const T* ky)
+                     const T* u_in, T* u_out, const T* kx, const T* ky, double alpha_ur,
+                     const T* p, const T* w)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
@@ -319,9 +449,11 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
                "null field");
     TL_REQUIRE(sd_in != sd_out && r_in != r_out && u_in != u_out, what,
                "input and output buffers must be distinct");
+    TL_REQUIRE((p == nullptr) == (w == nullptr), what, "p and w go together");
+    TL_REQUIRE(!p || init_from_r, what, "the cg_calc_ur update needs init_from_r");
     for (int f = 0; f < 4; ++f)
         TL_REQUIRE(neighbours[f] == -1, what, "every face must be external (single chunk)");
-    TL_REQUIRE(all_aligned(sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky), what,
+    TL_REQUIRE(all_aligned(sd_in, sd_out, r_in, r_out, u_in, u_out, kx, ky, p, w), what,
                "fields must be 16-byte aligned");
     if (init_from_r && theta == 0.0) return tl_fail(TLHIP_ERR_ARG, "%s: theta is zero", what);
     const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
@@ -338,14 +470,91 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
     hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0, tl_stream(), g,
                        y, rg);
     TL_TRY(tl_check_launch(what));
-    if (init_from_r)
-        return dispatch_fused<T, true>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out,
-                                       u_in, u_out, kx, ky, what);
-    return dispatch_fused<T, false>(count, g, alphas, betas, theta, sd_in, sd_out, r_in, r_out,
-                                    u_in, u_out, kx, ky, what);
+    const FusedArgs<T> a{g, alphas, betas, theta, alpha_ur, sd_in, sd_out, r_in, r_out, u_in,
+                         u_out, kx, ky, p, w, what};
+    if (!init_from_r) return dispatch_fused<T, 0>(count, a);
+    if (!p) return dispatch_fused<T, 1>(count, a);
+    return dispatch_fused<T, 2>(count, a);
+}
+
+int g_pw_ty = 32;
+
+template <class T>
+int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
+            double* pw, const T* p_in, T* p_out, const T* r, T* w, const T* kx, const T* ky)
+{
+    constexpr int N = Lane16<T>::N;
+    TL_TRY(tl_ensure_init());
+    if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || pitch < x || pitch % N ||
+        (long)pitch * y > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG,
+                       "%s: needs a row pitch that is a multiple of %d and >= x (x=%d pitch=%d "
+                       "y=%d hd=%d)", what, N, x, pitch, y, hd);
+    TL_REQUIRE(pw && p_in && w && kx && ky && (!update || (p_out && r)), what, "null argument");
+    TL_REQUIRE(w != p_in && (!update || (p_out != p_in && p_out != w && p_out != r)), what,
+               "p_out and w must be distinct from the inputs");
+    TL_REQUIRE(all_aligned(p_in, p_out, r, w, kx, ky), what, "fields must be 16-byte aligned");
+    const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
+    const StripTile t = strip_tile<T>(g, 1, g_pw_ty, kWaves);
+    const unsigned nb = (unsigned)(t.nbx * t.nby);
+    if ((long)t.nbx * t.nby > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    if (update) {
+        Ring3<T> rg{};
+        rg.src[0] = p_in;
+        rg.dst[0] = p_out;
+        rg.n = 1;
+        const long ring = (long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny;
+        hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
+                           tl_stream(), g, y, rg);
+        TL_TRY(tl_check_launch(what));
+    }
+    const int slot = tl_timer_begin(what);
+    if (update)
+        hipLaunchKernelGGL((k_calc_pw<T, true>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t,
+                           (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+    else
+        hipLaunchKernelGGL((k_calc_pw<T, false>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t,
+                           (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+    tl_timer_end(slot);
+    TL_TRY(tl_check_launch(what));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return fetch_add(pw, what);
 }
 
 }  // namespace
+
+namespace tl {
+namespace mix {
+
+int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,
+             const float* ky)
+{
+    *pw = 0.0;
+    return calc_pw<float>("tlhip2d_mix_calc_w32", x, y, hd, P, false, 0.0, pw, p, nullptr,
+                          nullptr, w, kx, ky);
+}
+
+int calc_pw32(int x, int y, int hd, int P, double beta, double* pw, const float* p_in,
+              float* p_out, const float* r, float* w, const float* kx, const float* ky)
+{
+    *pw = 0.0;
+    return calc_pw<float>("tlhip2d_mix_calc_pw32", x, y, hd, P, true, beta, pw, p_in, p_out, r,
+                          w, kx, ky);
+}
+
+}  // namespace mix
+}  // namespace tl
+
+extern "C" int tlhip2d_cg_calc_pw(int x, int y, int hd, double beta, double* pw,
+                                  const double* p_in, double* p_out, const double* r, double* w,
+                                  const double* kx, const double* ky)
+{
+    if (x % 2)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip2d_cg_calc_pw: needs an even x (got %d)", x);
+    return calc_pw<double>("tlhip2d_cg_calc_pw", x, y, hd, x, true, beta, pw, p_in, p_out, r, w,
+                           kx, ky);
+}
 
 extern "C" int tlhip_set_fused_tuning(int tile_rows, int prefetch)
 {
@@ -360,11 +569,12 @@ extern "C" int tlhip2d_ppcg_inner_iterations(int x, int y, int hd, int count,
                                              const int* neighbours, const double* sd_in,
                                              double* sd_out, const double* r_in, double* r_out,
                                              const double* u_in, double* u_out,
-                                             const double* kx, const double* ky)
+                                             const double* kx, const double* ky, double alpha,
+                                             const double* p, const double* w)
 {
     return inner_iterations<double>("tlhip2d_ppcg_inner_iterations", x, y, hd, x, count, alphas,
                                     betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
-                                    r_out, u_in, u_out, kx, ky);
+                                    r_out, u_in, u_out, kx, ky, alpha, p, w);
 }
 
 extern "C" int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int hd, int pitch, int count,
@@ -373,9 +583,10 @@ extern "C" int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int hd, int pitch
                                                  const int* neighbours, const float* sd_in,
                                                  float* sd_out, const float* r_in, float* r_out,
                                                  const float* u_in, float* u_out,
-                                                 const float* kx, const float* ky)
+                                                 const float* kx, const float* ky, double alpha,
+                                                 const float* p, const float* w)
 {
     return inner_iterations<float>("tlhip2d_ppcg_inner_iterations_f32", x, y, hd, pitch, count,
                                    alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
-                                   r_in, r_out, u_in, u_out, kx, ky);
+                                   r_in, r_out, u_in, u_out, kx, ky, alpha, p, w);
 }

@@ -14,6 +14,9 @@ int demote(const double* src, int x, int y, float* dst, int P);
 // w = A p; *pw = sum w*p (FP64)
 int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,
              const float* ky);
+// p_out = beta p_in + r, then w = A p_out; *pw = sum w*p_out (FP64)
+int calc_pw32(int x, int y, int hd, int P, double beta, double* pw, const float* p_in,
+              float* p_out, const float* r, float* w, const float* kx, const float* ky);
 // d += alpha p; r -= alpha w; *rrn = sum r*r (FP64)
 int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,
               float* r, const float* w);

@@ -230,73 +230,6 @@ __global__ void __launch_bounds__(kBlock) k_zero32(float* __restrict__ a, long n
         a[t] = 0.0f;
 }
 
-// ---------------------------------------------------------------- w = A p
-// One wave marches up the rows of its strip holding the rows below, at and
-// above the current one in registers (p, ky) and prefetching the next row.
-__global__ void __launch_bounds__(kBlock)
-k_calc_w32(PitchGrid g, StripTile t, const float* __restrict__ p, float* __restrict__ w,
-           const float* __restrict__ kx, const float* __restrict__ ky, TlReduce rd)
-{
-    constexpr int N = 4;
-    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
-    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
-    const int bx = tile % t.nbx, by = tile / t.nbx;
-    const int sx = bx * kWaves + wave;
-    double acc[1] = {0.0};
-    if (sx < t.ns) {
-        const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
-        const int o0 = xin0 + sx * t.w;
-        const int o1 = min(o0 + t.w, xin1);
-        const int c = t.cw0 + sx * t.w + N * lane;
-        const bool ld_ok = c >= 0 && c + N <= g.X;
-        const bool all_out = c >= o0 && c + N <= o1;
-        const bool any_out = c + N > o0 && c < o1;
-        const int j0 = yin0 + by * t.ty;
-        const int j1 = min(j0 + t.ty, yin1);
-        const long X = g.X;
-        auto ld = [&](const float* __restrict__ f, int row) -> C4 {
-            return ld_ok ? ldc(f, (long)row * X + c) : zeros<float>();
-        };
-        // rows j0-1 .. j0+1 exist: hd >= 1 and j0 < yin1
-        C4 dn = ld(p, j0 - 1), cen = ld(p, j0), up = ld(p, j0 + 1);
-        C4 kxc = ld(kx, j0), kyc = ld(ky, j0), kyu = ld(ky, j0 + 1);
-        for (int j = j0; j < j1; ++j) {
-            C4 up2 = zeros<float>(), kyu2 = zeros<float>(), kxn = zeros<float>();
-            if (j + 1 < j1) {            // rows j+2 <= yin1 <= y-1
-                up2 = ld(p, j + 2);
-                kyu2 = ld(ky, j + 2);
-                kxn = ld(kx, j + 1);
-            }
-            const C4 upm = j == yin1 - 1 ? cen : up;
-            const C4 dnm = j == yin0 ? cen : dn;
-            const C4 av = apply_a_lane(cen, upm, dnm, kxc, kyu, kyc, c, xin0, xin1);
-            if (any_out) {
-                const long i0 = (long)j * X + c;
-                if (all_out) {
-                    stc(w, i0, av);
-#pragma unroll
-                    for (int i = 0; i < N; ++i) acc[0] += (double)av.v[i] * (double)cen.v[i];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < N; ++i)
-                        if (c + i >= o0 && c + i < o1) {
-                            w[i0 + i] = av.v[i];
-                            acc[0] += (double)av.v[i] * (double)cen.v[i];
-                        }
-                }
-            }
-            dn = cen;
-            cen = up;
-            up = up2;
-            kyc = kyu;
-            kyu = kyu2;
-            kxc = kxn;
-        }
-    }
-    tl_reduce_partial<1, kBlock>(acc, rd);
-}
-
-int g_w32_ty = 32;
 
 }  // namespace
 
@@ -317,23 +250,6 @@ int demote(const double* src, int x, int y, float* dst, int P)
     hipLaunchKernelGGL(k_demote, dim3(flat_blocks((long)P * y)), dim3(kBlock), 0, tl_stream(), src,
                        x, y, dst, P);
     return tl_check_launch(what);
-}
-
-int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,
-             const float* ky)
-{
-    const char* what = "tlhip2d_mix_calc_w32";
-    const PitchGrid g{P, hd, x - 2 * hd, y - 2 * hd};
-    const StripTile t = strip_tile<float>(g, 1, g_w32_ty, kWaves);
-    const unsigned nb = (unsigned)(t.nbx * t.nby);
-    if (nb > TL_MAX_RED_BLOCKS) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
-    const int slot = tl_timer_begin(what);
-    hipLaunchKernelGGL(k_calc_w32, dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, p, w, kx, ky,
-                       tl_reduce_slots());
-    tl_timer_end(slot);
-    TL_TRY(tl_check_launch(what));
-    TL_TRY(tl_finish_reduce(1, nb, what));
-    return tl_fetch(1, pw, what);
 }
 
 int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,

@@ -330,6 +330,7 @@ eps = {eps}
 max_iters = {max_iters}
 end_time = 100.0
 end_step = {steps}
+{dt}
 halo_depth = 2
 num_chunks_per_rank = 1
 ppcg_inner_steps = {inner}
@@ -357,8 +358,10 @@ def layered_2d(n: int, steps: int, solver: int = CG, eps: float = 1e-15,
 
 def cuboid_3d(n: int, steps: int, solver: int = CG, eps: float = 1e-15,
               max_iters: int = 1000, inner: int = 10, presteps: int = 20,
-              coefficient: int = CONDUCTIVITY) -> str:
-    """The reference's 3D cuboid deck (3d/tea.in) at n^3 cells."""
+              coefficient: int = CONDUCTIVITY, dt: float | None = None) -> str:
+    """The reference's 3D cuboid deck (3d/tea.in) at n^3 cells (its timestep
+    is the settings default unless `dt` is given)."""
     coef = "coefficient_inverse_density" if coefficient == RECIP_CONDUCTIVITY else ""
     return CUBOID_3D.format(n=n, steps=steps, solver=_SOLVER_KEY[solver], eps=eps,
-                            max_iters=max_iters, inner=inner, presteps=presteps, coef=coef)
+                            max_iters=max_iters, inner=inner, presteps=presteps, coef=coef,
+                            dt="" if dt is None else f"initial_timestep = {dt}")

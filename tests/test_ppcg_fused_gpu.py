@@ -83,7 +83,7 @@ def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r, pf):
         hip.tlhip2d_ppcg_inner_iterations(
             x, y, hd, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
             nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
-            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr)
+            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr, 0.0, None, None)
     finally:
         hip.tlhip_set_fused_tuning(-1, -1)
     hip.tlhip_synchronize()
@@ -108,22 +108,27 @@ def test_fused_rejects_bad_layout(hip):
     bufs = [DeviceArray.from_host(np.zeros(x * y)) for _ in range(8)]
     p = [b.ptr for b in bufs]
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 1,
-                                           nb.ctypes.data, None, *p[:7])
+                                           nb.ctypes.data, None, *p[:7], 0.0, None, None)
     assert rc != 0 and "pitch" in hip.last_error()
     x = 16                               # in-place output
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[0], p[1], p[2], p[3], p[4],
-                                           p[5], p[6])
+                                           p[5], p[6], 0.0, None, None)
     assert rc != 0
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 6, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
-                                           p[6], p[7])
+                                           p[6], p[7], 0.0, None, None)
     assert rc != 0                       # count above 5
     nb[1] = 3                            # an internal face
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
-                                           p[6], p[7])
+                                           p[6], p[7], 0.0, None, None)
     assert rc != 0
+    nb[1] = -1                           # calc_ur folded in without init_from_r
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
+                                           nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
+                                           p[6], p[7], 0.5, p[0], p[1])
+    assert rc != 0 and "init_from_r" in hip.last_error()
     for b in bufs:
         b.free()
 
@@ -218,7 +223,7 @@ def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
         hip.tlhip2d_ppcg_inner_iterations_f32(
             x, y, hd, pitch, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
             nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
-            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr)
+            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr, 0.0, None, None)
     finally:
         hip.tlhip_set_fused_tuning(-1, -1)
     hip.tlhip_synchronize()
@@ -231,4 +236,85 @@ def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
         if name != "sd" or not from_r:
             np.testing.assert_array_equal(g[ring], src[name][ring], err_msg=f"{name} halo ring")
     for v in list(d.values()) + list(out.values()):
+        v.free()
+
+
+@pytest.mark.parametrize("grid", [(250, 70, 2, 0), (118, 41, 4, 2), (64, 20, 1, 0)],
+                         ids=["250x70", "118x41h4", "64x20h1"])
+@pytest.mark.parametrize("count", [1, 5])
+def test_fused_with_calc_ur_matches_oracle(hip, oracle, grid, count):
+    """calc_ur folded into the first launch: the oracle runs cg_calc_ur
+    (cg.c:116-143), ppcg_init and `count` halo-update + inner steps."""
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, hd, ty = grid
+    x, y = nx + 2 * hd, ny + 2 * hd
+    n = x * y
+    rng = np.random.default_rng(500 + nx + count)
+    r, u, p, w = (rng.uniform(-1, 1, n) for _ in range(4))
+    kx, ky = rng.uniform(0.05, 0.3, n), rng.uniform(0.05, 0.3, n)
+    alphas, betas = rng.uniform(0.1, 0.9, count), rng.uniform(0.1, 0.9, count)
+    theta, alpha = 1.3, 0.37
+    r_ur, u_ur = r.copy(), u.copy()
+    rrn = np.zeros(1)
+    oracle.or2d_cg_calc_ur(x, y, hd, alpha, ptr(rrn), ptr(u_ur), ptr(p), ptr(r_ur), ptr(w))
+    want_sd, want_r, want_u = _oracle_steps(oracle, x, y, hd, alphas, betas, theta, True,
+                                            np.zeros(n), r_ur, u_ur, kx, ky)
+    d = {k: DeviceArray.from_host(v) for k, v in dict(r=r, u=u, p=p, w=w, kx=kx, ky=ky).items()}
+    out = {k: DeviceArray.from_host(np.full(n, np.nan)) for k in ("sd", "r", "u")}
+    nb = np.full(4, -1, dtype=np.int32)
+    hip.tlhip_set_fused_tuning(ty if ty else -1, -1)
+    try:
+        hip.tlhip2d_ppcg_inner_iterations(
+            x, y, hd, count, alphas.ctypes.data, betas.ctypes.data, theta, 1, nb.ctypes.data,
+            None, out["sd"].ptr, d["r"].ptr, out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr,
+            d["ky"].ptr, alpha, d["p"].ptr, d["w"].ptr)
+    finally:
+        hip.tlhip_set_fused_tuning(-1, -1)
+    hip.tlhip_synchronize()
+    for name, want, src in (("r", want_r, r), ("u", want_u, u), ("sd", want_sd, None)):
+        g = out[name].download()
+        np.testing.assert_array_equal(_interior(g, x, y, hd), _interior(want, x, y, hd),
+                                      err_msg=f"{name} interior")
+        if src is not None:
+            np.testing.assert_array_equal(_ring(g, x, y, hd), _ring(src, x, y, hd))
+    for v in list(d.values()) + list(out.values()):
+        v.free()
+
+
+@pytest.mark.parametrize("grid", [(250, 70, 2), (74, 29, 1), (300, 21, 3), (6, 5, 2)],
+                         ids=["250x70", "74x29h1", "300x21h3", "6x5"])
+def test_calc_pw_matches_calc_p_halo_calc_w(hip, oracle, grid):
+    """tlhip2d_cg_calc_pw against the oracle's cg_calc_p, depth-1 halo update
+    of p and cg_calc_w: p_out and w bit-exact, pw to reduction order."""
+    import ctypes
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, hd = grid
+    x, y = nx + 2 * hd, ny + 2 * hd
+    n = x * y
+    rng = np.random.default_rng(900 + nx)
+    p, r = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    kx, ky = rng.uniform(0.05, 0.3, n), rng.uniform(0.05, 0.3, n)
+    beta = 0.61
+    pw_ref = np.zeros(1)
+    p_ref, w_ref = p.copy(), np.zeros(n)
+    oracle.or2d_cg_calc_p(x, y, hd, beta, ptr(p_ref), ptr(r))
+    nbr = np.full(4, -1, dtype=np.int32)
+    flags = np.zeros(6, dtype=np.bool_)
+    flags[4] = True                                   # p
+    dummy = np.zeros(n)
+    oracle.or2d_local_halos(x, y, 1, hd, ptr(nbr), ptr(flags), ptr(dummy), ptr(dummy), ptr(dummy),
+                            ptr(dummy), ptr(p_ref), ptr(dummy))
+    oracle.or2d_cg_calc_w(x, y, hd, ptr(pw_ref), ptr(p_ref), ptr(w_ref), ptr(kx), ptr(ky))
+    d = {k: DeviceArray.from_host(v) for k, v in dict(p=p, r=r, kx=kx, ky=ky).items()}
+    p_out, w = DeviceArray.from_host(np.full(n, np.nan)), DeviceArray.from_host(np.full(n, np.nan))
+    pw = ctypes.c_double(0.0)
+    hip.tlhip2d_cg_calc_pw(x, y, hd, beta, ctypes.addressof(pw), d["p"].ptr, p_out.ptr, d["r"].ptr,
+                           w.ptr, d["kx"].ptr, d["ky"].ptr)
+    gp, gw = p_out.download(), w.download()
+    np.testing.assert_array_equal(_interior(gp, x, y, hd), _interior(p_ref, x, y, hd))
+    np.testing.assert_array_equal(_interior(gw, x, y, hd), _interior(w_ref, x, y, hd))
+    np.testing.assert_array_equal(_ring(gp, x, y, hd), _ring(p, x, y, hd))
+    terms = np.sum(np.abs(_interior(w_ref, x, y, hd) * _interior(p_ref, x, y, hd)))
+    assert abs(pw.value - pw_ref[0]) <= 1e-13 * max(terms, 1e-300)
+    for v in list(d.values()) + [p_out, w]:
         v.free()
