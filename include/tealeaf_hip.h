@@ -346,6 +346,11 @@ typedef struct {
 
 typedef struct tlhip_problem tlhip_problem;
 
+/* Whether the driver (2D) runs a CG/PPCG step's calc_p fused into the next
+ * step's calc_w (tlhip2d_cg_calc_pw) or as calc_p + halo update + calc_w.
+ * Results are identical either way.  Default off: at 4096^2 on MI355X the
+ * fused pass (48 B/cell at 4.4 TB/s) is no faster than the two. */
+int tlhip_set_cg_fusion(int on);
 /* Steps per fused PPCG launch used by the driver (1..5; 0 or 1 = one step
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 5. */
 int tlhip_set_ppcg_fusion(int steps);

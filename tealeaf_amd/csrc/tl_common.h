@@ -34,6 +34,8 @@ TlReduce tl_reduce_slots();
 int tl_fail(int code, const char* fmt, ...);
 // Check the last launch; returns 0 or records the error.
 int tl_check_launch(const char* what);
+// Output rows per tile set by tlhip_set_tile_rows (<= 0: the kernel's default).
+int tl_tile_rows_override();
 // Wait for the stream and copy `n` published reduction values out.
 int tl_fetch(int n, double* out, const char* what);
 int tl_ensure_init();

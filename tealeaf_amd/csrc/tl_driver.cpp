@@ -36,6 +36,7 @@ namespace {
 enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 
 int g_ppcg_fusion = 5;   // inner steps per fused launch (2D, single chunk)
+int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
 
 double* dalloc(long n, int* rc)
 {
@@ -207,7 +208,7 @@ struct tlhip_problem {
     // one pass (p ping-pongs with p_alt); the reference's halo update of p in
     // between is the kernel's mirror.  The results are the reference's bit
     // for bit.
-    bool can_defer_p() const { return dims == 2 && x % 2 == 0 && p_alt; }
+    bool can_defer_p() const { return g_cg_fusion && dims == 2 && x % 2 == 0 && p_alt; }
 
     int calc_w_pending(double* pw)
     {
@@ -889,6 +890,12 @@ struct tlhip_problem {
             if (a) (void)tlhip_free(a);
     }
 };
+
+extern "C" int tlhip_set_cg_fusion(int on)
+{
+    g_cg_fusion = on ? 1 : 0;
+    return 0;
+}
 
 extern "C" int tlhip_set_ppcg_fusion(int steps)
 {

@@ -242,88 +242,79 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ur
     }
 }
 
-// w = A p, sum w*p (cg.c:88-113) on a row march with the depth-1 mirror
-// built in; with UPDATE, p is first replaced by beta p + r (cg_calc_p,
-// cg.c:146-164), computed for every row and neighbour the stencil reads
-// and stored to a second buffer.  Both are the reference's expressions in
-// its order, so p_out and w equal cg_calc_p, a depth-1 halo update of p and
-// cg_calc_w bit for bit; the dot product differs only by summation order.
+// w = A p, sum w*p (cg.c:88-113) with the depth-1 mirror built in; with
+// UPDATE, p is first replaced by beta p + r (cg_calc_p, cg.c:146-164),
+// computed at every cell the stencil reads and stored to a second buffer.
+// Both are the reference's expressions in its order, so p_out and w equal
+// cg_calc_p, a depth-1 halo update of p and cg_calc_w bit for bit; the dot
+// product differs only by summation order.  Layout as the single-step
+// stencils: a block is kWaves rows of one strip of 64*N columns (strips
+// start on a multiple of N), a wave one row; rows j-1, j+1 and the end
+// lanes' x neighbours are read directly (L2 serves the overlap).
 template <class T, bool UPDATE>
 __global__ void __launch_bounds__(kBlock)
-k_calc_pw(PitchGrid g, StripTile t, T beta, const T* __restrict__ p_in, T* __restrict__ p_out,
-          const T* __restrict__ r, T* __restrict__ w, const T* __restrict__ kx,
-          const T* __restrict__ ky, TlReduce rd)
+k_calc_pw(PitchGrid g, int cw0, int nbx, int nby, int ty, T beta, const T* __restrict__ p_in,
+          T* __restrict__ p_out, const T* __restrict__ r, T* __restrict__ w,
+          const T* __restrict__ kx, const T* __restrict__ ky, TlReduce rd)
 {
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
-    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
-    const int bx = tile % t.nbx, by = tile / t.nbx;
-    const int sx = bx * kWaves + wave;
+    const int tile = xcd_tile_of(blockIdx.x, nbx * nby);
+    const int sx = tile % nbx, by = tile / nbx;
+    const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
+    const int c = cw0 + sx * (TL_WAVE * N) + N * lane;
+    const bool ld_ok = c + N <= g.X;
+    const bool all_out = c >= xin0 && c + N <= xin1;
+    const bool any_out = c + N > xin0 && c < xin1;
+    const long X = g.X;
+    auto ld = [&](const T* __restrict__ f, long i) -> C {
+        return ld_ok ? ldc(f, i) : zeros<T>();
+    };
+    auto pnew = [&](long i) -> C {
+        C a = ld(p_in, i);
+        if constexpr (UPDATE) {
+            const C b = ld(r, i);
+#pragma unroll
+            for (int k = 0; k < N; ++k) a.v[k] = beta * a.v[k] + b.v[k];
+        }
+        return a;
+    };
+    auto pnew1 = [&](long i) -> T {
+        if constexpr (UPDATE) return beta * p_in[i] + r[i];
+        return p_in[i];
+    };
     double acc[1] = {0.0};
-    if (sx < t.ns) {                            // every wave reaches the block reduction
-        const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
-        const int o0 = xin0 + sx * t.w;
-        const int o1 = min(o0 + t.w, xin1);
-        const int c = t.cw0 + sx * t.w + N * lane;
-        const bool ld_ok = c >= 0 && c + N <= g.X;
-        const bool all_out = c >= o0 && c + N <= o1;
-        const bool any_out = c + N > o0 && c < o1;
-        const int j0 = yin0 + by * t.ty;
-        const int j1 = min(j0 + t.ty, yin1);
-        const long X = g.X;
-        auto ld = [&](const T* __restrict__ f, int row) -> C {
-            return ld_ok ? ldc(f, (long)row * X + c) : zeros<T>();
-        };
-        auto combine = [&](C a, const C& b) -> C {
-            if constexpr (UPDATE) {
+    const int jend = min(yin0 + (by + 1) * ty, yin1);
+    for (int j = yin0 + by * ty + wave; j < jend; j += kWaves) {   // uniform per wave
+        const long i0 = (long)j * X + c;
+        const C cen = pnew(i0);
+        // y neighbours outside the interior mirror the cell
+        const C up = j == yin1 - 1 ? cen : pnew(i0 + X);
+        const C dn = j == yin0 ? cen : pnew(i0 - X);
+        const C kxc = ld(kx, i0), kyc = ld(ky, i0), kyu = ld(ky, i0 + X);
+        T el = T(0), er = T(0), ek = T(0);
+        if (lane == 0 && c >= 1) el = pnew1(i0 - 1);
+        if (lane == TL_WAVE - 1 && c + N < g.X) {
+            er = pnew1(i0 + N);
+            ek = kx[i0 + N];
+        }
+        const C av = apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, el, er, ek);
+        if (any_out) {
+            if (all_out) {
+                stc(w, i0, av);
+                if constexpr (UPDATE) stc(p_out, i0, cen);
 #pragma unroll
-                for (int i = 0; i < N; ++i) a.v[i] = beta * a.v[i] + b.v[i];
-            }
-            return a;
-        };
-        auto pnew = [&](int row) -> C {
-            if constexpr (UPDATE) return combine(ld(p_in, row), ld(r, row));
-            return ld(p_in, row);
-        };
-        // rows j0-1 .. j0+1 exist: hd >= 1 and j0 < yin1
-        C dn = pnew(j0 - 1), cen = pnew(j0), up = pnew(j0 + 1);
-        C kxc = ld(kx, j0), kyc = ld(ky, j0), kyu = ld(ky, j0 + 1);
-        for (int j = j0; j < j1; ++j) {
-            // raw inputs of the next row, combined after this row's arithmetic
-            C pn = zeros<T>(), rn = zeros<T>(), kyn = zeros<T>(), kxn = zeros<T>();
-            if (j + 1 < j1) {                   // rows j+2 <= yin1 <= y-1
-                pn = ld(p_in, j + 2);
-                if constexpr (UPDATE) rn = ld(r, j + 2);
-                kyn = ld(ky, j + 2);
-                kxn = ld(kx, j + 1);
-            }
-            const C upm = j == yin1 - 1 ? cen : up;
-            const C dnm = j == yin0 ? cen : dn;
-            const C av = apply_a_lane(cen, upm, dnm, kxc, kyu, kyc, c, xin0, xin1);
-            if (any_out) {
-                const long i0 = (long)j * X + c;
-                if (all_out) {
-                    stc(w, i0, av);
-                    if constexpr (UPDATE) stc(p_out, i0, cen);
+                for (int k = 0; k < N; ++k) acc[0] += (double)av.v[k] * (double)cen.v[k];
+            } else {
 #pragma unroll
-                    for (int i = 0; i < N; ++i) acc[0] += (double)av.v[i] * (double)cen.v[i];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < N; ++i)
-                        if (c + i >= o0 && c + i < o1) {
-                            w[i0 + i] = av.v[i];
-                            if constexpr (UPDATE) p_out[i0 + i] = cen.v[i];
-                            acc[0] += (double)av.v[i] * (double)cen.v[i];
-                        }
-                }
+                for (int k = 0; k < N; ++k)
+                    if (c + k >= xin0 && c + k < xin1) {
+                        w[i0 + k] = av.v[k];
+                        if constexpr (UPDATE) p_out[i0 + k] = cen.v[k];
+                        acc[0] += (double)av.v[k] * (double)cen.v[k];
+                    }
             }
-            dn = cen;
-            cen = up;
-            up = combine(pn, rn);
-            kyc = kyu;
-            kyu = kyn;
-            kxc = kxn;
         }
     }
     tl_reduce_partial<1, kBlock>(acc, rd);
@@ -477,8 +468,6 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
     return dispatch_fused<T, 2>(count, a);
 }
 
-int g_pw_ty = 32;
-
 template <class T>
 int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
             double* pw, const T* p_in, T* p_out, const T* r, T* w, const T* kx, const T* ky)
@@ -495,10 +484,14 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
                "p_out and w must be distinct from the inputs");
     TL_REQUIRE(all_aligned(p_in, p_out, r, w, kx, ky), what, "fields must be 16-byte aligned");
     const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
-    const StripTile t = strip_tile<T>(g, 1, g_pw_ty, kWaves);
-    const unsigned nb = (unsigned)(t.nbx * t.nby);
-    if ((long)t.nbx * t.nby > TL_MAX_RED_BLOCKS)
-        return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    const int cw0 = hd / N * N;
+    const int nbx = (int)tl_cdiv(hd + g.nx - cw0, TL_WAVE * N);
+    int ty = tl_tile_rows_override() > 0 ? (int)tl_cdiv(tl_tile_rows_override(), kWaves) * kWaves
+                                         : kWaves;
+    while ((long)nbx * tl_cdiv(g.ny, ty) > TL_MAX_RED_BLOCKS) ty *= 2;
+    const int nby = (int)tl_cdiv(g.ny, ty);
+    const unsigned nb = (unsigned)(nbx * nby);
+    if (nbx > TL_MAX_RED_BLOCKS) return tl_fail(TLHIP_ERR_ARG, "%s: rows too long", what);
     if (update) {
         Ring3<T> rg{};
         rg.src[0] = p_in;
@@ -511,11 +504,11 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
     }
     const int slot = tl_timer_begin(what);
     if (update)
-        hipLaunchKernelGGL((k_calc_pw<T, true>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t,
-                           (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+        hipLaunchKernelGGL((k_calc_pw<T, true>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, cw0,
+                           nbx, nby, ty, (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
     else
-        hipLaunchKernelGGL((k_calc_pw<T, false>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t,
-                           (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+        hipLaunchKernelGGL((k_calc_pw<T, false>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, cw0,
+                           nbx, nby, ty, (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
     tl_timer_end(slot);
     TL_TRY(tl_check_launch(what));
     TL_TRY(tl_finish_reduce(1, nb, what));

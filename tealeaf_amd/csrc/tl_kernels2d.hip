@@ -44,19 +44,11 @@ __device__ __forceinline__ double apply_a(const Nb2<V>& n, int v)
                            - (n.kyu.v[v] * n.up.v[v] + n.kyc.v[v] * n.dn.v[v]);
 }
 
-// Reads a lane needs to compute one row (besides the rows kept in registers).
-template <int V>
-struct RowIn2 {
-    DV<V> up, kyu, kxc;
-    In4<V> in;
-    double el, er, ek;   // wave-edge neighbours: a[c0-1], a[c0+V], kx[c0+V]
-};
-
 struct Tile2 {
-    int ns;    // wave strips along x
-    int nbx;   // blocks along x (kWaves strips each)
-    int nby;   // tiles along y
-    int ty;    // rows per tile
+    int ns;    // strips along x (64*V columns each)
+    int nbx;   // blocks along x (one strip each)
+    int nby;   // blocks along y
+    int ty;    // rows per block (a multiple of kWaves)
 };
 
 // Blocks are dealt to the 8 XCDs round-robin; hand each XCD a contiguous
@@ -68,6 +60,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
     return k * q + (k < r ? k : r) + i;
 }
 
+// Stencil sweep: a block is kWaves consecutive rows of one strip of 64*V
+// columns, a wave one row, a lane V cells (one 16-byte load per row when
+// V = 2).  Each lane reads rows j-1, j, j+1 of the stencil field, ky at j
+// and j+1, kx at j and the op's own inputs directly; the rows that adjacent
+// waves and blocks share are served by L2 (blocks are dealt XCD-contiguously,
+// so vertically adjacent blocks share an XCD's L2).  x-neighbours come from
+// the adjacent lane by DPP; only the wave's end lanes read them from memory.
+// On MI355X this beats marching a wave up a tile with rows held in
+// registers: more independent rows in flight, more bytes in flight.  Halo
+// values are read as stored, as the reference reads them.
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
 k_stencil2d(TlGrid2 g, Tile2 t, const double* __restrict__ a,
@@ -75,76 +77,51 @@ k_stencil2d(TlGrid2 g, Tile2 t, const double* __restrict__ a,
 {
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
     const int tile = xcd_tile(blockIdx.x, t.nbx * t.nby);
-    const int bx = tile % t.nbx, by = tile / t.nbx;
-    const int sx = bx * kWaves + wave;
+    const int sx = tile % t.nbx, by = tile / t.nbx;
     const int xend = g.x - g.hd;
     const int c0 = g.hd + sx * (TL_WAVE * V) + lane * V;
     int nval = xend - c0;
     nval = nval < 0 ? 0 : (nval > V ? V : nval);
     const bool ld_ok = c0 + V <= g.x;        // whole vector inside the row
-    const int j0 = g.hd + by * t.ty;
-    const int j1 = min(j0 + t.ty, g.y - g.hd);
+    const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
     const long X = g.x;
     constexpr int NR = Op::NRED > 0 ? Op::NRED : 1;
     double acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
 
-    if (sx < t.ns) {   // uniform across the wave
-        // Computing row j needs the stencil field at j-1, j, j+1, ky at j and
-        // j+1, kx at j, and the op's inputs at j.  The reads of row j+1 are
-        // issued before row j is computed (two rows in flight).
-        const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
-        auto load_row = [&](RowIn2<V>& rw, int j) {
-            const long row = (long)j * X;
-            rw.el = rw.er = rw.ek = 0.0;
-            if (ld_ok) {
-                rw.up = ldv<V>(a, row + X + c0);
-                rw.kyu = ldv<V>(ky, row + X + c0);
-                rw.kxc = ldv<V>(kx, row + c0);
-                rw.in = op.template load<V>(row + c0);
-                if (lane == 0) rw.el = a[row + c0 - 1];
-            } else {
-                rw.up = rw.kyu = rw.kxc = zdv<V>();
-                rw.in = In4<V>{};
-            }
-            if (edge_r) {
-                rw.er = a[row + c0 + V];
-                rw.ek = kx[row + c0 + V];
-            }
-        };
-        DV<V> dn = zdv<V>(), cc = zdv<V>(), kyc = zdv<V>();
+    const int jend = min(g.hd + (by + 1) * t.ty, g.y - g.hd);
+    for (int j = g.hd + by * t.ty + wave; j < jend; j += kWaves) {   // uniform per wave
+        const long row = (long)j * X;
+        Nb2<V> n;
+        In4<V> in{};
+        double el = 0.0, er = 0.0, ek = 0.0;
         if (ld_ok) {
-            dn = ldv<V>(a, (long)(j0 - 1) * X + c0);
-            cc = ldv<V>(a, (long)j0 * X + c0);
-            kyc = ldv<V>(ky, (long)j0 * X + c0);
+            n.c = ldv<V>(a, row + c0);
+            n.up = ldv<V>(a, row + X + c0);
+            n.dn = ldv<V>(a, row - X + c0);
+            n.kxc = ldv<V>(kx, row + c0);
+            n.kyc = ldv<V>(ky, row + c0);
+            n.kyu = ldv<V>(ky, row + X + c0);
+            in = op.template load<V>(row + c0);
+            if (lane == 0) el = a[row + c0 - 1];
+        } else {
+            n.c = n.up = n.dn = n.kxc = n.kyc = n.kyu = zdv<V>();
         }
-        RowIn2<V> nxt;
-        load_row(nxt, j0);
-        for (int j = j0; j < j1; ++j) {
-            const RowIn2<V> cur = nxt;
-            if (j + 1 < j1) load_row(nxt, j + 1);
-            Nb2<V> n;
-            n.c = cc;
-            n.up = cur.up;
-            n.dn = dn;
-            n.kxc = cur.kxc;
-            n.kyc = kyc;
-            n.kyu = cur.kyu;
-            const double al = tl_lane_prev(cc.v[V - 1], cur.el);
-            const double ar = tl_lane_next(cc.v[0], cur.er);
-            const double kr = tl_lane_next(cur.kxc.v[0], cur.ek);
+        if (edge_r) {
+            er = a[row + c0 + V];
+            ek = kx[row + c0 + V];
+        }
+        const double al = tl_lane_prev(n.c.v[V - 1], el);
+        const double ar = tl_lane_next(n.c.v[0], er);
+        const double kr = tl_lane_next(n.kxc.v[0], ek);
 #pragma unroll
-            for (int v = 0; v < V; ++v) {
-                n.l.v[v] = v == 0 ? al : cc.v[v - 1];
-                n.r.v[v] = v == V - 1 ? ar : cc.v[v + 1];
-                n.kxr.v[v] = v == V - 1 ? kr : cur.kxc.v[v + 1];
-            }
-            if (nval > 0) op.template apply<V>((long)j * X + c0, nval, n, cur.in, acc);
-            dn = cc;
-            cc = cur.up;
-            kyc = cur.kyu;
+        for (int v = 0; v < V; ++v) {
+            n.l.v[v] = v == 0 ? al : n.c.v[v - 1];
+            n.r.v[v] = v == V - 1 ? ar : n.c.v[v + 1];
+            n.kxr.v[v] = v == V - 1 ? kr : n.kxc.v[v + 1];
         }
+        if (nval > 0) op.template apply<V>(row + c0, nval, n, in, acc);
     }
     if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
 }
@@ -398,20 +375,16 @@ int pick_v(const TlGrid2& g, P... ps)
     return (g.x % 2 == 0 && g.hd % 2 == 0 && g.hd >= 2 && all_aligned(ps...)) ? 2 : 1;
 }
 
+// One block per kWaves rows (or a multiple: t.ty rows, each wave every
+// kWaves-th) of one strip; rows per block grow only to keep the block count
+// within the reduction slots.
 Tile2 make_tile(const TlGrid2& g, int V)
 {
     Tile2 t;
     const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd;
     t.ns = (int)tl_cdiv(nx, TL_WAVE * V);
-    t.nbx = (int)tl_cdiv(t.ns, kWaves);
-    int ty;
-    if (g_ty_override > 0) {
-        ty = g_ty_override;
-    } else {
-        // Aim for ~2048 blocks (8 waves per SIMD), 4..64 rows per tile.
-        ty = (int)tl_cdiv((long)ny * t.nbx, 2048);
-        ty = ty < 4 ? 4 : (ty > 64 ? 64 : ty);
-    }
+    t.nbx = t.ns;
+    int ty = g_ty_override > 0 ? (int)tl_cdiv(g_ty_override, kWaves) * kWaves : kWaves;
     while ((long)t.nbx * tl_cdiv(ny, ty) > TL_MAX_RED_BLOCKS) ty *= 2;
     t.ty = ty;
     t.nby = (int)tl_cdiv(ny, ty);
@@ -423,7 +396,7 @@ int launch_stencil(const TlGrid2& g, int V, const double* a, const double* kx,
                    const double* ky, const Op& op, const char* what)
 {
     if (g.x - 2 * g.hd <= 0 || g.y - 2 * g.hd <= 0) return 0;
-    if (tl_cdiv(tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V), kWaves) > TL_MAX_RED_BLOCKS)
+    if (tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V) > TL_MAX_RED_BLOCKS)
         return tl_fail(TLHIP_ERR_ARG, "%s: rows too long (%d)", what, g.x);
     const Tile2 t = make_tile(g, V);
     const unsigned nb = (unsigned)(t.nbx * t.nby);
@@ -451,6 +424,8 @@ bool interior_empty(const TlGrid2& g) { return g.x - 2 * g.hd <= 0 || g.y - 2 * 
 #define REQUIRE TL_REQUIRE
 
 }  // namespace
+
+int tl_tile_rows_override() { return g_ty_override; }
 
 extern "C" int tlhip_set_tile_rows(int rows)
 {

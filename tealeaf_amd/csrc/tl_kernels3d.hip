@@ -39,14 +39,6 @@ __device__ __forceinline__ double apply_a(const Nb3<V>& n, int v)
                            - (n.kzu.v[v] * n.up.v[v] + n.kzc.v[v] * n.dn.v[v]);
 }
 
-// Reads a lane needs to compute one plane besides the planes in registers.
-template <int V>
-struct RowIn3 {
-    DV<V> up, kzu, kxc, kyc, kyn, no, so;
-    In4<V> in;
-    double el, er, ek;   // wave-edge neighbours: a[c0-1], a[c0+V], kx[c0+V]
-};
-
 struct Tile3 {
     int ns;    // strips along x
     int nbj;   // blocks along y (kWaves rows each)
@@ -60,6 +52,14 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
     return k * q + (k < r ? k : r) + i;
 }
 
+// Stencil sweep: a block is kWaves consecutive rows of one 64*V-column
+// strip in tz consecutive planes; a wave one row, a lane V cells.  Each lane
+// reads the field at planes k-1, k, k+1 and rows j-1, j+1 of plane k, kz at k
+// and k+1, kx and ky at k (ky also at row j+1) and the op's inputs directly;
+// what adjacent waves and blocks share is served by L2 (blocks are dealt
+// XCD-contiguously, so each XCD sweeps a run of consecutive planes).
+// x-neighbours come from the adjacent lane by DPP; only the wave's end lanes
+// read them from memory.  Halo values are read as stored.
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
 k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __restrict__ kx,
@@ -76,8 +76,7 @@ k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __re
     int nval = xend - c0;
     nval = nval < 0 ? 0 : (nval > V ? V : nval);
     const bool ld_ok = c0 + V <= g.x;
-    const int k0 = g.hd + kb * t.tz;
-    const int k1 = min(k0 + t.tz, g.z - g.hd);
+    const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
     const long X = g.x, XY = (long)g.x * g.y;
     constexpr int NR = Op::NRED > 0 ? Op::NRED : 1;
     double acc[NR];
@@ -85,68 +84,44 @@ k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __re
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
 
     if (j < g.y - g.hd) {   // uniform across the wave
-        // Plane k needs the field at k-1, k, k+1 and at rows j-1, j+1 of
-        // plane k, kz at k and k+1, kx and ky at k (ky also at row j+1), and
-        // the op's inputs; plane k+1's reads are issued before k is computed.
         const long jrow = (long)j * X + c0;
-        const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
-        auto load_plane = [&](RowIn3<V>& rw, int k) {
+        const int k1 = min(g.hd + (kb + 1) * t.tz, g.z - g.hd);
+        for (int k = g.hd + kb * t.tz; k < k1; ++k) {
             const long b = (long)k * XY + jrow;
-            rw.el = rw.er = rw.ek = 0.0;
+            Nb3<V> n;
+            In4<V> in{};
+            double el = 0.0, er = 0.0, ek = 0.0;
             if (ld_ok) {
-                rw.up = ldv<V>(a, b + XY);
-                rw.kzu = ldv<V>(kz, b + XY);
-                rw.kxc = ldv<V>(kx, b);
-                rw.kyc = ldv<V>(ky, b);
-                rw.kyn = ldv<V>(ky, b + X);
-                rw.no = ldv<V>(a, b + X);
-                rw.so = ldv<V>(a, b - X);
-                rw.in = op.template load<V>(b);
-                if (lane == 0) rw.el = a[b - 1];
+                n.c = ldv<V>(a, b);
+                n.up = ldv<V>(a, b + XY);
+                n.dn = ldv<V>(a, b - XY);
+                n.no = ldv<V>(a, b + X);
+                n.so = ldv<V>(a, b - X);
+                n.kxc = ldv<V>(kx, b);
+                n.kyc = ldv<V>(ky, b);
+                n.kyn = ldv<V>(ky, b + X);
+                n.kzc = ldv<V>(kz, b);
+                n.kzu = ldv<V>(kz, b + XY);
+                in = op.template load<V>(b);
+                if (lane == 0) el = a[b - 1];
             } else {
-                rw.up = rw.kzu = rw.kxc = rw.kyc = rw.kyn = rw.no = rw.so = zdv<V>();
-                rw.in = In4<V>{};
+                n.c = n.up = n.dn = n.no = n.so = zdv<V>();
+                n.kxc = n.kyc = n.kyn = n.kzc = n.kzu = zdv<V>();
             }
             if (edge_r) {
-                rw.er = a[b + V];
-                rw.ek = kx[b + V];
+                er = a[b + V];
+                ek = kx[b + V];
             }
-        };
-        DV<V> dn = zdv<V>(), cc = zdv<V>(), kzc = zdv<V>();
-        if (ld_ok) {
-            dn = ldv<V>(a, (long)(k0 - 1) * XY + jrow);
-            cc = ldv<V>(a, (long)k0 * XY + jrow);
-            kzc = ldv<V>(kz, (long)k0 * XY + jrow);
-        }
-        RowIn3<V> nxt;
-        load_plane(nxt, k0);
-        for (int k = k0; k < k1; ++k) {
-            const RowIn3<V> cur = nxt;
-            if (k + 1 < k1) load_plane(nxt, k + 1);
-            Nb3<V> n;
-            n.c = cc;
-            n.up = cur.up;
-            n.dn = dn;
-            n.no = cur.no;
-            n.so = cur.so;
-            n.kxc = cur.kxc;
-            n.kyc = cur.kyc;
-            n.kyn = cur.kyn;
-            n.kzc = kzc;
-            n.kzu = cur.kzu;
-            const double al = tl_lane_prev(cc.v[V - 1], cur.el);
-            const double ar = tl_lane_next(cc.v[0], cur.er);
-            const double kr = tl_lane_next(cur.kxc.v[0], cur.ek);
+            const double al = tl_lane_prev(n.c.v[V - 1], el);
+            const double ar = tl_lane_next(n.c.v[0], er);
+            const double kr = tl_lane_next(n.kxc.v[0], ek);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                n.l.v[v] = v == 0 ? al : cc.v[v - 1];
-                n.r.v[v] = v == V - 1 ? ar : cc.v[v + 1];
-                n.kxr.v[v] = v == V - 1 ? kr : cur.kxc.v[v + 1];
+                n.l.v[v] = v == 0 ? al : n.c.v[v - 1];
+                n.r.v[v] = v == V - 1 ? ar : n.c.v[v + 1];
+                n.kxr.v[v] = v == V - 1 ? kr : n.kxc.v[v + 1];
             }
-            if (nval > 0) op.template apply<V>((long)k * XY + jrow, nval, n, cur.in, acc);
-            dn = cc;
-            cc = cur.up;
-            kzc = cur.kzu;
+            if (nval > 0) op.template apply<V>(b, nval, n, in, acc);
         }
     }
     if constexpr (Op::NRED > 0) tl_reduce_partial<Op::NRED, kBlock>(acc, rd);
@@ -463,13 +438,7 @@ Tile3 make_tile(const TlGrid3& g, int V)
     const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
     t.ns = (int)tl_cdiv(nx, TL_WAVE * V);
     t.nbj = (int)tl_cdiv(ny, kWaves);
-    int tz;
-    if (g_tz_override > 0) {
-        tz = g_tz_override;
-    } else {
-        tz = (int)tl_cdiv((long)nz * t.ns * t.nbj, 2048);
-        tz = tz < 4 ? 4 : (tz > 64 ? 64 : tz);
-    }
+    int tz = g_tz_override > 0 ? g_tz_override : 1;
     while ((long)t.ns * t.nbj * tl_cdiv(nz, tz) > TL_MAX_RED_BLOCKS) tz *= 2;
     t.tz = tz;
     t.nbk = (int)tl_cdiv(nz, tz);

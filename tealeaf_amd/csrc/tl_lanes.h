@@ -87,16 +87,18 @@ __device__ __forceinline__ int xcd_tile_of(int bid, int nb)
 // A * a for the N cells of a lane (2d/shared.h:50-54, same association), with
 // x neighbours outside the interior mirroring the cell (a depth-1 reflective
 // halo).  up/dn are the rows above/below, already mirrored by the caller.
+// The first lane's left neighbour is el and the last lane's right neighbour
+// and kx are er, ek (they do not exist in the wave).
 template <class T>
 __device__ __forceinline__ Cells<T> apply_a_lane(const Cells<T>& cen, const Cells<T>& up,
                                                  const Cells<T>& dn, const Cells<T>& kxc,
                                                  const Cells<T>& kyu, const Cells<T>& kyc, int c,
-                                                 int xin0, int xin1)
+                                                 int xin0, int xin1, T el, T er, T ek)
 {
     constexpr int N = Lane16<T>::N;
-    const T lft_edge = tl_lane_prev(cen.v[N - 1], cen.v[0]);
-    const T rgt_edge = tl_lane_next(cen.v[0], cen.v[N - 1]);
-    const T kx_edge = tl_lane_next(kxc.v[0], kxc.v[N - 1]);
+    const T lft_edge = tl_lane_prev(cen.v[N - 1], el);
+    const T rgt_edge = tl_lane_next(cen.v[0], er);
+    const T kx_edge = tl_lane_next(kxc.v[0], ek);
     Cells<T> a;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -109,6 +111,19 @@ __device__ __forceinline__ Cells<T> apply_a_lane(const Cells<T>& cen, const Cell
         a.v[i] = d * cen.v[i] - (kxr * rgt + kxc.v[i] * lft) - (kyu.v[i] * up.v[i] + kyc.v[i] * dn.v[i]);
     }
     return a;
+}
+
+// The same where the wave's end lanes never produce kept results (windows
+// that overlap their neighbours): the edge neighbours are don't-cares.
+template <class T>
+__device__ __forceinline__ Cells<T> apply_a_lane(const Cells<T>& cen, const Cells<T>& up,
+                                                 const Cells<T>& dn, const Cells<T>& kxc,
+                                                 const Cells<T>& kyu, const Cells<T>& kyc, int c,
+                                                 int xin0, int xin1)
+{
+    constexpr int N = Lane16<T>::N;
+    return apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, cen.v[0], cen.v[N - 1],
+                        kxc.v[N - 1]);
 }
 
 }  // namespace tl
