@@ -12,6 +12,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <chrono>
 #include <vector>
 
 namespace {
@@ -22,6 +24,7 @@ struct Runtime {
     hipStream_t stream = nullptr;
     int own_stream = 0;
     TlReduce red{};
+    unsigned long long seq = 0;     // finisher launches so far (see tl_fetch)
     double* staging = nullptr;      // device scratch for pack/unpack
     size_t staging_len = 0;
 };
@@ -100,10 +103,32 @@ int tl_check_launch(const char* what)
     return 0;
 }
 
+// The mailbox holds the last finisher's results in doubles [0, TL_MAX_RED)
+// and, at kMailSeq, the sequence number of that finisher launch, stored
+// after the results with system-scope release.  Waiting for the number to
+// arrive is a few microseconds quicker than a stream synchronisation, and
+// the drivers wait for two scalars per CG iteration.  A finisher that does
+// not report within 2 ms falls back to synchronising the stream, which
+// also surfaces any launch error.
+constexpr int kMailSeq = 8;
+
 int tl_fetch(int n, double* out, const char* what)
 {
-    hipError_t e = hipStreamSynchronize(g_rt.stream);
-    if (e != hipSuccess) return hip_fail(e, what);
+    volatile unsigned long long* seq = (volatile unsigned long long*)(g_rt.red.mailbox + kMailSeq);
+    if (*seq != g_rt.seq) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned spin = 1; *seq != g_rt.seq; ++spin) {
+            if ((spin & 1023u) == 0 &&
+                std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+                hipError_t e = hipStreamSynchronize(g_rt.stream);
+                if (e != hipSuccess) return hip_fail(e, what);
+                if (*seq != g_rt.seq)
+                    return tl_fail(TLHIP_ERR_HIP, "%s: reduction result not published", what);
+                break;
+            }
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     for (int i = 0; i < n; ++i) out[i] = ((volatile double*)g_rt.red.mailbox)[i];
     return 0;
 }
@@ -278,7 +303,8 @@ int tl_staging(size_t doubles, double** out)
 // Grid total of the per-block partials, summed in a fixed order: thread t
 // takes slots t, t+256, ... in turn, then a fixed tree over the block.
 template <int NV>
-__global__ void __launch_bounds__(256) k_finish_reduce(unsigned nblocks, TlReduce rd)
+__global__ void __launch_bounds__(256)
+k_finish_reduce(unsigned nblocks, TlReduce rd, unsigned long long seq)
 {
     constexpr int NW = 256 / TL_WAVE;
     __shared__ double s_part[NW][NV];
@@ -296,12 +322,18 @@ __global__ void __launch_bounds__(256) k_finish_reduce(unsigned nblocks, TlReduc
         if (lane == 0) s_part[wave][i] = s;
     }
     __syncthreads();
-    if (tid < NV) {
-        double s = 0.0;
+    if (tid == 0) {
 #pragma unroll
-        for (int w = 0; w < NW; ++w) s += s_part[w][tid];
-        rd.result[tid] = s;
-        __hip_atomic_store(&rd.mailbox[tid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += s_part[w][i];
+            rd.result[i] = s;
+            __hip_atomic_store(&rd.mailbox[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        // published after the results (tl_fetch waits for it)
+        __hip_atomic_store((unsigned long long*)(rd.mailbox + kMailSeq), seq, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -310,13 +342,18 @@ int tl_finish_reduce(int nv, unsigned nblocks, const char* what)
     if (nblocks > TL_MAX_RED_BLOCKS)
         return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
                        TL_MAX_RED_BLOCKS);
-    if (nv == 1)
-        hipLaunchKernelGGL(k_finish_reduce<1>, dim3(1), dim3(256), 0, g_rt.stream, nblocks, g_rt.red);
-    else if (nv == 4)
-        hipLaunchKernelGGL(k_finish_reduce<4>, dim3(1), dim3(256), 0, g_rt.stream, nblocks, g_rt.red);
-    else
+    if (nv != 1 && nv != 4)
         return tl_fail(TLHIP_ERR_ARG, "%s: unsupported reduction width %d", what, nv);
-    return tl_check_launch(what);
+    const unsigned long long seq = g_rt.seq + 1;
+    if (nv == 1)
+        hipLaunchKernelGGL(k_finish_reduce<1>, dim3(1), dim3(256), 0, g_rt.stream, nblocks,
+                           g_rt.red, seq);
+    else
+        hipLaunchKernelGGL(k_finish_reduce<4>, dim3(1), dim3(256), 0, g_rt.stream, nblocks,
+                           g_rt.red, seq);
+    TL_TRY(tl_check_launch(what));
+    g_rt.seq = seq;
+    return 0;
 }
 
 namespace tl {
