@@ -5,12 +5,14 @@
 //    (+ ii*x*y in 3D), interior [hd, n-hd) on each axis;
 //  * one in-order HIP stream per process (tl_stream()); no kernel ever waits
 //    on another;
-//  * scalar results use a single-pass, deterministic grid reduction: every
-//    block writes its partial to a fixed slot, the last block to arrive (an
-//    agent-scope ticket) sums the slots in index order and publishes the
-//    result to device memory and to a pinned host mailbox.  The summation
-//    order depends only on the launch shape, so results are bitwise
-//    reproducible run to run.
+//  * scalar results use a deterministic two-kernel grid reduction: every
+//    block writes its partial to a fixed slot, then a one-block finisher
+//    sums the slots in index order and publishes the result to device
+//    memory and to a pinned host mailbox.  The summation order depends only
+//    on the launch shape, so results are bitwise reproducible run to run;
+//  * a driver that runs iterations ahead of the host (tl_loop.h) gates the
+//    kernels it launches: each takes a TlGate and does nothing once the
+//    device has decided that its iteration is past the end of the loop.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,11 +24,35 @@
 struct TlReduce {
     double* partials;         // [max_blocks * TL_MAX_RED], one slot per block
     double* result;           // [TL_MAX_RED] device copy of the last result
-    double* mailbox;          // pinned, host-mapped [TL_MAX_RED]
+    double* mailbox;          // pinned, host-mapped [TL_MAIL_SLOTS * TL_MAX_RED]
 };
 
 #define TL_MAX_RED 4
 #define TL_MAX_RED_BLOCKS 65536
+#define TL_MAIL_SLOTS 8
+
+// Pinned result slots are armed by the host with this signalling-NaN bit
+// pattern before the kernel that fills them is launched; a slot is ready
+// once its bits differ.  No arithmetic result is a signalling NaN, so a
+// finisher's value can never be mistaken for the mark.  Each value is
+// thereby its own ready flag, and the device publishes with plain
+// system-scope stores, without the release (a write-back of the XCD's L2)
+// that a separate ready flag would need.
+#define TL_MAIL_EMPTY 0x7FF4DEADBEEF0001ULL
+
+// Launch gate of the device-resident solver loops: a kernel of iteration
+// `iter` returns at once when *last < iter, where *last is the last
+// iteration the device decided to run (INT_MAX while the loop goes on).
+// The value is written only by one-block finisher kernels that run
+// strictly before or after the gated kernel on the same stream, so a plain
+// load sees a value that is constant for the kernel's lifetime.
+struct TlGate {
+    const int* last;   // null: always run
+    int iter;
+    __device__ __forceinline__ bool closed() const { return last && iter > *last; }
+};
+// The gate that launches issued now should carry (null outside a loop).
+TlGate tl_gate();
 
 hipStream_t tl_stream();
 TlReduce tl_reduce_slots();
@@ -125,6 +151,42 @@ __device__ __forceinline__ void tl_reduce_partial(const double* v, TlReduce rd)
 #pragma unroll
         for (int w = 0; w < NW; ++w) s += s_part[w][tid];
         rd.partials[(size_t)bid * NV + tid] = s;
+    }
+}
+
+// Grid total of `nblocks` per-block partials of NV values, in a fixed
+// order: thread t of a 256-thread block takes slots t, t+256, ... in turn,
+// then the waves' butterfly sums, then the four waves in order.  Every
+// finisher uses this, so a value is the same bits whichever finisher formed
+// it.  Must be called by all 256 threads; thread 0 receives the totals.
+template <int NV>
+__device__ __forceinline__ void tl_sum_partials(unsigned nblocks, const double* partials,
+                                                double* out)
+{
+    constexpr int NW = 256 / TL_WAVE;
+    __shared__ double s_tot[NW][NV];
+    const int tid = threadIdx.x, lane = tid & (TL_WAVE - 1), wave = tid / TL_WAVE;
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (unsigned b = tid; b < nblocks; b += 256) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += partials[(size_t)b * NV + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double s = tl_wave_sum(acc[i]);
+        if (lane == 0) s_tot[wave][i] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += s_tot[w][i];
+            out[i] = s;
+        }
     }
 }
 

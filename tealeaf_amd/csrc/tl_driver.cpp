@@ -10,8 +10,11 @@
 // Chebyshev coefficient arrays persist across timesteps, and alpha/beta and
 // the eigenvalue estimate are computed on the host in FP64.
 //
-// Every field lives on the device.  The only device->host traffic is the
-// scalars the algorithm needs on the host (two per CG iteration).
+// Every field lives on the device.  The CG and PPCG iterations run
+// device-resident (tl_loop.h): alpha, beta and the stopping rule are
+// evaluated on the device, the host enqueues iterations ahead and reads one
+// small report per iteration, and the results are those of the host-driven
+// loop bit for bit (tlhip_set_resident(0) selects that loop).
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -21,6 +24,7 @@
 
 #include "tealeaf_hip.h"
 #include "tl_adjoint.h"
+#include "tl_loop.h"
 #include "tl_mixed.h"
 
 int tl_fail(int code, const char* fmt, ...);
@@ -37,6 +41,7 @@ enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 
 int g_ppcg_fusion = 5;   // inner steps per fused launch (2D, single chunk)
 int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
+int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 
 double* dalloc(long n, int* rc)
 {
@@ -311,17 +316,187 @@ struct tlhip_problem {
         return 0;
     }
 
+    // -------------------------------------------------------- resident loops
+    // CG / PPCG iterations enqueued ahead of the host (tl_loop.h).  The
+    // bodies launch the same kernels as cg_main_step / ppcg_main_step with
+    // the same arguments, gated and with alpha and beta read on the device.
+    bool interior() const
+    {
+        return x > 2 * hd && y > 2 * hd && (dims == 2 || z > 2 * hd);
+    }
+    bool resident_cg() const { return g_resident && !can_defer_p() && interior(); }
+    bool resident_ppcg() const
+    {
+        return resident_cg() && fused_ppcg() && s.ppcg_inner_steps > 0;
+    }
+
+    // The stop test of stop() and, with sw, the switch_now test of the next
+    // iteration (sw 1: the error switch, 2: presteps), evaluated on the device.
+    tl::loop::StopRule stop_rule(int sw) const
+    {
+        tl::loop::StopRule r{};
+        r.eps = s.eps;
+        r.rel = rel_tol > 0.0;
+        r.rel_target = rel_tol * rel_tol * bb;
+        r.sqrt_err = dims == 3 && !r.rel;
+        r.sw = sw;
+        r.eps_lim = s.eps_lim;
+        r.presteps = s.presteps;
+        return r;
+    }
+
+    // What the host would have decided from the same rrn; a device report
+    // that disagrees is an error, not a silently different solve.
+    int check_report(const tl::loop::Report& rp, int j, int sw) const
+    {
+        const double err = dims == 3 && rel_tol <= 0.0 ? std::sqrt(rp.rrn) : rp.rrn;
+        int want = 0;
+        const bool conv = rel_tol > 0.0 ? rp.rrn <= rel_tol * rel_tol * bb : std::fabs(err) < s.eps;
+        if (conv) want |= tl::loop::kConverged;
+        if (sw && switch_now(0, j + 1, err)) want |= tl::loop::kSwitch;
+        const bool same_err = err == rp.err || (err != err && rp.err != rp.err);
+        if (want != rp.flags || !same_err)
+            return tl_fail(TLHIP_ERR_SOLVER,
+                           "resident loop: device report of iteration %d differs from the host's "
+                           "test (flags %d, host %d)", j, rp.flags, want);
+        return 0;
+    }
+
+    int cg_body(const tl::loop::StopRule& rule)
+    {
+        if (dims == 2) {
+            TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
+            TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
+            TRY(tl::loop2d::calc_p(x, y, hd, p, r));
+        } else {
+            TRY(tl::loop3d::calc_w(x, y, z, hd, p, w, kx, ky, kz));
+            TRY(tl::loop3d::calc_ur(x, y, z, hd, u, p, r, w, rule));
+            TRY(tl::loop3d::calc_p(x, y, z, hd, p, r));
+        }
+        return halo_update(1);
+    }
+
+    // ppcg_main_step's fused path (2D), then its halo update of p.
+    int ppcg_body(const tl::loop::StopRule& rule)
+    {
+        TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
+        int done = 0;
+        while (done < s.ppcg_inner_steps) {
+            const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
+            const bool first = done == 0;
+            TRY(tl::loop2d::ppcg_inner(x, y, hd, cnt, &cheby_alphas[done], &cheby_betas[done],
+                                       theta, first, neighbours, first ? nullptr : sd, sd_alt, r,
+                                       r_alt, u, u_alt, kx, ky, first ? p : nullptr,
+                                       first ? w : nullptr));
+            std::swap(sd, sd_alt);
+            std::swap(r, r_alt);
+            std::swap(u, u_alt);
+            done += cnt;
+        }
+        reset_fields();
+        fields[F_P] = true;
+        TRY(tl::loop2d::norm(x, y, hd, r, rule));
+        TRY(tl::loop2d::calc_p(x, y, hd, p, r));
+        return halo_update(1);
+    }
+
+    struct Ptrs {
+        double *u, *u_alt, *r, *r_alt, *sd, *sd_alt;
+    };
+
+    // Iterations t0, t0+1, ... of `body`, up to kAhead beyond the one whose
+    // report the host waits for, until a report carries a flag or the
+    // iteration max_iters - 1 has reported.  *last and *out: that
+    // iteration and its report; the field pointers are restored to what it
+    // left (later iterations were enqueued but their gate was closed), and
+    // *rro / *error take its rrn / err.  CG runs also copy their alpha and
+    // beta into cg_alphas / cg_betas for the eigenvalue estimate.
+    template <class Body>
+    int run_resident(int t0, int sw, bool cg, Body&& body, double* rro, double* error, int* last,
+                     tl::loop::Report* out)
+    {
+        constexpr int kSnap = tl::loop::kAhead + 2;
+        Ptrs snap[kSnap];
+        TRY(tl::loop::begin(*rro, s.max_iters));
+        struct Close {
+            ~Close() { tl::loop::end(); }
+        } close;
+        const tl::loop::StopRule rule = stop_rule(sw);
+        int next = t0, chk = t0;
+        tl::loop::Report rp{};
+        for (;;) {
+            while (next < s.max_iters && next <= chk + tl::loop::kAhead) {
+                tl::loop::set_iter(next);
+                TRY(body(rule));
+                snap[next % kSnap] = Ptrs{u, u_alt, r, r_alt, sd, sd_alt};
+                ++next;
+            }
+            TRY(tl::loop::wait(chk, &rp));
+            TRY(check_report(rp, chk, sw));
+            if (rp.flags || chk + 1 >= s.max_iters) break;
+            ++chk;
+        }
+        const Ptrs& q = snap[chk % kSnap];
+        u = q.u, u_alt = q.u_alt, r = q.r, r_alt = q.r_alt, sd = q.sd, sd_alt = q.sd_alt;
+        if (cg) TRY(tl::loop::logs(t0, chk - t0 + 1, &cg_alphas[t0], &cg_betas[t0]));
+        *last = chk;
+        *out = rp;
+        *rro = rp.rrn;
+        *error = rp.err;
+        return 0;
+    }
+
+    // The stop() decision for a flagged report (the true residual confirms a
+    // relative-residual candidate).
+    int confirm(const tl::loop::Report& rp, bool* done)
+    {
+        *done = false;
+        if (!(rp.flags & tl::loop::kConverged)) return 0;
+        if (rel_tol <= 0.0) {
+            *done = true;
+            return 0;
+        }
+        double rr;
+        TRY(true_residual(&rr));
+        *done = rr <= rel_tol * rel_tol * bb;
+        return 0;
+    }
+
+    // CG iterations from t0 until the stop test passes (*done, *tt = that
+    // iteration), the switch test of the next iteration passes (*tt = the
+    // last CG iteration), or max_iters - 1 has run (*tt = max_iters - 1).
+    int cg_resident(int t0, int sw, double* rro, double* error, int* tt, bool* done)
+    {
+        for (int from = t0;;) {
+            int last;
+            tl::loop::Report rp;
+            TRY(run_resident(from, sw, true, [&](const tl::loop::StopRule& r) { return cg_body(r); },
+                             rro, error, &last, &rp));
+            rep->n_cg_steps += last - from + 1;
+            *tt = last;
+            TRY(confirm(rp, done));
+            if (*done || (rp.flags & tl::loop::kSwitch) || last + 1 >= s.max_iters) return 0;
+            from = last + 1;    // a relative-residual candidate the true residual refused
+        }
+    }
+
     int cg_driver(double rx, double ry, double rz, double* error)
     {
         double rro;
         TRY(cg_init_driver(rx, ry, rz, &rro));
         int tt;
-        for (tt = 0; tt < s.max_iters; ++tt) {
-            TRY(cg_main_step(tt, &rro, error));
-            TRY(halo_update(1));
+        if (resident_cg()) {
             bool done;
-            TRY(stop(error, &done));
-            if (done) break;
+            TRY(cg_resident(0, 0, &rro, error, &tt, &done));
+            if (!done) tt = s.max_iters;
+        } else {
+            for (tt = 0; tt < s.max_iters; ++tt) {
+                TRY(cg_main_step(tt, &rro, error));
+                TRY(halo_update(1));
+                bool done;
+                TRY(stop(error, &done));
+                if (done) break;
+            }
         }
         TRY(flush_p());
         rep->printed_cg_iters = tt;
@@ -409,6 +584,13 @@ struct tlhip_problem {
         TRY(cg_init_driver(rx, ry, rz, &rro));
         int est = 0, ncheby = 0, tt;
         for (tt = 0; tt < s.max_iters; ++tt) {
+            if (!switch_now(ncheby, tt, *error) && resident_cg()) {
+                // the CG phase up to the switch; the loop goes on at tt + 1
+                bool done;
+                TRY(cg_resident(tt, s.error_switch ? 1 : 2, &rro, error, &tt, &done));
+                if (done) break;
+                continue;
+            }
             if (!switch_now(ncheby, tt, *error)) {
                 TRY(cg_main_step(tt, &rro, error));
             } else if (++ncheby == 1) {
@@ -517,6 +699,12 @@ struct tlhip_problem {
         TRY(cg_init_driver(rx, ry, rz, &rro));
         int nppcg = 0, tt;
         for (tt = 0; tt < s.max_iters; ++tt) {
+            if (!switch_now(nppcg, tt, *error) && resident_cg()) {
+                bool done;
+                TRY(cg_resident(tt, s.error_switch ? 1 : 2, &rro, error, &tt, &done));
+                if (done) break;
+                continue;
+            }
             if (!switch_now(nppcg, tt, *error)) {
                 TRY(cg_main_step(tt, &rro, error));
             } else {
@@ -528,6 +716,23 @@ struct tlhip_problem {
                     reset_fields();
                     fields[F_P] = true;
                     TRY(halo_update(1));
+                }
+                if (resident_ppcg()) {
+                    // PPCG iterations tt .. last; the loop goes on at last + 1
+                    int last;
+                    tl::loop::Report rp;
+                    TRY(run_resident(tt, 0, false,
+                                     [&](const tl::loop::StopRule& r) { return ppcg_body(r); },
+                                     &rro, error, &last, &rp));
+                    const int ran = last - tt + 1;
+                    rep->n_ppcg_outer += ran;
+                    rep->n_ppcg_inner += ran * s.ppcg_inner_steps;
+                    nppcg += ran - 1;
+                    tt = last;
+                    bool done;
+                    TRY(confirm(rp, &done));
+                    if (done) break;
+                    continue;
                 }
                 TRY(ppcg_main_step(&rro, error));
             }
@@ -890,6 +1095,12 @@ struct tlhip_problem {
             if (a) (void)tlhip_free(a);
     }
 };
+
+extern "C" int tlhip_set_resident(int on)
+{
+    g_resident = on ? 1 : 0;
+    return 0;
+}
 
 extern "C" int tlhip_set_cg_fusion(int on)
 {

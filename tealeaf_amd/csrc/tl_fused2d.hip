@@ -29,6 +29,7 @@
 // precedes the inner loop), also on the fly.
 #include "tl_lanes.h"
 #include "tl_mixed.h"
+#include "tl_loop.h"
 #include "tl_ops.h"
 #include "tealeaf_hip.h"
 
@@ -47,17 +48,21 @@ struct PpcgCoefs {
 // 2 first applies cg_calc_ur to r and u, then forms sd = r/theta.
 template <class T, int S, int MODE, bool PF>
 __global__ void __launch_bounds__(kBlock)
-k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ur,
+k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg,
+               const double* __restrict__ alpha_dev,
                const T* __restrict__ sd_in, T* __restrict__ sd_out,
                const T* __restrict__ r_in, T* __restrict__ r_out,
                const T* __restrict__ u_in, T* __restrict__ u_out,
                const T* __restrict__ kx, const T* __restrict__ ky,
-               const T* __restrict__ p_in, const T* __restrict__ w_in)
+               const T* __restrict__ p_in, const T* __restrict__ w_in, TlGate gate)
 {
     constexpr bool FROM_R = MODE >= 1;
     constexpr bool UR = MODE == 2;
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
+    if (gate.closed()) return;
+    // cg_calc_ur's alpha: the argument, or the resident loop's device value
+    const T alpha_ur = UR && alpha_dev ? (T)*alpha_dev : alpha_arg;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
     const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
     const int bx = tile % t.nbx, by = tile / t.nbx;
@@ -329,8 +334,10 @@ struct Ring3 {
 };
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_copy_rings2d(PitchGrid g, int rows, Ring3<T> rg)
+__global__ void __launch_bounds__(kBlock)
+k_copy_rings2d(PitchGrid g, int rows, Ring3<T> rg, TlGate gate)
 {
+    if (gate.closed()) return;
     // rows [0,hd) and [hd+ny,rows) whole; rows [hd,hd+ny) outside [hd,hd+nx)
     const int side = g.X - g.nx;
     const long nrow = (long)(rows - g.ny) * g.X;
@@ -365,6 +372,7 @@ struct FusedArgs {
     const double* al;
     const double* be;
     double theta, alpha_ur;
+    const double* alpha_dev;   // null: alpha_ur
     const T* sd_in;
     T* sd_out;
     const T* r_in;
@@ -396,12 +404,14 @@ int launch_fused(const FusedArgs<T>& a)
     const int slot = tl_timer_begin(a.what);
     if (g_fused_pf != 0)
         hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, true>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.sd_in, a.sd_out,
-                           a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w);
+                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev,
+                           a.sd_in, a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p,
+                           a.w, tl_gate());
     else
         hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, false>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.sd_in, a.sd_out,
-                           a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w);
+                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev,
+                           a.sd_in, a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p,
+                           a.w, tl_gate());
     tl_timer_end(slot);
     return tl_check_launch(a.what);
 }
@@ -424,7 +434,7 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
                      const double* alphas, const double* betas, double theta, int init_from_r,
                      const int* neighbours, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
                      const T* u_in, T* u_out, const T* kx, const T* ky, double alpha_ur,
-                     const T* p, const T* w)
+                     const T* p, const T* w, const double* alpha_dev = nullptr)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
@@ -459,10 +469,10 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
     }
     const long ring = ((long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny) * rg.n;
     hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0, tl_stream(), g,
-                       y, rg);
+                       y, rg, tl_gate());
     TL_TRY(tl_check_launch(what));
-    const FusedArgs<T> a{g, alphas, betas, theta, alpha_ur, sd_in, sd_out, r_in, r_out, u_in,
-                         u_out, kx, ky, p, w, what};
+    const FusedArgs<T> a{g, alphas, betas, theta, alpha_ur, alpha_dev, sd_in, sd_out, r_in, r_out,
+                         u_in, u_out, kx, ky, p, w, what};
     if (!init_from_r) return dispatch_fused<T, 0>(count, a);
     if (!p) return dispatch_fused<T, 1>(count, a);
     return dispatch_fused<T, 2>(count, a);
@@ -498,8 +508,10 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
         rg.dst[0] = p_out;
         rg.n = 1;
         const long ring = (long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny;
+        // ungated: calc_pw returns pw to the host, so it never runs inside a
+        // resident loop
         hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
-                           tl_stream(), g, y, rg);
+                           tl_stream(), g, y, rg, TlGate{});
         TL_TRY(tl_check_launch(what));
     }
     const int slot = tl_timer_begin(what);
@@ -518,6 +530,24 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
 }  // namespace
 
 namespace tl {
+namespace loop2d {
+
+// The fused inner loop of a resident PPCG iteration: gated, and the first
+// launch (init_from_r with p, w) takes cg_calc_ur's alpha from the loop.
+int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const double* betas,
+               double theta, int init_from_r, const int* neighbours, const double* sd_in,
+               double* sd_out, const double* r_in, double* r_out, const double* u_in,
+               double* u_out, const double* kx, const double* ky, const double* p,
+               const double* w)
+{
+    return inner_iterations<double>("tlhip2d_ppcg_inner_iterations", x, y, hd, x, count, alphas,
+                                    betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
+                                    r_out, u_in, u_out, kx, ky, 0.0, p, w,
+                                    p ? loop::alpha() : nullptr);
+}
+
+}  // namespace loop2d
+
 namespace mix {
 
 int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, const float* kx,

@@ -17,6 +17,7 @@
 // Vector (V=2) kernels are used when rows and pointers are 16-byte aligned
 // and halo_depth >= 2; otherwise the same code runs with V=1.
 #include "tl_ops.h"
+#include "tl_loop.h"
 #include "tealeaf_hip.h"
 
 #include <stdint.h>
@@ -73,8 +74,10 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
 k_stencil2d(TlGrid2 g, Tile2 t, const double* __restrict__ a,
-            const double* __restrict__ kx, const double* __restrict__ ky, Op op, TlReduce rd)
+            const double* __restrict__ kx, const double* __restrict__ ky, Op op, TlReduce rd,
+            TlGate gate)
 {
+    if (gate.closed()) return;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
     const int tile = xcd_tile(blockIdx.x, t.nbx * t.nby);
     const int sx = tile % t.nbx, by = tile / t.nbx;
@@ -311,8 +314,9 @@ struct HaloArgs2 {
     int depth;
 };
 
-__global__ void __launch_bounds__(kBlock) k_local_halos2d(TlGrid2 g, HaloArgs2 h)
+__global__ void __launch_bounds__(kBlock) k_local_halos2d(TlGrid2 g, HaloArgs2 h, TlGate gate)
 {
+    if (gate.closed()) return;
     double* f = h.f[blockIdx.y];
     const int ny = g.y - 2 * g.hd, nx = g.x - 2 * g.hd;
     const long nlr = (long)ny * h.depth, ntb = (long)nx * h.depth;
@@ -391,9 +395,11 @@ Tile2 make_tile(const TlGrid2& g, int V)
     return t;
 }
 
+// Under the current launch gate; with nb_out, a reducing op's partials are
+// left for the caller's finisher (see launch_interior).
 template <class Op>
 int launch_stencil(const TlGrid2& g, int V, const double* a, const double* kx,
-                   const double* ky, const Op& op, const char* what)
+                   const double* ky, const Op& op, const char* what, unsigned* nb_out = nullptr)
 {
     if (g.x - 2 * g.hd <= 0 || g.y - 2 * g.hd <= 0) return 0;
     if (tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V) > TL_MAX_RED_BLOCKS)
@@ -403,20 +409,25 @@ int launch_stencil(const TlGrid2& g, int V, const double* a, const double* kx,
     const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_stencil2d<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
-                           kx, ky, op, tl_reduce_slots());
+                           kx, ky, op, tl_reduce_slots(), tl_gate());
     else
         hipLaunchKernelGGL((k_stencil2d<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
-                           kx, ky, op, tl_reduce_slots());
+                           kx, ky, op, tl_reduce_slots(), tl_gate());
     tl_timer_end(slot);
     TL_TRY(tl_check_launch(what));
+    if (nb_out) {
+        *nb_out = nb;
+        return 0;
+    }
     if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
     return 0;
 }
 
 template <class Op>
-int launch_interior2(const TlGrid2& g, int V, const Op& op, const char* what)
+int launch_interior2(const TlGrid2& g, int V, const Op& op, const char* what,
+                     unsigned* nb_out = nullptr)
 {
-    return launch_interior(rows_of(g), g.x - 2 * g.hd, V, op, what);
+    return launch_interior(rows_of(g), g.x - 2 * g.hd, V, op, what, nb_out);
 }
 
 bool interior_empty(const TlGrid2& g) { return g.x - 2 * g.hd <= 0 || g.y - 2 * g.hd <= 0; }
@@ -507,7 +518,7 @@ extern "C" int tlhip2d_local_halos(int x, int y, int depth, int hd, const int* n
     const long ny = y - 2 * hd, nx = x - 2 * hd;
     const long tot = 4 * (ny > nx ? ny : nx) * depth;
     hipLaunchKernelGGL(k_local_halos2d, dim3(flat_blocks(tot), h.nf), dim3(kBlock), 0,
-                       tl_stream(), g, h);
+                       tl_stream(), g, h, tl_gate());
     return tl_check_launch(what);
 }
 
@@ -797,3 +808,63 @@ extern "C" int tlhip2d_finalise(int x, int y, int hd, double* energy, const doub
     OpSimple op{energy, u, 3, density, 0.0};
     return launch_interior2(g, pick_v(g, energy, density, u), op, what);
 }
+
+// ---------------------------------------------------------------- resident loop
+// The CG pieces of one iteration of a device-resident loop (tl_loop.h):
+// the same kernels as the entry points above, gated, with alpha and beta
+// taken from the loop's device state and the reductions finished by the
+// loop's finishers.
+
+namespace tl {
+namespace loop2d {
+
+int calc_w(int x, int y, int hd, const double* p, double* w, const double* kx, const double* ky)
+{
+    const char* what = "tlhip2d_cg_calc_w";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(p && w && kx && ky && p != w, what, "bad arguments");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_stencil(g, pick_v(g, p, w, kx, ky), p, kx, ky, OpCalcW{w}, what, &nb));
+    return loop::finish_alpha(nb, what);
+}
+
+int calc_ur(int x, int y, int hd, double* u, const double* p, double* r, const double* w,
+            const loop::StopRule& rule)
+{
+    const char* what = "tlhip2d_cg_calc_ur";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && p && r && w, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    OpCalcUr op{u, p, r, w, 0.0, loop::alpha()};
+    unsigned nb = 0;
+    TL_TRY(launch_interior2(g, pick_v(g, u, p, r, w), op, what, &nb));
+    return loop::finish_beta(nb, rule, what);
+}
+
+int calc_p(int x, int y, int hd, double* p, const double* r)
+{
+    const char* what = "tlhip2d_cg_calc_p";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(p && r, what, "null argument");
+    OpCalcP op{p, r, 0.0, loop::beta()};
+    return launch_interior2(g, pick_v(g, p, r), op, what);
+}
+
+int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule)
+{
+    const char* what = "tlhip2d_calculate_2norm";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(buf, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_interior2(g, pick_v(g, buf), OpNorm{buf}, what, &nb));
+    return loop::finish_beta(nb, rule, what);
+}
+
+}  // namespace loop2d
+}  // namespace tl

@@ -12,6 +12,7 @@
 // waves of the adjacent rows fetch (a block is 4 consecutive rows of one
 // strip, and tiles are dealt out XCD-contiguously).
 #include "tl_ops.h"
+#include "tl_loop.h"
 #include "tealeaf_hip.h"
 
 #include <stdint.h>
@@ -63,8 +64,10 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
 k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __restrict__ kx,
-            const double* __restrict__ ky, const double* __restrict__ kz, Op op, TlReduce rd)
+            const double* __restrict__ ky, const double* __restrict__ kz, Op op, TlReduce rd,
+            TlGate gate)
 {
+    if (gate.closed()) return;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
     const int tile = xcd_tile(blockIdx.x, t.ns * t.nbj * t.nbk);
     const int sx = tile % t.ns;
@@ -349,8 +352,9 @@ struct HaloArgs3 {
     int depth;
 };
 
-__global__ void __launch_bounds__(kBlock) k_local_halos3d(TlGrid3 g, HaloArgs3 h)
+__global__ void __launch_bounds__(kBlock) k_local_halos3d(TlGrid3 g, HaloArgs3 h, TlGate gate)
 {
+    if (gate.closed()) return;
     double* f = h.f[blockIdx.y];
     const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
     const long xy = (long)g.x * g.y;
@@ -445,9 +449,11 @@ Tile3 make_tile(const TlGrid3& g, int V)
     return t;
 }
 
+// Under the current launch gate; with nb_out, a reducing op's partials are
+// left for the caller's finisher (see launch_interior).
 template <class Op>
 int launch_stencil(const TlGrid3& g, int V, const double* a, const double* kx, const double* ky,
-                   const double* kz, const Op& op, const char* what)
+                   const double* kz, const Op& op, const char* what, unsigned* nb_out = nullptr)
 {
     if (interior_empty(g)) return 0;
     if ((long)tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V) * tl_cdiv(g.y - 2 * g.hd, kWaves) >
@@ -458,21 +464,26 @@ int launch_stencil(const TlGrid3& g, int V, const double* a, const double* kx, c
     const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_stencil3d<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
-                           kx, ky, kz, op, tl_reduce_slots());
+                           kx, ky, kz, op, tl_reduce_slots(), tl_gate());
     else
         hipLaunchKernelGGL((k_stencil3d<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, t, a,
-                           kx, ky, kz, op, tl_reduce_slots());
+                           kx, ky, kz, op, tl_reduce_slots(), tl_gate());
     tl_timer_end(slot);
     TL_TRY(tl_check_launch(what));
+    if (nb_out) {
+        *nb_out = nb;
+        return 0;
+    }
     if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
     return 0;
 }
 
 template <class Op>
-int launch_interior3(const TlGrid3& g, int V, const Op& op, const char* what)
+int launch_interior3(const TlGrid3& g, int V, const Op& op, const char* what,
+                     unsigned* nb_out = nullptr)
 {
     if (interior_empty(g)) return 0;
-    return launch_interior(rows_of(g), g.x - 2 * g.hd, V, op, what);
+    return launch_interior(rows_of(g), g.x - 2 * g.hd, V, op, what, nb_out);
 }
 
 int copy_ring(const TlGrid3& g, const double* src, double* dst, const char* what)
@@ -573,7 +584,7 @@ extern "C" int tlhip3d_local_halos(int x, int y, int z, int depth, int hd, const
     const long nx = x - 2 * hd, ny = y - 2 * hd, nz = z - 2 * hd;
     const long tot = 2 * (ny * nz + nx * nz + nx * ny) * depth;
     hipLaunchKernelGGL(k_local_halos3d, dim3(flat_blocks(tot), h.nf), dim3(kBlock), 0,
-                       tl_stream(), g, h);
+                       tl_stream(), g, h, tl_gate());
     return tl_check_launch(what);
 }
 
@@ -868,3 +879,50 @@ extern "C" int tlhip3d_solver_finalise(int x, int y, int z, int hd, double* ener
     OpSimple op{energy, u, 3, density, 0.0};
     return launch_interior3(g, pick_v(g, energy, density, u), op, what);
 }
+
+// ---------------------------------------------------------------- resident loop
+// The CG pieces of one iteration of a device-resident loop (tl_loop.h), as
+// in the 2D file.
+
+namespace tl {
+namespace loop3d {
+
+int calc_w(int x, int y, int z, int hd, const double* p, double* w, const double* kx,
+           const double* ky, const double* kz)
+{
+    const char* what = "tlhip3d_cg_solver_calc_w";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(p && w && kx && ky && kz && p != w, what, "bad arguments");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_stencil(g, pick_v(g, p, w, kx, ky, kz), p, kx, ky, kz, OpCalcW{w}, what, &nb));
+    return loop::finish_alpha(nb, what);
+}
+
+int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, const double* w,
+            const loop::StopRule& rule)
+{
+    const char* what = "tlhip3d_cg_solver_calc_ur";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && p && r && w, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    OpCalcUr op{u, p, r, w, 0.0, loop::alpha()};
+    unsigned nb = 0;
+    TL_TRY(launch_interior3(g, pick_v(g, u, p, r, w), op, what, &nb));
+    return loop::finish_beta(nb, rule, what);
+}
+
+int calc_p(int x, int y, int z, int hd, double* p, const double* r)
+{
+    const char* what = "tlhip3d_cg_solver_calc_p";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(p && r, what, "null argument");
+    OpCalcP op{p, r, 0.0, loop::beta()};
+    return launch_interior3(g, pick_v(g, p, r), op, what);
+}
+
+}  // namespace loop3d
+}  // namespace tl

@@ -1,0 +1,92 @@
+// Device-resident solver loops: the host enqueues CG / PPCG iterations
+// ahead of the device and reads each iteration's outcome afterwards.
+//
+// The reference's drivers (2d/drivers/cg_driver.c, ppcg_driver.c) return
+// two scalars to the host every iteration (pw, then rrn) and form
+// alpha = rro / pw and beta = rrn / rro there; each is a round trip that
+// idles the device.  Here the finishers of those two reductions form alpha
+// and beta on the device, in the same FP64 operations, and later kernels
+// of the iteration read them from device memory.  The second finisher also
+// applies the driver's stopping rule (convergence, and the CG -> Chebyshev
+// / PPCG switch of switch_now) and, when it fires, closes the launch gate:
+// every kernel already enqueued for a later iteration then returns at once.
+// The host meanwhile keeps kAhead iterations in flight and reads each
+// iteration's report from a pinned ring.  Values, iteration counts and the
+// order of every floating-point operation are those of the host-driven
+// loop.
+#pragma once
+
+#include "tl_common.h"
+
+namespace tl {
+namespace loop {
+
+// The stopping rule the second finisher of iteration j applies to rrn.
+struct StopRule {
+    double eps;          // reference test: |err| < eps
+    double rel_target;   // relative test (rel != 0): rrn <= rel_target
+    double eps_lim;      // switch rule 1: err < eps_lim && j + 1 > 20
+    int sqrt_err;        // err = sqrt(rrn) (the 3D reference test), else err = rrn
+    int rel;
+    int sw;              // 0: no switch; 1: rule 1 (errswitch); 2: j + 1 > presteps && err < 1
+    int presteps;
+};
+
+enum : int { kConverged = 1, kSwitch = 2 };
+
+struct Report {
+    double rrn, err;
+    int flags;           // kConverged | kSwitch
+};
+
+constexpr int kAhead = 4;   // iterations in flight beyond the one the host waits for
+
+// Arm a loop: the device's rro takes `rro`, the gate opens, and the
+// alpha/beta logs hold at least `max_iters` entries.
+int begin(double rro, int max_iters);
+// Launches issued from now on belong to iteration j.
+void set_iter(int j);
+// Launches are ungated again (the device state stays as it is).
+void end();
+// Device scalars that gated kernels read.
+const double* alpha();
+const double* beta();
+// Finisher of a pw reduction: alpha = rro / pw (cg_driver.c:88-90).
+int finish_alpha(unsigned nblocks, const char* what);
+// Finisher of an rrn reduction: beta = rrn / rro, rro = rrn, the stop rule,
+// and the iteration's report.
+int finish_beta(unsigned nblocks, const StopRule& rule, const char* what);
+// The report of iteration j, once the device has published it.
+int wait(int j, Report* out);
+// alpha and beta of iterations j0 .. j0+n-1 into host arrays.
+int logs(int j0, int n, double* alphas, double* betas);
+// Free the loop's device and pinned memory (tlhip_finalise).
+void release();
+
+}  // namespace loop
+
+// Gated single-iteration pieces (tl_kernels2d.hip, tl_kernels3d.hip,
+// tl_fused2d.hip).  Each launches under tl_gate() and takes its scalar
+// from the loop's device state.
+namespace loop2d {
+int calc_w(int x, int y, int hd, const double* p, double* w, const double* kx, const double* ky);
+int calc_ur(int x, int y, int hd, double* u, const double* p, double* r, const double* w,
+            const loop::StopRule& rule);
+int calc_p(int x, int y, int hd, double* p, const double* r);
+int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
+int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const double* betas,
+               double theta, int init_from_r, const int* neighbours, const double* sd_in,
+               double* sd_out, const double* r_in, double* r_out, const double* u_in,
+               double* u_out, const double* kx, const double* ky, const double* p,
+               const double* w);
+}  // namespace loop2d
+
+namespace loop3d {
+int calc_w(int x, int y, int z, int hd, const double* p, double* w, const double* kx,
+           const double* ky, const double* kz);
+int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, const double* w,
+            const loop::StopRule& rule);
+int calc_p(int x, int y, int z, int hd, double* p, const double* r);
+}  // namespace loop3d
+
+}  // namespace tl

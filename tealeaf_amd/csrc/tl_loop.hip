@@ -1,0 +1,256 @@
+// Device state, finishers and host side of the device-resident solver loops
+// (tl_loop.h).
+#include "tl_loop.h"
+#include "tealeaf_hip.h"
+
+#include <limits.h>
+#include <math.h>
+
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+
+namespace tl {
+namespace loop {
+namespace {
+
+struct State {
+    double rro, alpha, beta;
+    int last;            // last iteration to run; INT_MAX while the loop goes on
+    int pad;
+};
+
+// One iteration's report in pinned host memory.  The host arms the three
+// values with TL_MAIL_EMPTY before it enqueues the iteration's finisher;
+// the report is complete once none of them holds the mark (each value is
+// its own ready flag; the flags travel as a double).
+struct Slot {
+    double rrn, err, flags, pad;
+};
+
+constexpr int kRing = 8;
+static_assert(kRing > kAhead + 1, "a slot must outlive the iterations in flight");
+
+struct Host {
+    State* st = nullptr;           // device
+    double* alog = nullptr;        // device, [cap]
+    double* blog = nullptr;
+    int cap = 0;
+    Slot* ring = nullptr;          // pinned, host-mapped
+    bool active = false;
+    int iter = 0;
+};
+
+Host g_loop;
+
+__global__ void __launch_bounds__(64) k_loop_begin(State* st, double rro)
+{
+    if (threadIdx.x == 0) {
+        st->rro = rro;
+        st->last = INT_MAX;
+    }
+}
+
+// alpha = rro / pw of iteration j (cg_driver.c:88-90, ppcg_driver.c:102-104)
+__global__ void __launch_bounds__(256)
+k_loop_alpha(unsigned nblocks, const double* __restrict__ partials, State* st, double* alog,
+             int j)
+{
+    if (j > st->last) return;                 // uniform: the whole block returns
+    double pw[1];
+    tl_sum_partials<1>(nblocks, partials, pw);
+    if (threadIdx.x == 0) {
+        const double a = st->rro / pw[0];
+        st->alpha = a;
+        alog[j] = a;
+    }
+}
+
+// beta = rrn / rro, rro = rrn, and the driver's tests of iteration j
+// (cg_driver.c:92-99 and the loops of cg_driver.c:30-46, cheby_driver.c,
+// ppcg_driver.c); the report goes to ring slot j % kRing.
+__global__ void __launch_bounds__(256)
+k_loop_beta(unsigned nblocks, const double* __restrict__ partials, State* st, double* blog,
+            Slot* ring, int j, StopRule rule)
+{
+    if (j > st->last) return;
+    double tot[1];
+    tl_sum_partials<1>(nblocks, partials, tot);
+    if (threadIdx.x != 0) return;
+    const double rrn = tot[0];
+    const double b = rrn / st->rro;
+    st->beta = b;
+    blog[j] = b;
+    st->rro = rrn;
+    const double err = rule.sqrt_err ? sqrt(rrn) : rrn;
+    int flags = 0;
+    if (rule.rel ? rrn <= rule.rel_target : fabs(err) < rule.eps) flags |= kConverged;
+    if (rule.sw == 1 && err < rule.eps_lim && j + 1 > 20) flags |= kSwitch;
+    if (rule.sw == 2 && j + 1 > rule.presteps && err < 1.0) flags |= kSwitch;
+    if (flags) st->last = j;
+    Slot* s = ring + (j % kRing);
+    __hip_atomic_store(&s->rrn, rrn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&s->err, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&s->flags, (double)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+volatile unsigned long long* slot_bits(int j)
+{
+    return (volatile unsigned long long*)(g_loop.ring + (j % kRing));
+}
+
+bool slot_ready(volatile unsigned long long* b)
+{
+    return b[0] != TL_MAIL_EMPTY && b[1] != TL_MAIL_EMPTY && b[2] != TL_MAIL_EMPTY;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return tl_fail(TLHIP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define LOOP_HIP(call, what)                                  \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return hip_fail(_e, what);      \
+    } while (0)
+
+}  // namespace
+
+int begin(double rro, int max_iters)
+{
+    const char* what = "tl::loop::begin";
+    TL_TRY(tl_ensure_init());
+    Host& h = g_loop;
+    if (!h.st) {
+        LOOP_HIP(hipMalloc(&h.st, sizeof(State)), what);
+        LOOP_HIP(hipHostMalloc(&h.ring, sizeof(Slot) * kRing,
+                               hipHostMallocMapped | hipHostMallocCoherent), what);
+        for (int i = 0; i < kRing; ++i) h.ring[i] = Slot{};
+    }
+    const int need = max_iters > 1 ? max_iters : 1;
+    if (need > h.cap) {
+        LOOP_HIP(hipStreamSynchronize(tl_stream()), what);
+        if (h.alog) LOOP_HIP(hipFree(h.alog), what);
+        if (h.blog) LOOP_HIP(hipFree(h.blog), what);
+        h.alog = h.blog = nullptr;
+        h.cap = 0;
+        LOOP_HIP(hipMalloc(&h.alog, sizeof(double) * need), what);
+        LOOP_HIP(hipMalloc(&h.blog, sizeof(double) * need), what);
+        h.cap = need;
+    }
+    hipLaunchKernelGGL(k_loop_begin, dim3(1), dim3(64), 0, tl_stream(), h.st, rro);
+    TL_TRY(tl_check_launch(what));
+    h.active = true;
+    h.iter = 0;
+    return 0;
+}
+
+void set_iter(int j) { g_loop.iter = j; }
+
+void end() { g_loop.active = false; }
+
+const double* alpha() { return &g_loop.st->alpha; }
+const double* beta() { return &g_loop.st->beta; }
+
+int finish_alpha(unsigned nblocks, const char* what)
+{
+    Host& h = g_loop;
+    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
+    if (h.iter < 0 || h.iter >= h.cap)
+        return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
+    if (nblocks > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
+                       TL_MAX_RED_BLOCKS);
+    hipLaunchKernelGGL(k_loop_alpha, dim3(1), dim3(256), 0, tl_stream(), nblocks,
+                       tl_reduce_slots().partials, h.st, h.alog, h.iter);
+    return tl_check_launch(what);
+}
+
+int finish_beta(unsigned nblocks, const StopRule& rule, const char* what)
+{
+    Host& h = g_loop;
+    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
+    if (h.iter < 0 || h.iter >= h.cap)
+        return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
+    if (nblocks > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
+                       TL_MAX_RED_BLOCKS);
+    // The slot's previous report (iteration iter - kRing) was read before
+    // this iteration could be enqueued.
+    volatile unsigned long long* b = slot_bits(h.iter);
+    b[0] = b[1] = b[2] = TL_MAIL_EMPTY;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    hipLaunchKernelGGL(k_loop_beta, dim3(1), dim3(256), 0, tl_stream(), nblocks,
+                       tl_reduce_slots().partials, h.st, h.blog, h.ring, h.iter, rule);
+    return tl_check_launch(what);
+}
+
+// Spin on the slot.  Every millisecond the stream is queried: an error
+// surfaces there, and an idle stream with no report means the iteration
+// was never run (its gate was closed), which the callers rule out by
+// waiting only for iterations after the last one that reported.
+int wait(int j, Report* out)
+{
+    const char* what = "tl::loop::wait";
+    volatile unsigned long long* b = slot_bits(j);
+    if (!slot_ready(b)) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (unsigned spin = 1; !slot_ready(b); ++spin) {
+            if ((spin & 1023u) != 0) continue;
+            const auto now = std::chrono::steady_clock::now();
+            if (now - t0 < std::chrono::milliseconds(1)) continue;
+            t0 = now;
+            const hipError_t e = hipStreamQuery(tl_stream());
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return hip_fail(e, what);
+            if (!slot_ready(b))
+                return tl_fail(TLHIP_ERR_HIP, "%s: iteration %d did not report", what, j);
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    double v[3];
+    for (int i = 0; i < 3; ++i) {
+        const unsigned long long bits = b[i];
+        memcpy(&v[i], &bits, sizeof(double));
+    }
+    out->rrn = v[0];
+    out->err = v[1];
+    out->flags = (int)v[2];
+    return 0;
+}
+
+int logs(int j0, int n, double* alphas, double* betas)
+{
+    const char* what = "tl::loop::logs";
+    Host& h = g_loop;
+    if (n <= 0) return 0;
+    if (j0 < 0 || j0 + n > h.cap) return tl_fail(TLHIP_ERR_ARG, "%s: range outside the logs", what);
+    LOOP_HIP(hipMemcpyAsync(alphas, h.alog + j0, sizeof(double) * n, hipMemcpyDeviceToHost,
+                            tl_stream()), what);
+    LOOP_HIP(hipMemcpyAsync(betas, h.blog + j0, sizeof(double) * n, hipMemcpyDeviceToHost,
+                            tl_stream()), what);
+    LOOP_HIP(hipStreamSynchronize(tl_stream()), what);
+    return 0;
+}
+
+void release()
+{
+    Host& h = g_loop;
+    if (h.st) (void)hipFree(h.st);
+    if (h.alog) (void)hipFree(h.alog);
+    if (h.blog) (void)hipFree(h.blog);
+    if (h.ring) (void)hipHostFree(h.ring);
+    h = Host{};
+}
+
+}  // namespace loop
+}  // namespace tl
+
+TlGate tl_gate()
+{
+    const tl::loop::Host& h = tl::loop::g_loop;
+    if (!h.active) return TlGate{nullptr, 0};
+    return TlGate{&h.st->last, h.iter};
+}

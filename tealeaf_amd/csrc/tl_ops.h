@@ -93,8 +93,9 @@ __device__ __forceinline__ Seg seg_of(const TlRows& g, int segs_per_row, long s)
 // columns, two segments in flight per thread.
 template <int V, class Op>
 __global__ void __launch_bounds__(kBlock)
-k_interior(TlRows g, int segs_per_row, long nseg, Op op, TlReduce rd)
+k_interior(TlRows g, int segs_per_row, long nseg, Op op, TlReduce rd, TlGate gate)
 {
+    if (gate.closed()) return;
     constexpr int NR = Op::NRED > 0 ? Op::NRED : 1;
     double acc[NR];
 #pragma unroll
@@ -122,6 +123,7 @@ struct OpCalcUr {   // u += alpha p ; r -= alpha w ; rrn += r.r  (2d/c_kernels/o
     double* __restrict__ r;
     const double* __restrict__ w;
     double alpha;
+    const double* __restrict__ alpha_dev = nullptr;   // when set, alpha is read from device memory
     template <int V>
     __device__ In4<V> load(long i) const
     {
@@ -130,11 +132,12 @@ struct OpCalcUr {   // u += alpha p ; r -= alpha w ; rrn += r.r  (2d/c_kernels/o
     template <int V>
     __device__ void apply(long i, int nval, const In4<V>& in, double* acc) const
     {
+        const double a = alpha_dev ? *alpha_dev : alpha;
         DV<V> uu = in.a, rr = in.c;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            uu.v[v] += alpha * in.b.v[v];
-            rr.v[v] -= alpha * in.d.v[v];
+            uu.v[v] += a * in.b.v[v];
+            rr.v[v] -= a * in.d.v[v];
             if (v < nval) acc[0] += rr.v[v] * rr.v[v];
         }
         stv<V>(u, i, uu, nval);
@@ -147,6 +150,7 @@ struct OpCalcP {    // p = beta p + r  (2d/c_kernels/omp3/cg.c:146-164)
     double* __restrict__ p;
     const double* __restrict__ r;
     double beta;
+    const double* __restrict__ beta_dev = nullptr;    // when set, beta is read from device memory
     template <int V>
     __device__ In4<V> load(long i) const
     {
@@ -158,9 +162,10 @@ struct OpCalcP {    // p = beta p + r  (2d/c_kernels/omp3/cg.c:146-164)
     template <int V>
     __device__ void apply(long i, int nval, const In4<V>& in, double*) const
     {
+        const double b = beta_dev ? *beta_dev : beta;
         DV<V> pp;
 #pragma unroll
-        for (int v = 0; v < V; ++v) pp.v[v] = beta * in.a.v[v] + in.b.v[v];
+        for (int v = 0; v < V; ++v) pp.v[v] = b * in.a.v[v] + in.b.v[v];
         stv<V>(p, i, pp, nval);
     }
 };
@@ -437,9 +442,12 @@ static inline bool all_aligned(P... ps)
     return (aligned16(ps) && ...);
 }
 
-// Launch a pointwise op over the interior rows.
+// Launch a pointwise op over the interior rows, under the current launch
+// gate.  With nb_out, a reducing op's partials are left for the caller's
+// finisher and the block count is returned there.
 template <class Op>
-static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const char* what)
+static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const char* what,
+                           unsigned* nb_out = nullptr)
 {
     if (nx <= 0 || g.nrows <= 0) return 0;
     const int spr = (int)tl_cdiv(nx, kBlock * V);
@@ -450,12 +458,16 @@ static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const c
     const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_interior<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
-                           nseg, op, tl_reduce_slots());
+                           nseg, op, tl_reduce_slots(), tl_gate());
     else
         hipLaunchKernelGGL((k_interior<1, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
-                           nseg, op, tl_reduce_slots());
+                           nseg, op, tl_reduce_slots(), tl_gate());
     tl_timer_end(slot);
     TL_TRY(tl_check_launch(what));
+    if (nb_out) {
+        *nb_out = nb;
+        return 0;
+    }
     if constexpr (Op::NRED > 0) return tl_finish_reduce(Op::NRED, nb, what);
     return 0;
 }

@@ -6,6 +6,7 @@
 // rank, 2d/drivers/*_driver.c call run_* one after another), and so is this
 // runtime: calls from several host threads at once are not supported.
 #include "tl_ops.h"
+#include "tl_loop.h"
 #include "tealeaf_hip.h"
 
 #include <stdarg.h>
@@ -83,9 +84,10 @@ extern "C" int tlhip_init(int device)
     HIP_TRY(hipMalloc(&g_rt.red.partials, sizeof(double) * TL_MAX_RED * TL_MAX_RED_BLOCKS),
             "hipMalloc(partials)");
     HIP_TRY(hipMalloc(&g_rt.red.result, sizeof(double) * TL_MAX_RED), "hipMalloc(result)");
-    HIP_TRY(hipHostMalloc(&g_rt.red.mailbox, 256, hipHostMallocMapped | hipHostMallocCoherent),
+    HIP_TRY(hipHostMalloc(&g_rt.red.mailbox, sizeof(double) * TL_MAIL_SLOTS * TL_MAX_RED,
+                          hipHostMallocMapped | hipHostMallocCoherent),
             "hipHostMalloc(mailbox)");
-    memset(g_rt.red.mailbox, 0, 256);
+    memset(g_rt.red.mailbox, 0, sizeof(double) * TL_MAIL_SLOTS * TL_MAX_RED);
     g_rt.device = dev;
     g_rt.initialised = 1;
     return 0;
@@ -103,33 +105,48 @@ int tl_check_launch(const char* what)
     return 0;
 }
 
-// The mailbox holds the last finisher's results in doubles [0, TL_MAX_RED)
-// and, at kMailSeq, the sequence number of that finisher launch, stored
-// after the results with system-scope release.  Waiting for the number to
-// arrive is a few microseconds quicker than a stream synchronisation, and
-// the drivers wait for two scalars per CG iteration.  A finisher that does
-// not report within 2 ms falls back to synchronising the stream, which
-// also surfaces any launch error.
-constexpr int kMailSeq = 8;
+// Finisher launch `seq` writes its values to mailbox slot seq % TL_MAIL_SLOTS,
+// which the host armed (TL_MAIL_EMPTY) just before the launch.  Spinning
+// on the slot is a few microseconds quicker than a stream synchronisation,
+// and the drivers wait for two scalars per CG iteration.  Every finisher
+// is fetched before the next is launched, so a slot is rewritten only
+// after its last value was read.  A finisher that does not report within
+// 2 ms falls back to synchronising the stream, which also surfaces any
+// launch error.
+static volatile unsigned long long* mail_slot(unsigned long long seq)
+{
+    return (volatile unsigned long long*)(g_rt.red.mailbox +
+                                          (size_t)(seq % TL_MAIL_SLOTS) * TL_MAX_RED);
+}
+
+static bool mail_ready(volatile unsigned long long* m, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (m[i] == TL_MAIL_EMPTY) return false;
+    return true;
+}
 
 int tl_fetch(int n, double* out, const char* what)
 {
-    volatile unsigned long long* seq = (volatile unsigned long long*)(g_rt.red.mailbox + kMailSeq);
-    if (*seq != g_rt.seq) {
+    volatile unsigned long long* m = mail_slot(g_rt.seq);
+    if (!mail_ready(m, n)) {
         const auto t0 = std::chrono::steady_clock::now();
-        for (unsigned spin = 1; *seq != g_rt.seq; ++spin) {
+        for (unsigned spin = 1; !mail_ready(m, n); ++spin) {
             if ((spin & 1023u) == 0 &&
                 std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
                 hipError_t e = hipStreamSynchronize(g_rt.stream);
                 if (e != hipSuccess) return hip_fail(e, what);
-                if (*seq != g_rt.seq)
+                if (!mail_ready(m, n))
                     return tl_fail(TLHIP_ERR_HIP, "%s: reduction result not published", what);
                 break;
             }
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
-    for (int i = 0; i < n; ++i) out[i] = ((volatile double*)g_rt.red.mailbox)[i];
+    for (int i = 0; i < n; ++i) {
+        const unsigned long long bits = m[i];
+        memcpy(&out[i], &bits, sizeof(double));
+    }
     return 0;
 }
 
@@ -163,6 +180,7 @@ extern "C" int tlhip_finalise(void)
     if (!g_rt.initialised) return 0;
     // Teardown: report the first failure but release everything regardless.
     hipError_t first = hipStreamSynchronize(g_rt.stream);
+    tl::loop::release();
     hipError_t e[6] = {hipFree(g_rt.red.partials), hipSuccess,
                        hipFree(g_rt.red.result), hipHostFree(g_rt.red.mailbox),
                        g_rt.staging ? hipFree(g_rt.staging) : hipSuccess,
@@ -300,40 +318,21 @@ int tl_staging(size_t doubles, double** out)
     return 0;
 }
 
-// Grid total of the per-block partials, summed in a fixed order: thread t
-// takes slots t, t+256, ... in turn, then a fixed tree over the block.
+// Grid total of the per-block partials (tl_sum_partials), published to
+// rd.result and to the armed mailbox slot `mail` (plain system-scope
+// stores: each value is its own ready flag).
 template <int NV>
 __global__ void __launch_bounds__(256)
-k_finish_reduce(unsigned nblocks, TlReduce rd, unsigned long long seq)
+k_finish_reduce(unsigned nblocks, TlReduce rd, double* mail)
 {
-    constexpr int NW = 256 / TL_WAVE;
-    __shared__ double s_part[NW][NV];
-    const int tid = threadIdx.x, lane = tid & (TL_WAVE - 1), wave = tid / TL_WAVE;
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    for (unsigned b = tid; b < nblocks; b += 256) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += rd.partials[(size_t)b * NV + i];
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const double s = tl_wave_sum(acc[i]);
-        if (lane == 0) s_part[wave][i] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
+    double tot[NV];
+    tl_sum_partials<NV>(nblocks, rd.partials, tot);
+    if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) s += s_part[w][i];
-            rd.result[i] = s;
-            __hip_atomic_store(&rd.mailbox[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            rd.result[i] = tot[i];
+            __hip_atomic_store(&mail[i], tot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
-        // published after the results (tl_fetch waits for it)
-        __hip_atomic_store((unsigned long long*)(rd.mailbox + kMailSeq), seq, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -345,12 +344,16 @@ int tl_finish_reduce(int nv, unsigned nblocks, const char* what)
     if (nv != 1 && nv != 4)
         return tl_fail(TLHIP_ERR_ARG, "%s: unsupported reduction width %d", what, nv);
     const unsigned long long seq = g_rt.seq + 1;
+    volatile unsigned long long* m = mail_slot(seq);
+    for (int i = 0; i < TL_MAX_RED; ++i) m[i] = TL_MAIL_EMPTY;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    double* mail = (double*)m;
     if (nv == 1)
         hipLaunchKernelGGL(k_finish_reduce<1>, dim3(1), dim3(256), 0, g_rt.stream, nblocks,
-                           g_rt.red, seq);
+                           g_rt.red, mail);
     else
         hipLaunchKernelGGL(k_finish_reduce<4>, dim3(1), dim3(256), 0, g_rt.stream, nblocks,
-                           g_rt.red, seq);
+                           g_rt.red, mail);
     TL_TRY(tl_check_launch(what));
     g_rt.seq = seq;
     return 0;
