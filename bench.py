@@ -53,7 +53,7 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--solvers", default="cg,cheby,ppcg")
-    ap.add_argument("--ppcg-fusion", type=int, default=5,
+    ap.add_argument("--ppcg-fusion", type=int, default=10,
                     help="PPCG inner steps per fused launch (1 = one launch per step)")
     ap.add_argument("--cpu-iters", type=int, default=300,
                     help="CG iterations of the bounded CPU reference sample")
@@ -216,7 +216,7 @@ def profile_run(args):
     ncopy = (1 << 30) // 8
     src, dst = DeviceArray(ncopy, np.float64), DeviceArray(ncopy, np.float64)
     nb = np.full(4, -1, dtype=np.int32)
-    al, be = np.full(5, 0.7), np.full(5, 0.2)
+    al, be = np.full(10, 0.7), np.full(10, 0.2)
     pw = C.c_double()
     for _ in range(3):
         L.tlhip_stream_copy(dst.ptr, src.ptr, C.c_size_t(ncopy))
@@ -224,7 +224,7 @@ def profile_run(args):
                                         nb.ctypes.data, f64["sd"].ptr, f64["sd2"].ptr, f64["r"].ptr,
                                         f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
                                         f64["ky"].ptr, 0.0, None, None)
-        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 1,
+        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 10, al.ctypes.data, be.ctypes.data, 1.3, 1,
                                         nb.ctypes.data, None, f64["sd2"].ptr, f64["r"].ptr,
                                         f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
                                         f64["ky"].ptr, 0.3, f64["p"].ptr, f64["w"].ptr)

@@ -154,7 +154,7 @@ int tlhip2d_ppcg_init(int x, int y, int halo_depth, double theta,
 int tlhip2d_ppcg_inner_iteration(int x, int y, int halo_depth, double alpha,
         double beta, double* u, double* r, const double* kx, const double* ky,
         const double* sd_in, double* sd_out);
-/* Temporally blocked PPCG inner loop, 1 <= count <= 5 steps in one pass.
+/* Temporally blocked PPCG inner loop, 1 <= count <= 10 steps in one pass.
  * No single counterpart in the kernel interface: it replaces `count` turns
  * of the loop in 2d/drivers/ppcg_driver.c:165-183 (a depth-1 halo update of
  * sd, then run_ppcg_inner_iteration with alphas[i], betas[i]) for a chunk
@@ -194,8 +194,9 @@ int tlhip2d_cg_calc_pw(int x, int y, int halo_depth, double beta, double* pw,
         const double* p_in, double* p_out, const double* r, double* w,
         const double* kx, const double* ky);
 /* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
- * 96 for FP64 and 48 for FP32) and next-row prefetch (0 off, 1 on, < 0:
- * default on). */
+ * the height whose block count fills ~0.88 of what the device holds at
+ * once, from the kernel's occupancy and the CU count) and next-row prefetch
+ * (0 off, 1 on, < 0: default on). */
 int tlhip_set_fused_tuning(int tile_rows, int prefetch);
 /* 2d/kernel_interface.h:64 run_copy_u */
 int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);
@@ -361,8 +362,10 @@ int tlhip_set_cg_fusion(int on);
  * Default on.  No counterpart in the reference: its drivers take pw and rrn
  * to the host (2d/drivers/cg_driver.c:78-99). */
 int tlhip_set_resident(int on);
-/* Steps per fused PPCG launch used by the driver (1..5; 0 or 1 = one step
- * per launch through tlhip2d_ppcg_inner_iteration).  Default 5. */
+/* Steps per fused PPCG launch used by the driver (1..10; 0 or 1 = one step
+ * per launch through tlhip2d_ppcg_inner_iteration).  Default 10: the
+ * reference deck's 10 inner steps in one FP64 launch (the FP32 sweeps of
+ * the mixed solve use at most 5 per launch). */
 int tlhip_set_ppcg_fusion(int steps);
 
 /* How tlhip_problem_step solves (no counterpart in the reference, which

@@ -39,7 +39,14 @@ namespace {
 
 enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 
-int g_ppcg_fusion = 5;   // inner steps per fused launch (2D, single chunk)
+// Inner steps per fused PPCG launch (2D, single chunk).  FP64: the whole
+// inner loop of the default deck (10 steps) in one launch, which at 4096^2
+// on MI355X runs 10 steps in ~0.39-0.47 ms against ~0.25 ms per 5 (one pass
+// over memory either way; the deeper kernel holds 1 wave per SIMD instead
+// of 2).  The FP32 sweeps of the mixed solve stay at 5 per launch, where
+// the 10-step launch measured no faster.
+int g_ppcg_fusion = 10;
+constexpr int kFusionF32 = 5;
 int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
 int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 
@@ -827,7 +834,7 @@ struct tlhip_problem {
             double unused = 0.0;
             TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
         }
-        const int per = std::max(1, g_ppcg_fusion);
+        const int per = std::max(1, std::min(g_ppcg_fusion, kFusionF32));
         int done = 0;
         while (done < s.ppcg_inner_steps) {
             // the first launch also applies calc_ur (u is d here)
@@ -1110,7 +1117,7 @@ extern "C" int tlhip_set_cg_fusion(int on)
 
 extern "C" int tlhip_set_ppcg_fusion(int steps)
 {
-    if (steps < 0 || steps > 5) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_ppcg_fusion: 0..5");
+    if (steps < 0 || steps > 10) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_ppcg_fusion: 0..10");
     g_ppcg_fusion = steps;
     return 0;
 }

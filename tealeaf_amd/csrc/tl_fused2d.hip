@@ -33,11 +33,13 @@
 #include "tl_ops.h"
 #include "tealeaf_hip.h"
 
+#include <algorithm>
+
 using namespace tl;
 
 namespace {
 
-constexpr int kMaxFused = 5;
+constexpr int kMaxFused = 10;
 
 template <class T, int S>
 struct PpcgCoefs {
@@ -386,13 +388,53 @@ struct FusedArgs {
     const char* what;
 };
 
+// Compute units of the current device (cached; the runtime uses one device).
+int device_cus()
+{
+    static int cus = 0;
+    if (cus <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// Default tile height: as many row tiles as fill one round of the blocks
+// the device holds at once.  A wave marches its strip down the tile, and
+// the S-1 rows above and below it are recomputed by the neighbouring
+// tile, so taller tiles waste less; but a block count beyond what is
+// resident at once leaves a second, mostly idle round.  On MI355X
+// (256 CUs) a sweep at 4096^2 put the best block count at ~0.88 of the
+// resident capacity for every S (S <= 5: 2 blocks per CU; S >= 6, over
+// 256 VGPRs: 1), e.g. S = 10 at 225 blocks (ty 164) ran 391 us and at
+// 234 blocks (ty 158) 657 us.
+template <class T, int S, int MODE, bool PF>
+int auto_tile_rows(const PitchGrid& g, int nbx)
+{
+    static int occ = 0;
+    if (occ <= 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ppcg_fused2d<T, S, MODE, PF>, kBlock,
+                                                      0) != hipSuccess ||
+         occ <= 0))
+        occ = 1;
+    const long cap = (long)(0.88 * device_cus() * occ);
+    const long nby = std::max(1L, cap / std::max(1, nbx));
+    return (int)std::max(1L, ((long)g.ny + nby - 1) / nby);
+}
+
 template <class T, int S, int MODE>
 int launch_fused(const FusedArgs<T>& a)
 {
     // Windows reach S columns beyond their output columns on both sides.
-    // Default tile heights from a sweep at 4096^2 (S = 5, prefetch on).
-    const int ty = g_fused_ty > 0 ? g_fused_ty : (sizeof(T) == 8 ? 96 : 48);
-    const StripTile t = strip_tile<T>(a.g, S, ty, kWaves);
+    const bool pf = g_fused_pf != 0;
+    StripTile t = strip_tile<T>(a.g, S, 1, kWaves);
+    const int ty = g_fused_ty > 0 ? g_fused_ty
+                   : pf ? auto_tile_rows<T, S, MODE, true>(a.g, t.nbx)
+                        : auto_tile_rows<T, S, MODE, false>(a.g, t.nbx);
+    t = strip_tile<T>(a.g, S, ty, kWaves);
     if ((long)t.nbx * t.nby > (1L << 30))
         return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", a.what);
     PpcgCoefs<T, S> cf;
@@ -402,7 +444,7 @@ int launch_fused(const FusedArgs<T>& a)
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
     const int slot = tl_timer_begin(a.what);
-    if (g_fused_pf != 0)
+    if (pf)
         hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, true>), dim3(nb), dim3(kBlock), 0,
                            tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev,
                            a.sd_in, a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p,
@@ -425,6 +467,11 @@ int dispatch_fused(int count, const FusedArgs<T>& a)
     case 3: return launch_fused<T, 3, MODE>(a);
     case 4: return launch_fused<T, 4, MODE>(a);
     case 5: return launch_fused<T, 5, MODE>(a);
+    case 6: return launch_fused<T, 6, MODE>(a);
+    case 7: return launch_fused<T, 7, MODE>(a);
+    case 8: return launch_fused<T, 8, MODE>(a);
+    case 9: return launch_fused<T, 9, MODE>(a);
+    case 10: return launch_fused<T, 10, MODE>(a);
     default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", a.what, kMaxFused);
     }
 }

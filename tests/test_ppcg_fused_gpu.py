@@ -54,7 +54,7 @@ def _ring(a, x, y, hd):
 
 
 @pytest.mark.parametrize("grid", GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in GRIDS])
-@pytest.mark.parametrize("count", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("count", [1, 2, 3, 4, 5, 7, 10])
 @pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
 @pytest.mark.parametrize("pf", [1, 0], ids=["pf", "nopf"])
 def test_fused_matches_separate_steps(hip, oracle, grid, count, from_r, pf):
@@ -115,10 +115,10 @@ def test_fused_rejects_bad_layout(hip):
                                            nb.ctypes.data, p[0], p[0], p[1], p[2], p[3], p[4],
                                            p[5], p[6], 0.0, None, None)
     assert rc != 0
-    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 6, a.ctypes.data, a.ctypes.data, 1.0, 0,
+    rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 11, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
                                            p[6], p[7], 0.0, None, None)
-    assert rc != 0                       # count above 5
+    assert rc != 0                       # count above 10
     nb[1] = 3                            # an internal face
     rc = raw.tlhip2d_ppcg_inner_iterations(x, y, 2, 2, a.ctypes.data, a.ctypes.data, 1.0, 0,
                                            nb.ctypes.data, p[0], p[1], p[2], p[3], p[4], p[5],
@@ -140,16 +140,16 @@ def test_driver_fusion_is_bit_identical(hip, inner):
     from tealeaf_amd.problem import Problem
     text = tdeck.layered_2d(90, 2, solver=tdeck.PPCG, presteps=15, inner=inner, max_iters=2000)
     res = {}
-    for fusion in (1, 5, 3):
+    for fusion in (1, 10, 5, 3):
         hip.tlhip_set_ppcg_fusion(fusion)
         try:
             with Problem.from_deck(text, 2) as prob:
                 steps = prob.run()
                 res[fusion] = (steps, prob.get("u"), prob.get("energy"), prob.field_summary())
         finally:
-            hip.tlhip_set_ppcg_fusion(5)
+            hip.tlhip_set_ppcg_fusion(10)
     base = res[1]
-    for f in (3, 5):
+    for f in (3, 5, 10):
         steps, u, e, summ = res[f]
         assert [s["n_ppcg_inner"] for s in steps] == [s["n_ppcg_inner"] for s in base[0]]
         assert [s["n_ppcg_outer"] for s in steps] == [s["n_ppcg_outer"] for s in base[0]]
@@ -159,9 +159,9 @@ def test_driver_fusion_is_bit_identical(hip, inner):
 
 
 def test_set_ppcg_fusion_range(hip):
-    assert hip.dll.tlhip_set_ppcg_fusion(6) != 0
+    assert hip.dll.tlhip_set_ppcg_fusion(11) != 0
     assert hip.dll.tlhip_set_ppcg_fusion(-1) != 0
-    assert hip.dll.tlhip_set_ppcg_fusion(5) == 0
+    assert hip.dll.tlhip_set_ppcg_fusion(10) == 0
 
 
 def _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
@@ -196,7 +196,7 @@ F32_GRIDS = [(32, 30, 2, 0), (250, 70, 2, 0), (510, 37, 2, 4), (6, 5, 2, 0), (61
 
 
 @pytest.mark.parametrize("grid", F32_GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in F32_GRIDS])
-@pytest.mark.parametrize("count", [1, 3, 5])
+@pytest.mark.parametrize("count", [1, 3, 5, 10])
 @pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
 @pytest.mark.parametrize("pad", [0, 4])
 def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
@@ -241,7 +241,7 @@ def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
 
 @pytest.mark.parametrize("grid", [(250, 70, 2, 0), (118, 41, 4, 2), (64, 20, 1, 0)],
                          ids=["250x70", "118x41h4", "64x20h1"])
-@pytest.mark.parametrize("count", [1, 5])
+@pytest.mark.parametrize("count", [1, 5, 10])
 def test_fused_with_calc_ur_matches_oracle(hip, oracle, grid, count):
     """calc_ur folded into the first launch: the oracle runs cg_calc_ur
     (cg.c:116-143), ppcg_init and `count` halo-update + inner steps."""
