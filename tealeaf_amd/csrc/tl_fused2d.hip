@@ -34,12 +34,20 @@
 #include "tealeaf_hip.h"
 
 #include <algorithm>
+#include <type_traits>
 
 using namespace tl;
 
 namespace {
 
 constexpr int kMaxFused = 10;
+
+// Raw buffer descriptors (gfx9 family): word 3 of a descriptor with no
+// format conversion, the largest range a descriptor is given here, and the
+// offset that marks a lane's load as outside it (returns zeros).
+constexpr int kBufferWord3 = 0x00020000;
+constexpr unsigned kBufferRange = 0xFFFFFF00u;
+constexpr unsigned kBufferOutside = 0xFFFFFFF0u;
 
 template <class T, int S>
 struct PpcgCoefs {
@@ -48,7 +56,7 @@ struct PpcgCoefs {
 
 // Level 0 of the loop: MODE 0 reads sd; 1 forms sd = r/theta (ppcg_init);
 // 2 first applies cg_calc_ur to r and u, then forms sd = r/theta.
-template <class T, int S, int MODE, bool PF>
+template <class T, int S, int MODE, int PF>
 __global__ void __launch_bounds__(kBlock)
 k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg,
                const double* __restrict__ alpha_dev,
@@ -86,10 +94,27 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
     const int niter = j1 - j0 + 2 * S - 2;
     const long X = g.X;
 
-    auto ld_field = [&](const T* __restrict__ f, int row, int lo, int hi) -> C {
-        if (ld_ok && row >= lo && row < hi) return ldc(f, (long)row * X + c);
-        return zeros<T>();
+    // Field rows are read through buffer descriptors based at the first
+    // row this tile touches.  A lane whose row is outside [lo, hi) or whose
+    // cells are outside the array passes an offset beyond the descriptor's
+    // range, and the hardware returns zeros: no load sits behind a branch,
+    // so the compiler can keep prefetched rows in flight across iterations.
+    const int base_row = max(rs - S, 0);
+    const long span = (long)(g.ny + 2 * g.hd - base_row) * X * (long)sizeof(T);
+    const int nrec = (int)min(span, (long)kBufferRange);
+    auto rsrc = [&](const T* f) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(f + (long)base_row * X), (short)0, nrec,
+                                                 kBufferWord3);
     };
+    auto ld_field = [&](__amdgpu_buffer_rsrc_t f, int row, int lo, int hi) -> C {
+        const bool ok = ld_ok && row >= lo && row < hi;
+        const unsigned off =
+            ok ? (unsigned)(((long)(row - base_row) * X + c) * (long)sizeof(T)) : kBufferOutside;
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(f, off, 0, 0));
+    };
+    const __amdgpu_buffer_rsrc_t b_sd = rsrc(FROM_R ? r_in : sd_in), b_r = rsrc(r_in),
+                                 b_u = rsrc(u_in), b_kx = rsrc(kx), b_ky = rsrc(ky),
+                                 b_p = rsrc(UR ? p_in : u_in), b_w = rsrc(UR ? w_in : u_in);
     auto over_theta = [&](C a) -> C {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.v[i] = a.v[i] / theta;
@@ -121,8 +146,8 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
         return a;
     };
     auto r_level0 = [&](int row) -> C {
-        C a = ld_field(r_in, row, yin0, yin1);
-        if constexpr (UR) a = minus_aw(a, ld_field(w_in, row, yin0, yin1));
+        C a = ld_field(b_r, row, yin0, yin1);
+        if constexpr (UR) a = minus_aw(a, ld_field(b_w, row, yin0, yin1));
         return a;
     };
 
@@ -133,13 +158,13 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
         rcarry = r_level0(rs);
         sdw[0][2] = over_theta(rcarry);
     } else {
-        sdw[0][1] = ld_field(sd_in, rs - 1, yin0, yin1);
-        sdw[0][2] = ld_field(sd_in, rs, yin0, yin1);
+        sdw[0][1] = ld_field(b_sd, rs - 1, yin0, yin1);
+        sdw[0][2] = ld_field(b_sd, rs, yin0, yin1);
     }
 #pragma unroll
-    for (int m = 0; m + 1 < S; ++m) kxw[m] = ld_field(kx, rs - 1 - m, yin0, yin1);
+    for (int m = 0; m + 1 < S; ++m) kxw[m] = ld_field(b_kx, rs - 1 - m, yin0, yin1);
 #pragma unroll
-    for (int m = 0; m < S; ++m) kyw[m] = ld_field(ky, rs - m, yin0, yin1 + 1);
+    for (int m = 0; m < S; ++m) kyw[m] = ld_field(b_ky, rs - m, yin0, yin1 + 1);
 
     // Level-0 inputs of the iteration whose level 1 computes row rho1:
     // a = sd (or r for FROM_R) at rho1+1, b = r at rho1 (or, for UR, w at
@@ -151,28 +176,63 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
     };
     auto load_in = [&](int rho1) -> In {
         In v;
-        v.a = ld_field(FROM_R ? r_in : sd_in, rho1 + 1, yin0, yin1);
-        if constexpr (!FROM_R) v.b = ld_field(r_in, rho1, yin0, yin1);
+        v.a = ld_field(b_sd, rho1 + 1, yin0, yin1);
+        if constexpr (!FROM_R) v.b = ld_field(b_r, rho1, yin0, yin1);
         if constexpr (UR) {
-            v.b = ld_field(w_in, rho1 + 1, yin0, yin1);
-            v.p = ld_field(p_in, rho1, yin0, yin1);
+            v.b = ld_field(b_w, rho1 + 1, yin0, yin1);
+            v.p = ld_field(b_p, rho1, yin0, yin1);
         }
-        v.u = ld_field(u_in, rho1, yin0, yin1);
-        v.kx = ld_field(kx, rho1, yin0, yin1);
-        v.ky = ld_field(ky, rho1 + 1, yin0, yin1 + 1);
+        v.u = ld_field(b_u, rho1, yin0, yin1);
+        v.kx = ld_field(b_kx, rho1, yin0, yin1);
+        v.ky = ld_field(b_ky, rho1 + 1, yin0, yin1 + 1);
         return v;
     };
-    In cur;
-    if constexpr (PF) cur = load_in(rs);
+    // With PF > 0 the loads of the row PF iterations ahead are issued
+    // before this iteration's arithmetic, which then hides their latency.
+    In pre[PF > 0 ? PF : 1];
+    if constexpr (PF > 0) {
+#pragma unroll
+        for (int q = 0; q < PF; ++q) pre[q] = load_in(rs + q);
+    }
 
-    for (int it = 0; it < niter; ++it) {
+    // With PRE, the diagonal and next-lane kx of rows rho1 - m (m < S) are
+    // formed once per row for all the levels that use it; deep loops (whose
+    // windows already fill the registers) form them per level instead.
+    constexpr bool PRE = S <= 6;
+    constexpr int SW = PRE ? S : 1;
+    C dgw[SW];
+    T kxew[SW];
+#pragma unroll
+    for (int m = 0; m < SW; ++m) {
+        dgw[m] = zeros<T>();
+        kxew[m] = T(0);
+    }
+    if constexpr (PRE) {
+#pragma unroll
+        for (int m = 0; m + 1 < S; ++m) {  // rows rs-1-m, from the prologue's kx, ky
+            kxew[m] = kx_next_lane(kxw[m]);
+            dgw[m] = diag_lane(kxw[m], kxew[m], kyw[m], kyw[m + 1]);
+        }
+    }
+
+    // A wave whose window holds neither the first nor the last interior
+    // column never mirrors in x; an iteration none of whose levels is on
+    // the first or last interior row never mirrors in y.  The loop body is
+    // instantiated for each case so that the common one carries no tests.
+    const int wc0 = t.cw0 + sx * t.w;
+    const bool xedge = (xin0 >= wc0 && xin0 < wc0 + TL_WAVE * N) ||
+                       (xin1 - 1 >= wc0 && xin1 - 1 < wc0 + TL_WAVE * N);
+
+    auto body = [&](int it, auto xm, auto ym) __attribute__((always_inline)) {
+        constexpr bool XM = decltype(xm)::value;
+        constexpr bool YM = decltype(ym)::value;
         const int rho1 = rs + it;
-        // With PF the next iteration's loads are issued before this
-        // iteration's arithmetic, which then hides their latency.
-        In nxt;
-        if constexpr (PF) {
-            if (it + 1 < niter) nxt = load_in(rho1 + 1);
-            else nxt = cur;
+        In cur;
+        if constexpr (PF > 0) {
+            cur = pre[0];
+#pragma unroll
+            for (int q = 0; q + 1 < PF; ++q) pre[q] = pre[q + 1];
+            pre[PF - 1] = load_in(rho1 + PF);    // rows past the tile: loaded, unused
         } else {
             cur = load_in(rho1);
         }
@@ -195,7 +255,15 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
 #pragma unroll
         for (int m = S; m > 0; --m) kyw[m] = kyw[m - 1];
         kyw[0] = cur.ky;
-        if constexpr (PF) cur = nxt;
+        if constexpr (PRE) {
+#pragma unroll
+            for (int m = S - 1; m > 0; --m) {
+                kxew[m] = kxew[m - 1];
+                dgw[m] = dgw[m - 1];
+            }
+            kxew[0] = kx_next_lane(kxw[0]);
+            dgw[0] = diag_lane(kxw[0], kxew[0], kyw[0], kyw[1]);
+        }
 
         C rcur[S], ucur[S];
 #pragma unroll
@@ -203,14 +271,25 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
             const int rho = rho1 - (k - 1);
             const C cen = sdw[k - 1][1];
             // y neighbours; outside the interior they mirror the cell
-            const C upv = rho == yin1 - 1 ? cen : sdw[k - 1][2];
-            const C dnv = rho == yin0 ? cen : sdw[k - 1][0];
-            const C kxc = kxw[k - 1];
-            const C kyu = kyw[k - 1], kyc = kyw[k];
+            C upv = sdw[k - 1][2], dnv = sdw[k - 1][0];
+            if constexpr (YM) {
+                if (rho == yin1 - 1) upv = cen;
+                if (rho == yin0) dnv = cen;
+            }
             const C rin = k == 1 ? r0 : rprev[k - 1];
             const C uin = k == 1 ? u0 : uprev[k - 1];
             // SMVP (2d/shared.h:50-54) and the ppcg.c:46-59 updates, same order
-            const C av = apply_a_lane(cen, upv, dnv, kxc, kyu, kyc, c, xin0, xin1);
+            T kxe;
+            C dg;
+            if constexpr (PRE) {
+                kxe = kxew[k - 1];
+                dg = dgw[k - 1];
+            } else {
+                kxe = kx_next_lane(kxw[k - 1]);
+                dg = diag_lane(kxw[k - 1], kxe, kyw[k - 1], kyw[k]);
+            }
+            const C av = apply_a_pre<XM>(cen, upv, dnv, kxw[k - 1], kxe, kyw[k - 1], kyw[k], dg, c,
+                                         xin0, xin1);
             C rn, un, sn;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
@@ -245,6 +324,21 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
         for (int k = 1; k < S; ++k) {
             rprev[k] = rcur[k - 1];
             uprev[k] = ucur[k - 1];
+        }
+    };
+
+    using Yes = std::integral_constant<bool, true>;
+    using No = std::integral_constant<bool, false>;
+    for (int it = 0; it < niter; ++it) {
+        // levels of this iteration cover rows rho1 - S + 1 .. rho1
+        const int top = rs + it, bot = top - S + 1;
+        const bool yedge = (yin0 >= bot && yin0 <= top) || (yin1 - 1 >= bot && yin1 - 1 <= top);
+        if (xedge) {
+            if (yedge) body(it, Yes{}, Yes{});
+            else body(it, Yes{}, No{});
+        } else {
+            if (yedge) body(it, No{}, Yes{});
+            else body(it, No{}, No{});
         }
     }
 }
@@ -411,7 +505,7 @@ int device_cus()
 // resident capacity for every S (S <= 5: 2 blocks per CU; S >= 6, over
 // 256 VGPRs: 1), e.g. S = 10 at 225 blocks (ty 164) ran 391 us and at
 // 234 blocks (ty 158) 657 us.
-template <class T, int S, int MODE, bool PF>
+template <class T, int S, int MODE, int PF>
 int auto_tile_rows(const PitchGrid& g, int nbx)
 {
     static int occ = 0;
@@ -429,14 +523,25 @@ template <class T, int S, int MODE>
 int launch_fused(const FusedArgs<T>& a)
 {
     // Windows reach S columns beyond their output columns on both sides.
-    const bool pf = g_fused_pf != 0;
+    const int pf = g_fused_pf < 0 ? 1 : std::min(g_fused_pf, 3);
     StripTile t = strip_tile<T>(a.g, S, 1, kWaves);
-    const int ty = g_fused_ty > 0 ? g_fused_ty
-                   : pf ? auto_tile_rows<T, S, MODE, true>(a.g, t.nbx)
-                        : auto_tile_rows<T, S, MODE, false>(a.g, t.nbx);
+    int ty = g_fused_ty;
+    if (ty <= 0) {
+        switch (pf) {
+        case 0: ty = auto_tile_rows<T, S, MODE, 0>(a.g, t.nbx); break;
+        case 1: ty = auto_tile_rows<T, S, MODE, 1>(a.g, t.nbx); break;
+        case 2: ty = auto_tile_rows<T, S, MODE, 2>(a.g, t.nbx); break;
+        default: ty = auto_tile_rows<T, S, MODE, 3>(a.g, t.nbx); break;
+        }
+    }
     t = strip_tile<T>(a.g, S, ty, kWaves);
     if ((long)t.nbx * t.nby > (1L << 30))
         return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", a.what);
+    // The rows one tile reads (ty + 2S - 2 marched, S below, PF + 1 above)
+    // must lie within one buffer descriptor's range.
+    if ((long)(t.ty + 3 * S + pf + 2) * a.g.X * (long)sizeof(T) > (long)kBufferRange)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %d-row tiles of %d-element rows exceed a buffer range",
+                       a.what, t.ty, a.g.X);
     PpcgCoefs<T, S> cf;
     for (int k = 0; k < S; ++k) {
         cf.alpha[k] = (T)a.al[k];
@@ -444,16 +549,17 @@ int launch_fused(const FusedArgs<T>& a)
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
     const int slot = tl_timer_begin(a.what);
-    if (pf)
-        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, true>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev,
-                           a.sd_in, a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p,
-                           a.w, tl_gate());
-    else
-        hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, false>), dim3(nb), dim3(kBlock), 0,
-                           tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev,
-                           a.sd_in, a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p,
-                           a.w, tl_gate());
+#define TL_FUSED_LAUNCH(PFD)                                                                   \
+    hipLaunchKernelGGL((k_ppcg_fused2d<T, S, MODE, PFD>), dim3(nb), dim3(kBlock), 0, tl_stream(), \
+                       a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev, a.sd_in, a.sd_out,     \
+                       a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w, tl_gate())
+    switch (pf) {
+    case 0: TL_FUSED_LAUNCH(0); break;
+    case 1: TL_FUSED_LAUNCH(1); break;
+    case 2: TL_FUSED_LAUNCH(2); break;
+    default: TL_FUSED_LAUNCH(3); break;
+    }
+#undef TL_FUSED_LAUNCH
     tl_timer_end(slot);
     return tl_check_launch(a.what);
 }

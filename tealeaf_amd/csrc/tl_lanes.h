@@ -126,4 +126,57 @@ __device__ __forceinline__ Cells<T> apply_a_lane(const Cells<T>& cen, const Cell
                         kxc.v[N - 1]);
 }
 
+// The coefficient parts of that product that do not depend on a: the
+// diagonal (1 + (kx[i+1] + kx[i])) + (ky[i+x] + ky[i]) of each cell and the
+// kx[i+1] of the lane's last cell (the next lane's first kx; the wave's end
+// lane gets a don't-care).  A kernel that applies A to several fields on
+// the same row forms them once; the values are the same bits either way.
+template <class T>
+__device__ __forceinline__ T kx_next_lane(const Cells<T>& kxc)
+{
+    constexpr int N = Lane16<T>::N;
+    return tl_lane_next(kxc.v[0], kxc.v[N - 1]);
+}
+
+template <class T>
+__device__ __forceinline__ Cells<T> diag_lane(const Cells<T>& kxc, T kxe, const Cells<T>& kyu,
+                                              const Cells<T>& kyc)
+{
+    constexpr int N = Lane16<T>::N;
+    Cells<T> d;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const T kxr = i == N - 1 ? kxe : kxc.v[i + 1];
+        d.v[i] = (T(1) + (kxr + kxc.v[i])) + (kyu.v[i] + kyc.v[i]);
+    }
+    return d;
+}
+
+// apply_a_lane (don't-care wave ends) with the diagonal and next-lane kx
+// formed beforehand.  XMIRROR: the lane's cells may include the first or
+// last interior column (a wave whose window holds neither skips the tests).
+template <bool XMIRROR, class T>
+__device__ __forceinline__ Cells<T> apply_a_pre(const Cells<T>& cen, const Cells<T>& up,
+                                                const Cells<T>& dn, const Cells<T>& kxc, T kxe,
+                                                const Cells<T>& kyu, const Cells<T>& kyc,
+                                                const Cells<T>& d, int c, int xin0, int xin1)
+{
+    constexpr int N = Lane16<T>::N;
+    const T lft_edge = tl_lane_prev(cen.v[N - 1], cen.v[0]);
+    const T rgt_edge = tl_lane_next(cen.v[0], cen.v[N - 1]);
+    Cells<T> a;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T lft = i == 0 ? lft_edge : cen.v[i - 1];
+        T rgt = i == N - 1 ? rgt_edge : cen.v[i + 1];
+        if constexpr (XMIRROR) {
+            if (c + i == xin0) lft = cen.v[i];
+            if (c + i == xin1 - 1) rgt = cen.v[i];
+        }
+        const T kxr = i == N - 1 ? kxe : kxc.v[i + 1];
+        a.v[i] = d.v[i] * cen.v[i] - (kxr * rgt + kxc.v[i] * lft) - (kyu.v[i] * up.v[i] + kyc.v[i] * dn.v[i]);
+    }
+    return a;
+}
+
 }  // namespace tl
