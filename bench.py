@@ -210,7 +210,7 @@ def profile_run(args):
     for k in ("kx", "ky"):
         f64[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, N))
     f32 = {k: DeviceArray.from_host(rng.uniform(-1, 1, P * y).astype(np.float32))
-           for k in ("u", "u2", "r", "r2", "sd", "sd2")}
+           for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
     for k in ("kx", "ky"):
         f32[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, P * y).astype(np.float32))
     ncopy = (1 << 30) // 8
@@ -220,21 +220,23 @@ def profile_run(args):
     pw = C.c_double()
     for _ in range(3):
         L.tlhip_stream_copy(dst.ptr, src.ptr, C.c_size_t(ncopy))
-        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
-                                        nb.ctypes.data, f64["sd"].ptr, f64["sd2"].ptr, f64["r"].ptr,
-                                        f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
-                                        f64["ky"].ptr, 0.0, None, None)
+        # the driver's launch: cg_calc_ur + ppcg_init + 10 inner steps (MODE 2)
         L.tlhip2d_ppcg_inner_iterations(x, y, hd, 10, al.ctypes.data, be.ctypes.data, 1.3, 1,
                                         nb.ctypes.data, None, f64["sd2"].ptr, f64["r"].ptr,
                                         f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
                                         f64["ky"].ptr, 0.3, f64["p"].ptr, f64["w"].ptr)
+        # 10 inner steps continuing from sd (MODE 0)
+        L.tlhip2d_ppcg_inner_iterations(x, y, hd, 10, al.ctypes.data, be.ctypes.data, 1.3, 0,
+                                        nb.ctypes.data, f64["sd"].ptr, f64["sd2"].ptr, f64["r"].ptr,
+                                        f64["r2"].ptr, f64["u"].ptr, f64["u2"].ptr, f64["kx"].ptr,
+                                        f64["ky"].ptr, 0.0, None, None)
         L.tlhip2d_cg_calc_pw(x, y, hd, 0.5, C.addressof(pw), f64["p"].ptr, f64["sd"].ptr,
                              f64["r"].ptr, f64["w"].ptr, f64["kx"].ptr, f64["ky"].ptr)
-        L.tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, 5, al.ctypes.data, be.ctypes.data, 1.3, 0,
-                                            nb.ctypes.data, f32["sd"].ptr, f32["sd2"].ptr,
+        L.tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, 10, al.ctypes.data, be.ctypes.data, 1.3, 1,
+                                            nb.ctypes.data, None, f32["sd2"].ptr,
                                             f32["r"].ptr, f32["r2"].ptr, f32["u"].ptr,
-                                            f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr, 0.0,
-                                            None, None)
+                                            f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr, 0.3,
+                                            f32["p"].ptr, f32["w"].ptr)
         L.tlhip2d_ppcg_inner_iteration(x, y, hd, 0.7, 0.2, f64["u"].ptr, f64["r"].ptr, f64["kx"].ptr,
                                        f64["ky"].ptr, f64["sd"].ptr, f64["sd2"].ptr)
         L.tlhip2d_cg_calc_w(x, y, hd, C.addressof(pw), f64["p"].ptr, f64["w"].ptr, f64["kx"].ptr,

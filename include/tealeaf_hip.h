@@ -364,8 +364,8 @@ int tlhip_set_cg_fusion(int on);
 int tlhip_set_resident(int on);
 /* Steps per fused PPCG launch used by the driver (1..10; 0 or 1 = one step
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 10: the
- * reference deck's 10 inner steps in one FP64 launch (the FP32 sweeps of
- * the mixed solve use at most 5 per launch). */
+ * reference deck's 10 inner steps in one launch (FP64, and FP32 in the
+ * mixed-precision solve). */
 int tlhip_set_ppcg_fusion(int steps);
 
 /* How tlhip_problem_step solves (no counterpart in the reference, which

@@ -39,14 +39,12 @@ namespace {
 
 enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 
-// Inner steps per fused PPCG launch (2D, single chunk).  FP64: the whole
-// inner loop of the default deck (10 steps) in one launch, which at 4096^2
-// on MI355X runs 10 steps in ~0.39-0.47 ms against ~0.25 ms per 5 (one pass
-// over memory either way; the deeper kernel holds 1 wave per SIMD instead
-// of 2).  The FP32 sweeps of the mixed solve stay at 5 per launch, where
-// the 10-step launch measured no faster.
+// Inner steps per fused PPCG launch (2D, single chunk): the whole inner
+// loop of the default deck (10 steps) in one launch.  At 4096^2 on MI355X
+// 10 FP64 steps take ~0.43 ms in one launch against ~0.25 ms per 5 (one
+// pass over memory either way; the deeper kernel holds 1 wave per SIMD
+// instead of 2); in FP32 at 8192^2, 0.94 ms against 0.50 + 0.56 ms.
 int g_ppcg_fusion = 10;
-constexpr int kFusionF32 = 5;
 int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
 int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 
@@ -834,7 +832,7 @@ struct tlhip_problem {
             double unused = 0.0;
             TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
         }
-        const int per = std::max(1, std::min(g_ppcg_fusion, kFusionF32));
+        const int per = std::max(1, g_ppcg_fusion);
         int done = 0;
         while (done < s.ppcg_inner_steps) {
             // the first launch also applies calc_ur (u is d here)
