@@ -193,6 +193,19 @@ int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int halo_depth, int pitch,
 int tlhip2d_cg_calc_pw(int x, int y, int halo_depth, double beta, double* pw,
         const double* p_in, double* p_out, const double* r, double* w,
         const double* kx, const double* ky);
+/* Temporally blocked Chebyshev loop, 1 <= count <= 10 iterations in one
+ * pass.  No single counterpart in the kernel interface: it replaces `count`
+ * turns of the Chebyshev loop of 2d/drivers/cheby_driver.c (a depth-1 halo
+ * update of u, then run_cheby_iterate, 2d/kernel_interface.h:48, with
+ * alphas[i], betas[i]) for a chunk whose four faces are external.  u and p
+ * are read from *_in and written to *_out (interiors bit-exact with the
+ * reference's in-place results, halo rings carried over); r receives the
+ * last iteration's r; w is not written (nothing reads it before cg_init or
+ * cg_calc_w rewrite it).  Needs an even x and 16-byte aligned fields. */
+int tlhip2d_cheby_iterations(int x, int y, int halo_depth, int count,
+        const double* alphas, const double* betas, const int* neighbours,
+        const double* u_in, double* u_out, const double* u0, const double* p_in,
+        double* p_out, double* r, const double* kx, const double* ky);
 /* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
  * the height whose block count fills ~0.88 of what the device holds at
  * once, from the kernel's occupancy and the CU count) and next-row prefetch
@@ -367,6 +380,10 @@ int tlhip_set_resident(int on);
  * reference deck's 10 inner steps in one launch (FP64, and FP32 in the
  * mixed-precision solve). */
 int tlhip_set_ppcg_fusion(int steps);
+/* Chebyshev iterations per fused launch used by the 2D driver with the
+ * reference stop test (0 or 1 = one tlhip2d_cheby_iterate per iteration).
+ * Default 10; results are identical either way. */
+int tlhip_set_cheby_fusion(int steps);
 
 /* How tlhip_problem_step solves (no counterpart in the reference, which
  * always solves in FP64 and stops on tl_eps):

@@ -45,6 +45,8 @@ enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 // pass over memory either way; the deeper kernel holds 1 wave per SIMD
 // instead of 2); in FP32 at 8192^2, 0.94 ms against 0.50 + 0.56 ms.
 int g_ppcg_fusion = 10;
+// Chebyshev iterations per fused launch (2D, FP64, reference stop test).
+int g_cheby_fusion = 10;
 int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
 int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 
@@ -613,6 +615,30 @@ struct tlhip_problem {
                 TRY(halo_update(1));
                 TRY(cheby_step(ncheby, true, error));
                 est = est_iterations(*error, bb);
+            } else if (fused_cheby()) {
+                // Iterations tt, tt+1, ... in one pass, up to and including
+                // the next one that computes the norm (the only ones after
+                // which the stop test can change its answer), or max_iters;
+                // the depth-1 halo updates of u in between are the kernel's
+                // mirror, the last one is the loop's own below.
+                auto norm_at = [&](int t, int k) { return k >= est && (t + 1) % 10 == 0; };
+                int cnt = 1;
+                while (!norm_at(tt + cnt - 1, ncheby + cnt - 1) && tt + cnt < s.max_iters &&
+                       cnt < g_cheby_fusion)
+                    ++cnt;
+                TRY(tlhip2d_cheby_iterations(x, y, hd, cnt, &cheby_alphas[ncheby],
+                                             &cheby_betas[ncheby], neighbours, u, u_alt, u0, p,
+                                             p_alt, r, kx, ky));
+                std::swap(u, u_alt);
+                std::swap(p, p_alt);
+                rep->n_cheby_steps += cnt;
+                const bool calc = norm_at(tt + cnt - 1, ncheby + cnt - 1);
+                ncheby += cnt - 1;
+                tt += cnt - 1;
+                if (calc) {
+                    *error = 0.0;
+                    TRY(norm2(r, error));
+                }
             } else {
                 TRY(cheby_step(ncheby, ncheby >= est && (tt + 1) % 10 == 0, error));
             }
@@ -626,6 +652,15 @@ struct tlhip_problem {
         rep->printed_inner_iters = ncheby;
         rep->est_iters = est;
         return 0;
+    }
+
+    // Several Chebyshev iterations per pass (2D).  With the relative stop
+    // the reference-order loop re-tests the true residual every iteration
+    // after a refused candidate, so batching is only exact for the
+    // reference's own test.
+    bool fused_cheby() const
+    {
+        return dims == 2 && g_cheby_fusion > 1 && x % 2 == 0 && p_alt && u_alt && rel_tol <= 0.0;
     }
 
     // -------------------------------------------------------- PPCG
@@ -1107,6 +1142,13 @@ extern "C" int tlhip_set_resident(int on)
     return 0;
 }
 
+extern "C" int tlhip_set_cheby_fusion(int steps)
+{
+    if (steps < 0 || steps > 10) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_cheby_fusion: 0..10");
+    g_cheby_fusion = steps;
+    return 0;
+}
+
 extern "C" int tlhip_set_cg_fusion(int on)
 {
     g_cg_fusion = on ? 1 : 0;
@@ -1183,7 +1225,7 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
         P->kz = dalloc(n, &rc);
         P->z_area = dalloc(n, &rc);
     } else {
-        P->p_alt = dalloc(n, &rc);      // partner of p for the fused calc_p + calc_w
+        P->p_alt = dalloc(n, &rc);      // partner of p: fused calc_p + calc_w, fused Chebyshev
         if (s->solver == 3) P->r_alt = dalloc(n, &rc);   // partner of r for the fused PPCG loop
     }
     P->cell_x = dalloc(P->x + 1, &rc);

@@ -343,6 +343,237 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
     }
 }
 
+// Temporally blocked Chebyshev iterations (2D, FP64): S consecutive turns
+// of the Chebyshev loop of 2d/drivers/cheby_driver.c (a depth-1 halo update
+// of u, then cheby_iterate: w = A u, r = u0 - w, p = alpha_k p + beta_k r,
+// then cheby_calc_u: u += p; 2d/c_kernels/omp3/cheby.c:8-24,57-85) in one
+// pass.  The scheme is the PPCG loop's: each level needs only level k-1
+// within one cell, the halo update is a mirror, a wave carries S levels
+// while it marches up its rows and writes only its centre columns.  Per
+// level it keeps three rows of u and the row of p the next level needs;
+// u0, kx and ky rows are shared by all levels.  r and p of the last level
+// and u are stored; w is not (nothing reads it before cg_init or cg_calc_w
+// rewrite it).  Bit-exact with S single iterations.
+template <class T, int S, int PF>
+__global__ void __launch_bounds__(kBlock)
+k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
+                const T* __restrict__ u_in, T* __restrict__ u_out,
+                const T* __restrict__ u0f, const T* __restrict__ p_in, T* __restrict__ p_out,
+                T* __restrict__ r_out, const T* __restrict__ kx, const T* __restrict__ ky)
+{
+    constexpr int N = Lane16<T>::N;
+    using C = Cells<T>;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
+    const int bx = tile % t.nbx, by = tile / t.nbx;
+    const int sx = bx * kWaves + wave;
+    if (sx >= t.ns) return;                      // uniform per wave; no block sync below
+
+    const int xin0 = g.hd, xin1 = g.hd + g.nx;
+    const int yin0 = g.hd, yin1 = g.hd + g.ny;
+    const int o0 = xin0 + sx * t.w;
+    const int o1 = min(o0 + t.w, xin1);
+    const int c = t.cw0 + sx * t.w + N * lane;
+    const bool ld_ok = c >= 0 && c + N <= g.X;
+    const bool all_out = c >= o0 && c + N <= o1;
+    const bool any_out = c + N > o0 && c < o1;
+
+    const int j0 = yin0 + by * t.ty;
+    const int j1 = min(j0 + t.ty, yin1);
+    const int rs = j0 - S + 1;
+    const int niter = j1 - j0 + 2 * S - 2;
+    const long X = g.X;
+
+    // Buffer-descriptor loads, as in the PPCG loop.
+    const int base_row = max(rs - S, 0);
+    const long span = (long)(g.ny + 2 * g.hd - base_row) * X * (long)sizeof(T);
+    const int nrec = (int)min(span, (long)kBufferRange);
+    auto rsrc = [&](const T* f) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(f + (long)base_row * X), (short)0, nrec,
+                                                 kBufferWord3);
+    };
+    auto ld_field = [&](__amdgpu_buffer_rsrc_t f, int row, int lo, int hi) -> C {
+        const bool ok = ld_ok && row >= lo && row < hi;
+        const unsigned off =
+            ok ? (unsigned)(((long)(row - base_row) * X + c) * (long)sizeof(T)) : kBufferOutside;
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(f, off, 0, 0));
+    };
+    const __amdgpu_buffer_rsrc_t b_u = rsrc(u_in), b_u0 = rsrc(u0f), b_p = rsrc(p_in),
+                                 b_kx = rsrc(kx), b_ky = rsrc(ky);
+
+    // Windows (index 0 oldest).  uw[k][*]: rows of u at level k for k < S;
+    // pprev[k]: level k's p from the previous iteration; u0w, kxw: rows
+    // rho1 - m; kyw: rows rho1 + 1 - m.
+    C uw[S][3];
+    C pprev[S];
+    C u0w[S], kxw[S], kyw[S + 1];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        uw[k][0] = uw[k][1] = uw[k][2] = zeros<T>();
+        pprev[k] = u0w[k] = kxw[k] = zeros<T>();
+    }
+#pragma unroll
+    for (int m = 0; m <= S; ++m) kyw[m] = zeros<T>();
+
+    // Prologue: level-0 u rows rs-1, rs; u0 and kx rows below rs; ky rows up to rs.
+    uw[0][1] = ld_field(b_u, rs - 1, yin0, yin1);
+    uw[0][2] = ld_field(b_u, rs, yin0, yin1);
+#pragma unroll
+    for (int m = 0; m + 1 < S; ++m) {
+        kxw[m] = ld_field(b_kx, rs - 1 - m, yin0, yin1);
+        u0w[m] = ld_field(b_u0, rs - 1 - m, yin0, yin1);
+    }
+#pragma unroll
+    for (int m = 0; m < S; ++m) kyw[m] = ld_field(b_ky, rs - m, yin0, yin1 + 1);
+
+    // Level-0 inputs of the iteration whose level 1 computes row rho1:
+    // u at rho1+1, p, u0 and kx at rho1, ky at rho1+1.
+    struct In {
+        C u, p, u0, kx, ky;
+    };
+    auto load_in = [&](int rho1) -> In {
+        In v;
+        v.u = ld_field(b_u, rho1 + 1, yin0, yin1);
+        v.p = ld_field(b_p, rho1, yin0, yin1);
+        v.u0 = ld_field(b_u0, rho1, yin0, yin1);
+        v.kx = ld_field(b_kx, rho1, yin0, yin1);
+        v.ky = ld_field(b_ky, rho1 + 1, yin0, yin1 + 1);
+        return v;
+    };
+    In pre[PF > 0 ? PF : 1];
+    if constexpr (PF > 0) {
+#pragma unroll
+        for (int q = 0; q < PF; ++q) pre[q] = load_in(rs + q);
+    }
+
+    constexpr bool PRE = S <= 6;
+    constexpr int SW = PRE ? S : 1;
+    C dgw[SW];
+    T kxew[SW];
+#pragma unroll
+    for (int m = 0; m < SW; ++m) {
+        dgw[m] = zeros<T>();
+        kxew[m] = T(0);
+    }
+    if constexpr (PRE) {
+#pragma unroll
+        for (int m = 0; m + 1 < S; ++m) {
+            kxew[m] = kx_next_lane(kxw[m]);
+            dgw[m] = diag_lane(kxw[m], kxew[m], kyw[m], kyw[m + 1]);
+        }
+    }
+
+    const int wc0 = t.cw0 + sx * t.w;
+    const bool xedge = (xin0 >= wc0 && xin0 < wc0 + TL_WAVE * N) ||
+                       (xin1 - 1 >= wc0 && xin1 - 1 < wc0 + TL_WAVE * N);
+
+    auto body = [&](int it, auto xm, auto ym) __attribute__((always_inline)) {
+        constexpr bool XM = decltype(xm)::value;
+        constexpr bool YM = decltype(ym)::value;
+        const int rho1 = rs + it;
+        In cur;
+        if constexpr (PF > 0) {
+            cur = pre[0];
+#pragma unroll
+            for (int q = 0; q + 1 < PF; ++q) pre[q] = pre[q + 1];
+            pre[PF - 1] = load_in(rho1 + PF);
+        } else {
+            cur = load_in(rho1);
+        }
+        uw[0][0] = uw[0][1];
+        uw[0][1] = uw[0][2];
+        uw[0][2] = cur.u;
+#pragma unroll
+        for (int m = S - 1; m > 0; --m) {
+            kxw[m] = kxw[m - 1];
+            u0w[m] = u0w[m - 1];
+        }
+        kxw[0] = cur.kx;
+        u0w[0] = cur.u0;
+#pragma unroll
+        for (int m = S; m > 0; --m) kyw[m] = kyw[m - 1];
+        kyw[0] = cur.ky;
+        if constexpr (PRE) {
+#pragma unroll
+            for (int m = S - 1; m > 0; --m) {
+                kxew[m] = kxew[m - 1];
+                dgw[m] = dgw[m - 1];
+            }
+            kxew[0] = kx_next_lane(kxw[0]);
+            dgw[0] = diag_lane(kxw[0], kxew[0], kyw[0], kyw[1]);
+        }
+
+        C pcur[S];
+#pragma unroll
+        for (int k = 1; k <= S; ++k) {
+            const int rho = rho1 - (k - 1);
+            const C cen = uw[k - 1][1];
+            C upv = uw[k - 1][2], dnv = uw[k - 1][0];
+            if constexpr (YM) {
+                if (rho == yin1 - 1) upv = cen;
+                if (rho == yin0) dnv = cen;
+            }
+            T kxe;
+            C dg;
+            if constexpr (PRE) {
+                kxe = kxew[k - 1];
+                dg = dgw[k - 1];
+            } else {
+                kxe = kx_next_lane(kxw[k - 1]);
+                dg = diag_lane(kxw[k - 1], kxe, kyw[k - 1], kyw[k]);
+            }
+            const C pin = k == 1 ? cur.p : pprev[k - 1];
+            // SMVP (2d/shared.h:50-54) and cheby.c:76-84, same order
+            const C wv = apply_a_pre<XM>(cen, upv, dnv, kxw[k - 1], kxe, kyw[k - 1], kyw[k], dg,
+                                         c, xin0, xin1);
+            C rn, pn, un;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                rn.v[i] = u0w[k - 1].v[i] - wv.v[i];
+                pn.v[i] = cf.alpha[k - 1] * pin.v[i] + cf.beta[k - 1] * rn.v[i];
+                un.v[i] = cen.v[i] + pn.v[i];
+            }
+            pcur[k - 1] = pn;
+            if (k < S) {
+                uw[k][0] = uw[k][1];
+                uw[k][1] = uw[k][2];
+                uw[k][2] = un;
+            } else if (any_out && rho >= j0 && rho < j1) {
+                const long i0 = (long)rho * X + c;
+                if (all_out) {
+                    stc(u_out, i0, un);
+                    stc(p_out, i0, pn);
+                    stc(r_out, i0, rn);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (c + i >= o0 && c + i < o1) {
+                            u_out[i0 + i] = un.v[i];
+                            p_out[i0 + i] = pn.v[i];
+                            r_out[i0 + i] = rn.v[i];
+                        }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 1; k < S; ++k) pprev[k] = pcur[k - 1];
+    };
+
+    using Yes = std::integral_constant<bool, true>;
+    using No = std::integral_constant<bool, false>;
+    for (int it = 0; it < niter; ++it) {
+        const int top = rs + it, bot = top - S + 1;
+        const bool yedge = (yin0 >= bot && yin0 <= top) || (yin1 - 1 >= bot && yin1 - 1 <= top);
+        if (xedge) {
+            if (yedge) body(it, Yes{}, Yes{});
+            else body(it, Yes{}, No{});
+        } else {
+            if (yedge) body(it, No{}, Yes{});
+            else body(it, No{}, No{});
+        }
+    }
+}
+
 // w = A p, sum w*p (cg.c:88-113) with the depth-1 mirror built in; with
 // UPDATE, p is first replaced by beta p + r (cg_calc_p, cg.c:146-164),
 // computed at every cell the stencil reads and stored to a second buffer.
@@ -631,6 +862,88 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
     return dispatch_fused<T, 2>(count, a);
 }
 
+template <int S>
+int launch_cheby(const PitchGrid& g, const double* al, const double* be, const double* u_in,
+                 double* u_out, const double* u0, const double* p_in, double* p_out, double* r,
+                 const double* kx, const double* ky, const char* what)
+{
+    const int pf = g_fused_pf < 0 ? 1 : std::min(g_fused_pf, 1);
+    StripTile t = strip_tile<double>(g, S, 1, kWaves);
+    static int occ = 0;
+    if (occ <= 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cheby_fused2d<double, S, 1>, kBlock,
+                                                      0) != hipSuccess ||
+         occ <= 0))
+        occ = 1;
+    int ty = g_fused_ty;
+    if (ty <= 0) {
+        const long cap = (long)(0.88 * device_cus() * occ);
+        const long nby = std::max(1L, cap / std::max(1, t.nbx));
+        ty = (int)std::max(1L, ((long)g.ny + nby - 1) / nby);
+    }
+    t = strip_tile<double>(g, S, ty, kWaves);
+    if ((long)t.nbx * t.nby > (1L << 30))
+        return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    if ((long)(t.ty + 3 * S + pf + 2) * g.X * (long)sizeof(double) > (long)kBufferRange)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %d-row tiles of %d-element rows exceed a buffer range",
+                       what, t.ty, g.X);
+    PpcgCoefs<double, S> cf;
+    for (int k = 0; k < S; ++k) {
+        cf.alpha[k] = al[k];
+        cf.beta[k] = be[k];
+    }
+    const unsigned nb = (unsigned)(t.nbx * t.nby);
+    const int slot = tl_timer_begin(what);
+    if (pf)
+        hipLaunchKernelGGL((k_cheby_fused2d<double, S, 1>), dim3(nb), dim3(kBlock), 0, tl_stream(),
+                           g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky);
+    else
+        hipLaunchKernelGGL((k_cheby_fused2d<double, S, 0>), dim3(nb), dim3(kBlock), 0, tl_stream(),
+                           g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky);
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
+int cheby_iterations(int x, int y, int hd, int count, const double* alphas, const double* betas,
+                     const int* neighbours, const double* u_in, double* u_out, const double* u0,
+                     const double* p_in, double* p_out, double* r, const double* kx,
+                     const double* ky)
+{
+    const char* what = "tlhip2d_cheby_iterations";
+    TL_TRY(tl_ensure_init());
+    if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || x % 2 || (long)x * y > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x (x=%d y=%d hd=%d)", what, x, y, hd);
+    if (count < 1 || count > kMaxFused)
+        return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d (got %d)", what, kMaxFused, count);
+    TL_REQUIRE(alphas && betas && neighbours, what, "null coefficients/neighbours");
+    TL_REQUIRE(u_in && u_out && u0 && p_in && p_out && r && kx && ky, what, "null field");
+    TL_REQUIRE(u_in != u_out && p_in != p_out, what, "input and output buffers must be distinct");
+    TL_REQUIRE(r != u_in && r != u_out && r != p_in && r != p_out && r != u0, what,
+               "r must be distinct from the other fields");
+    for (int f = 0; f < 4; ++f)
+        TL_REQUIRE(neighbours[f] == -1, what, "every face must be external (single chunk)");
+    TL_REQUIRE(all_aligned(u_in, u_out, u0, p_in, p_out, r, kx, ky), what,
+               "fields must be 16-byte aligned");
+    const PitchGrid g{x, hd, x - 2 * hd, y - 2 * hd};
+    Ring3<double> rg{};
+    rg.src[0] = u_in;  rg.dst[0] = u_out;
+    rg.src[1] = p_in;  rg.dst[1] = p_out;
+    rg.n = 2;
+    const long ring = ((long)(y - g.ny) * x + (long)(x - g.nx) * g.ny) * rg.n;
+    hipLaunchKernelGGL(k_copy_rings2d<double>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
+                       tl_stream(), g, y, rg, tl_gate());
+    TL_TRY(tl_check_launch(what));
+    switch (count) {
+#define TL_CHEBY_CASE(S)                                                                      \
+    case S:                                                                                   \
+        return launch_cheby<S>(g, alphas, betas, u_in, u_out, u0, p_in, p_out, r, kx, ky, what);
+    TL_CHEBY_CASE(1) TL_CHEBY_CASE(2) TL_CHEBY_CASE(3) TL_CHEBY_CASE(4) TL_CHEBY_CASE(5)
+    TL_CHEBY_CASE(6) TL_CHEBY_CASE(7) TL_CHEBY_CASE(8) TL_CHEBY_CASE(9) TL_CHEBY_CASE(10)
+#undef TL_CHEBY_CASE
+    default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", what, kMaxFused);
+    }
+}
+
 template <class T>
 int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
             double* pw, const T* p_in, T* p_out, const T* r, T* w, const T* kx, const T* ky)
@@ -765,4 +1078,14 @@ extern "C" int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int hd, int pitch
     return inner_iterations<float>("tlhip2d_ppcg_inner_iterations_f32", x, y, hd, pitch, count,
                                    alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
                                    r_in, r_out, u_in, u_out, kx, ky, alpha, p, w);
+}
+
+extern "C" int tlhip2d_cheby_iterations(int x, int y, int hd, int count, const double* alphas,
+                                        const double* betas, const int* neighbours,
+                                        const double* u_in, double* u_out, const double* u0,
+                                        const double* p_in, double* p_out, double* r,
+                                        const double* kx, const double* ky)
+{
+    return cheby_iterations(x, y, hd, count, alphas, betas, neighbours, u_in, u_out, u0, p_in,
+                            p_out, r, kx, ky);
 }
