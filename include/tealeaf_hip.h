@@ -208,8 +208,8 @@ int tlhip2d_cheby_iterations(int x, int y, int halo_depth, int count,
         double* p_out, double* r, const double* kx, const double* ky);
 /* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
  * the height whose block count fills ~0.88 of what the device holds at
- * once, from the kernel's occupancy and the CU count) and next-row prefetch
- * (0 off, 1 on, < 0: default on). */
+ * once, from the kernel's occupancy and the CU count) and prefetch distance
+ * in rows (0-3; < 0: default 1; the Chebyshev loop uses 0 or 1). */
 int tlhip_set_fused_tuning(int tile_rows, int prefetch);
 /* 2d/kernel_interface.h:64 run_copy_u */
 int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);

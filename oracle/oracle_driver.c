@@ -528,7 +528,12 @@ or_problem* or_problem_create(const tl_settings* s, const tl_state* states,
     P->cell_x = zalloc(P->x + 1); P->vertex_x = zalloc(P->x + 1);
     P->cell_y = zalloc(P->y + 1); P->vertex_y = zalloc(P->y + 1);
     P->cell_z = zalloc(P->z + 1); P->vertex_z = zalloc(P->z + 1);
-    const int mi = s->max_iters + 1;
+    /* The reference sizes these max_iters (2d/c_kernels/omp3/kernel_initialise.c:68)
+     * while PPCG writes ppcg_inner_steps Chebyshev coefficients into them
+     * (cheby_coef_driver(chunks, settings->ppcg_inner_steps)): a deck with
+     * more inner steps than max_iters overruns the reference's arrays.  Here
+     * they hold both. */
+    const int mi = (s->max_iters > s->ppcg_inner_steps ? s->max_iters : s->ppcg_inner_steps) + 1;
     P->cg_alphas = zalloc(mi); P->cg_betas = zalloc(mi);
     P->cheby_alphas = zalloc(mi); P->cheby_betas = zalloc(mi);
 

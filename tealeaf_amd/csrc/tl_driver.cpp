@@ -1234,7 +1234,11 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
     P->vertex_x = dalloc(P->x + 1, &rc);
     P->vertex_y = dalloc(P->y + 1, &rc);
     P->vertex_z = dalloc(P->z + 1, &rc);
-    const size_t mi = (size_t)s->max_iters + 1;
+    // The reference sizes these max_iters (2d/c_kernels/omp3/kernel_initialise.c:68)
+    // while PPCG writes ppcg_inner_steps Chebyshev coefficients into them; a
+    // deck with more inner steps than max_iters overruns the reference's
+    // arrays.  Here they hold both.
+    const size_t mi = (size_t)std::max(s->max_iters, s->ppcg_inner_steps) + 1;
     P->cg_alphas.assign(mi, 0.0);
     P->cg_betas.assign(mi, 0.0);
     P->cheby_alphas.assign(mi, 0.0);

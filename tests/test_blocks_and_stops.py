@@ -92,3 +92,15 @@ def test_oracle_relative_residual_stop(name, text, dims, blk):
     assert rep["n_true_residuals"] >= 1
     assert got <= tol * (1 + 1e-6), got
     assert abs(got - rep["rel_residual"]) <= 1e-6 * tol + 1e-3 * got
+
+
+def test_oracle_more_inner_steps_than_max_iters():
+    """A PPCG deck whose 10 inner steps exceed max_iters (5): the reference
+    sizes its coefficient arrays max_iters and would overrun them; the
+    oracle holds both and runs the deck to the max_iters cut, PPCG iterations
+    included (the switch comes after presteps = 1)."""
+    from tests.problem_helpers import oracle_run
+    res = oracle_run(tdeck.layered_2d(48, 2, solver=tdeck.PPCG, presteps=1, max_iters=5))
+    counts = [(s["printed_cg_iters"], s["printed_inner_iters"]) for s in res["steps"]]
+    assert all(cg + inner >= 5 for cg, inner in counts), counts
+    assert all(inner > 0 for _, inner in counts), counts

@@ -82,6 +82,14 @@ def test_max_iters_cutoff(hip, solver, max_iters):
     assert all(r["printed_cg_iters"] + r["printed_inner_iters"] >= max_iters - 1 for r in reps)
 
 
+@pytest.mark.parametrize("solver", [tdeck.CG, tdeck.CHEBY, tdeck.PPCG])
+@pytest.mark.parametrize("max_iters", [1, 2, 5])
+def test_tiny_max_iters(hip, solver, max_iters):
+    """Fewer iterations than the loop keeps in flight."""
+    reps = _same(hip, tdeck.layered_2d(48, 2, solver=solver, presteps=1, max_iters=max_iters))
+    assert all(r["printed_cg_iters"] + r["printed_inner_iters"] >= max_iters for r in reps)
+
+
 @pytest.mark.parametrize("solver", [tdeck.CG, tdeck.PPCG])
 def test_relative_residual_stop(hip, solver):
     """The relative-residual stop: the device flags a candidate, the host
