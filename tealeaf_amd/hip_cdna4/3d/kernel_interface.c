@@ -67,8 +67,12 @@ void run_kernel_initialise(Chunk* chunk, Settings* settings)
     chunk->vertex_z = dev_buffer(z + 1);
     chunk->cg_alphas = host_buffer(settings->max_iters);
     chunk->cg_betas = host_buffer(settings->max_iters);
-    chunk->cheby_alphas = host_buffer(settings->max_iters);
-    chunk->cheby_betas = host_buffer(settings->max_iters);
+    /* PPCG writes ppcg_inner_steps Chebyshev coefficients here; the omp3
+     * suite sizes these max_iters only (kernel_initialise.c:68). */
+    chunk->cheby_alphas = host_buffer(settings->max_iters > settings->ppcg_inner_steps
+                                          ? settings->max_iters : settings->ppcg_inner_steps);
+    chunk->cheby_betas = host_buffer(settings->max_iters > settings->ppcg_inner_steps
+                                         ? settings->max_iters : settings->ppcg_inner_steps);
     chunk->ext->u_alt = dev_buffer(n);
     chunk->ext->sd_alt = dev_buffer(n);
 }
