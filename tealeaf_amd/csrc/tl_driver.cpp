@@ -422,6 +422,8 @@ struct tlhip_problem {
     int run_resident(int t0, int sw, bool cg, Body&& body, double* rro, double* error, int* last,
                      tl::loop::Report* out)
     {
+        if (t0 < 0 || t0 >= s.max_iters)
+            return tl_fail(TLHIP_ERR_ARG, "resident loop: no iteration to run from %d", t0);
         constexpr int kSnap = tl::loop::kAhead + 2;
         Ptrs snap[kSnap];
         TRY(tl::loop::begin(*rro, s.max_iters));
@@ -477,8 +479,9 @@ struct tlhip_problem {
         for (int from = t0;;) {
             int last;
             tl::loop::Report rp;
-            TRY(run_resident(from, sw, true, [&](const tl::loop::StopRule& r) { return cg_body(r); },
-                             rro, error, &last, &rp));
+            TRY(run_resident(from, sw, true,
+                             [&](const tl::loop::StopRule& rule) { return cg_body(rule); }, rro,
+                             error, &last, &rp));
             rep->n_cg_steps += last - from + 1;
             *tt = last;
             TRY(confirm(rp, done));
@@ -491,8 +494,8 @@ struct tlhip_problem {
     {
         double rro;
         TRY(cg_init_driver(rx, ry, rz, &rro));
-        int tt;
-        if (resident_cg()) {
+        int tt = 0;
+        if (resident_cg() && s.max_iters > 0) {
             bool done;
             TRY(cg_resident(0, 0, &rro, error, &tt, &done));
             if (!done) tt = s.max_iters;
@@ -762,7 +765,7 @@ struct tlhip_problem {
                     int last;
                     tl::loop::Report rp;
                     TRY(run_resident(tt, 0, false,
-                                     [&](const tl::loop::StopRule& r) { return ppcg_body(r); },
+                                     [&](const tl::loop::StopRule& rule) { return ppcg_body(rule); },
                                      &rro, error, &last, &rp));
                     const int ran = last - tt + 1;
                     rep->n_ppcg_outer += ran;
