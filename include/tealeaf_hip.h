@@ -365,15 +365,17 @@ typedef struct tlhip_problem tlhip_problem;
  * Results are identical either way.  Default off: at 4096^2 on MI355X the
  * fused pass (48 B/cell at 4.4 TB/s) is no faster than the two. */
 int tlhip_set_cg_fusion(int on);
-/* Whether the driver runs CG iterations (2D, 3D) and fused PPCG outer
- * iterations (2D) device-resident: alpha = rro/pw, beta = rrn/rro and the
- * stop and switch tests of cg_driver.c / cheby_driver.c / ppcg_driver.c are
- * evaluated on the device, the host keeps several iterations enqueued and
- * reads one report per iteration, and kernels of iterations past the stop
- * return at once.  Results are identical either way (same operations in the
- * same order); on = no host round trip between the kernels of an iteration.
- * Default on.  No counterpart in the reference: its drivers take pw and rrn
- * to the host (2d/drivers/cg_driver.c:78-99). */
+/* Whether the driver runs CG and Jacobi iterations (2D, 3D) and fused PPCG
+ * outer iterations (2D) device-resident: alpha = rro/pw, beta = rrn/rro, the
+ * Jacobi error and the stop and switch tests of cg_driver.c /
+ * cheby_driver.c / ppcg_driver.c / jacobi_driver.c are evaluated on the
+ * device, the host keeps several iterations enqueued and reads one report
+ * per iteration, and kernels of iterations past the stop return at once.
+ * Jacobi then sweeps between u and a partner buffer instead of copying u
+ * into r every iteration.  Results are identical either way (same values,
+ * same operations in the same order); on = no host round trip between the
+ * kernels of an iteration.  Default on.  No counterpart in the reference:
+ * its drivers take pw and rrn to the host (2d/drivers/cg_driver.c:78-99). */
 int tlhip_set_resident(int on);
 /* Steps per fused PPCG launch used by the driver (1..10; 0 or 1 = one step
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 10: the

@@ -10,8 +10,8 @@
 // Chebyshev coefficient arrays persist across timesteps, and alpha/beta and
 // the eigenvalue estimate are computed on the host in FP64.
 //
-// Every field lives on the device.  The CG and PPCG iterations run
-// device-resident (tl_loop.h): alpha, beta and the stopping rule are
+// Every field lives on the device.  The CG, PPCG and Jacobi iterations run
+// device-resident (tl_loop.h): alpha, beta, the Jacobi error and the stopping rule are
 // evaluated on the device, the host enqueues iterations ahead and reads one
 // small report per iteration, and the results are those of the host-driven
 // loop bit for bit (tlhip_set_resident(0) selects that loop).
@@ -407,6 +407,27 @@ struct tlhip_problem {
         return halo_update(1);
     }
 
+    // Iteration j of jacobi_driver.c's loop (jacobi_driver.c:22-40,
+    // 3d/drivers/jacobi_solver_driver.c:15-23): the sweep reads u and writes
+    // u_alt, and the two swap (the reference copies u into r and sweeps from
+    // the copy into u: the same values, one copy fewer); every 50th
+    // iteration the 2-norm of the residual is added to the sweep's error.
+    int jacobi_body(const tl::loop::StopRule& rule, int j)
+    {
+        const bool res = j % 50 == 0;
+        const tl::loop::ErrMode mode = res ? tl::loop::kErrSet : tl::loop::kErrSetTest;
+        if (dims == 2) TRY(tl::loop2d::jacobi(x, y, hd, kx, ky, u0, u, u_alt, mode, rule));
+        else TRY(tl::loop3d::jacobi(x, y, z, hd, kx, ky, kz, u0, u, u_alt, mode, rule));
+        std::swap(u, u_alt);
+        if (res) {
+            TRY(halo_update(1));
+            TRY(residual());
+            if (dims == 2) TRY(tl::loop2d::norm_err(x, y, hd, r, rule));
+            else TRY(tl::loop3d::norm_err(x, y, z, hd, r, rule));
+        }
+        return halo_update(1);
+    }
+
     struct Ptrs {
         double *u, *u_alt, *r, *r_alt, *sd, *sd_alt;
     };
@@ -436,7 +457,7 @@ struct tlhip_problem {
         for (;;) {
             while (next < s.max_iters && next <= chk + tl::loop::kAhead) {
                 tl::loop::set_iter(next);
-                TRY(body(rule));
+                TRY(body(rule, next));
                 snap[next % kSnap] = Ptrs{u, u_alt, r, r_alt, sd, sd_alt};
                 ++next;
             }
@@ -480,7 +501,7 @@ struct tlhip_problem {
             int last;
             tl::loop::Report rp;
             TRY(run_resident(from, sw, true,
-                             [&](const tl::loop::StopRule& rule) { return cg_body(rule); }, rro,
+                             [&](const tl::loop::StopRule& rule, int) { return cg_body(rule); }, rro,
                              error, &last, &rp));
             rep->n_cg_steps += last - from + 1;
             *tt = last;
@@ -765,7 +786,9 @@ struct tlhip_problem {
                     int last;
                     tl::loop::Report rp;
                     TRY(run_resident(tt, 0, false,
-                                     [&](const tl::loop::StopRule& rule) { return ppcg_body(rule); },
+                                     [&](const tl::loop::StopRule& rule, int) {
+                                         return ppcg_body(rule);
+                                     },
                                      &rro, error, &last, &rp));
                     const int ran = last - tt + 1;
                     rep->n_ppcg_outer += ran;
@@ -960,6 +983,7 @@ struct tlhip_problem {
         TRY(start_solve());
         reset_fields();
         fields[F_U] = true;
+        if (g_resident && interior()) return jacobi_resident(error);
         int tt;
         for (tt = 0; tt < s.max_iters; ++tt) {
             if (dims == 2) TRY(tlhip2d_jacobi_iterate(x, y, hd, error, kx, ky, u0, u, r));
@@ -976,6 +1000,37 @@ struct tlhip_problem {
             if (done) break;
         }
         rep->printed_cg_iters = tt;
+        return 0;
+    }
+
+    // The loop of jacobi_driver with jacobi_body's iterations enqueued ahead
+    // of the host.  u_alt starts as a copy of u: sweeps write the interior
+    // and halo updates depth 1 only, so the deeper halo of both buffers stays
+    // the same and u ends as the host-driven loop leaves it.  r is then made
+    // what the host-driven loop leaves: the last sweep's input, with the
+    // residual in its interior when that iteration computed one.
+    int jacobi_resident(double* error)
+    {
+        const size_t bytes = sizeof(double) * (size_t)n;
+        TRY(tlhip_copy_device(u_alt, u, bytes));
+        int last = 0;
+        bool done = false;
+        for (int from = 0;;) {
+            double rro = 0.0;
+            tl::loop::Report rp;
+            TRY(run_resident(from, 0, false,
+                             [&](const tl::loop::StopRule& rule, int j) {
+                                 return jacobi_body(rule, j);
+                             },
+                             &rro, error, &last, &rp));
+            rep->n_jacobi += last - from + 1;
+            TRY(confirm(rp, &done));
+            if (done || last + 1 >= s.max_iters) break;
+            from = last + 1;    // a relative-residual candidate the true residual refused
+        }
+        TRY(tlhip_copy_device(r, u_alt, bytes));
+        if (last % 50 == 0) TRY(residual());
+        rep->printed_cg_iters = done ? last : s.max_iters;
         return 0;
     }
 

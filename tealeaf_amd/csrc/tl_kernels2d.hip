@@ -868,5 +868,31 @@ int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule)
     return loop::finish_beta(nb, rule, what);
 }
 
+int jacobi(int x, int y, int hd, const double* kx, const double* ky, const double* u0,
+           const double* u_in, double* u_out, loop::ErrMode mode, const loop::StopRule& rule)
+{
+    const char* what = "tlhip2d_jacobi_iterate";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(kx && ky && u0 && u_in && u_out && u_in != u_out, what, "bad arguments");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    OpJacobi2 op{u0, u_out};
+    unsigned nb = 0;
+    TL_TRY(launch_stencil(g, pick_v(g, kx, ky, u0, u_in, u_out), u_in, kx, ky, op, what, &nb));
+    return loop::finish_err(nb, mode, rule, what);
+}
+
+int norm_err(int x, int y, int hd, const double* buf, const loop::StopRule& rule)
+{
+    const char* what = "tlhip2d_calculate_2norm";
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(buf, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_interior2(g, pick_v(g, buf), OpNorm{buf}, what, &nb));
+    return loop::finish_err(nb, loop::kErrAddTest, rule, what);
+}
+
 }  // namespace loop2d
 }  // namespace tl

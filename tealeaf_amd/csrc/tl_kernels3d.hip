@@ -924,5 +924,33 @@ int calc_p(int x, int y, int z, int hd, double* p, const double* r)
     return launch_interior3(g, pick_v(g, p, r), op, what);
 }
 
+int jacobi(int x, int y, int z, int hd, const double* kx, const double* ky, const double* kz,
+           const double* u0, const double* u_in, double* u_out, loop::ErrMode mode,
+           const loop::StopRule& rule)
+{
+    const char* what = "tlhip3d_jacobi_solver_iterate";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(kx && ky && kz && u0 && u_in && u_out && u_in != u_out, what, "bad arguments");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    OpJacobi3 op{u0, u_out};
+    unsigned nb = 0;
+    TL_TRY(launch_stencil(g, pick_v(g, kx, ky, kz, u0, u_in, u_out), u_in, kx, ky, kz, op, what,
+                          &nb));
+    return loop::finish_err(nb, mode, rule, what);
+}
+
+int norm_err(int x, int y, int z, int hd, const double* buf, const loop::StopRule& rule)
+{
+    const char* what = "tlhip3d_calculate_2norm";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(buf, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_interior3(g, pick_v(g, buf), OpNorm{buf}, what, &nb));
+    return loop::finish_err(nb, loop::kErrAddTest, rule, what);
+}
+
 }  // namespace loop3d
 }  // namespace tl

@@ -1,5 +1,6 @@
-// Device-resident solver loops: the host enqueues CG / PPCG iterations
-// ahead of the device and reads each iteration's outcome afterwards.
+// Device-resident solver loops: the host enqueues CG / PPCG / Jacobi
+// iterations ahead of the device and reads each iteration's outcome
+// afterwards.
 //
 // The reference's drivers (2d/drivers/cg_driver.c, ppcg_driver.c) return
 // two scalars to the host every iteration (pw, then rrn) and form
@@ -13,7 +14,8 @@
 // The host meanwhile keeps kAhead iterations in flight and reads each
 // iteration's report from a pinned ring.  Values, iteration counts and the
 // order of every floating-point operation are those of the host-driven
-// loop.
+// loop.  Jacobi (jacobi_driver.c) has one reduction, the error, whose
+// finisher applies the same rule.
 #pragma once
 
 #include "tl_common.h"
@@ -56,6 +58,12 @@ int finish_alpha(unsigned nblocks, const char* what);
 // Finisher of an rrn reduction: beta = rrn / rro, rro = rrn, the stop rule,
 // and the iteration's report.
 int finish_beta(unsigned nblocks, const StopRule& rule, const char* what);
+// Finisher of the Jacobi error (jacobi_driver.c:57-83): the sum of the
+// partials is assigned to the error (kErrSet: no test yet, the 2-norm of the
+// residual follows) or assigned / added to it, and then the stop rule and
+// the iteration's report as in finish_beta.
+enum ErrMode : int { kErrSet = 0, kErrSetTest = 1, kErrAddTest = 2 };
+int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char* what);
 // The report of iteration j, once the device has published it.
 int wait(int j, Report* out);
 // alpha and beta of iterations j0 .. j0+n-1 into host arrays.
@@ -74,6 +82,12 @@ int calc_ur(int x, int y, int hd, double* u, const double* p, double* r, const d
             const loop::StopRule& rule);
 int calc_p(int x, int y, int hd, double* p, const double* r);
 int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
+// One Jacobi sweep from u_in into u_out (run_jacobi_iterate without the copy
+// of u into r: the sweep reads u_in where the reference reads that copy).
+int jacobi(int x, int y, int hd, const double* kx, const double* ky, const double* u0,
+           const double* u_in, double* u_out, loop::ErrMode mode, const loop::StopRule& rule);
+// The 2-norm of buf added to the Jacobi error.
+int norm_err(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
 int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const double* betas,
                double theta, int init_from_r, const int* neighbours, const double* sd_in,
                double* sd_out, const double* r_in, double* r_out, const double* u_in,
@@ -87,6 +101,10 @@ int calc_w(int x, int y, int z, int hd, const double* p, double* w, const double
 int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, const double* w,
             const loop::StopRule& rule);
 int calc_p(int x, int y, int z, int hd, double* p, const double* r);
+int jacobi(int x, int y, int z, int hd, const double* kx, const double* ky, const double* kz,
+           const double* u0, const double* u_in, double* u_out, loop::ErrMode mode,
+           const loop::StopRule& rule);
+int norm_err(int x, int y, int z, int hd, const double* buf, const loop::StopRule& rule);
 }  // namespace loop3d
 
 }  // namespace tl

@@ -17,6 +17,7 @@ namespace {
 
 struct State {
     double rro, alpha, beta;
+    double err;          // the Jacobi error of the current iteration
     int last;            // last iteration to run; INT_MAX while the loop goes on
     int pad;
 };
@@ -52,6 +53,21 @@ __global__ void __launch_bounds__(64) k_loop_begin(State* st, double rro)
     }
 }
 
+// The stop rule and the report of iteration j (thread 0 of a finisher).
+__device__ void test_and_report(State* st, Slot* ring, int j, double rrn, const StopRule& rule)
+{
+    const double err = rule.sqrt_err ? sqrt(rrn) : rrn;
+    int flags = 0;
+    if (rule.rel ? rrn <= rule.rel_target : fabs(err) < rule.eps) flags |= kConverged;
+    if (rule.sw == 1 && err < rule.eps_lim && j + 1 > 20) flags |= kSwitch;
+    if (rule.sw == 2 && j + 1 > rule.presteps && err < 1.0) flags |= kSwitch;
+    if (flags) st->last = j;
+    Slot* s = ring + (j % kRing);
+    __hip_atomic_store(&s->rrn, rrn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&s->err, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&s->flags, (double)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // alpha = rro / pw of iteration j (cg_driver.c:88-90, ppcg_driver.c:102-104)
 __global__ void __launch_bounds__(256)
 k_loop_alpha(unsigned nblocks, const double* __restrict__ partials, State* st, double* alog,
@@ -83,16 +99,22 @@ k_loop_beta(unsigned nblocks, const double* __restrict__ partials, State* st, do
     st->beta = b;
     blog[j] = b;
     st->rro = rrn;
-    const double err = rule.sqrt_err ? sqrt(rrn) : rrn;
-    int flags = 0;
-    if (rule.rel ? rrn <= rule.rel_target : fabs(err) < rule.eps) flags |= kConverged;
-    if (rule.sw == 1 && err < rule.eps_lim && j + 1 > 20) flags |= kSwitch;
-    if (rule.sw == 2 && j + 1 > rule.presteps && err < 1.0) flags |= kSwitch;
-    if (flags) st->last = j;
-    Slot* s = ring + (j % kRing);
-    __hip_atomic_store(&s->rrn, rrn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&s->err, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&s->flags, (double)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    test_and_report(st, ring, j, rrn, rule);
+}
+
+// The Jacobi error of iteration j: assigned (jacobi.c:99) or, for the
+// residual norm of every 50th iteration, added (calculate_2norm's +=).
+__global__ void __launch_bounds__(256)
+k_loop_err(unsigned nblocks, const double* __restrict__ partials, State* st, Slot* ring, int j,
+           int mode, StopRule rule)
+{
+    if (j > st->last) return;
+    double tot[1];
+    tl_sum_partials<1>(nblocks, partials, tot);
+    if (threadIdx.x != 0) return;
+    const double e = mode == kErrAddTest ? st->err + tot[0] : tot[0];
+    st->err = e;
+    if (mode != kErrSet) test_and_report(st, ring, j, e, rule);
 }
 
 volatile unsigned long long* slot_bits(int j)
@@ -184,6 +206,25 @@ int finish_beta(unsigned nblocks, const StopRule& rule, const char* what)
     std::atomic_thread_fence(std::memory_order_seq_cst);
     hipLaunchKernelGGL(k_loop_beta, dim3(1), dim3(256), 0, tl_stream(), nblocks,
                        tl_reduce_slots().partials, h.st, h.blog, h.ring, h.iter, rule);
+    return tl_check_launch(what);
+}
+
+int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char* what)
+{
+    Host& h = g_loop;
+    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
+    if (h.iter < 0 || h.iter >= h.cap)
+        return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
+    if (nblocks > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
+                       TL_MAX_RED_BLOCKS);
+    if (mode != kErrSet) {
+        volatile unsigned long long* b = slot_bits(h.iter);
+        b[0] = b[1] = b[2] = TL_MAIL_EMPTY;
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+    }
+    hipLaunchKernelGGL(k_loop_err, dim3(1), dim3(256), 0, tl_stream(), nblocks,
+                       tl_reduce_slots().partials, h.st, h.ring, h.iter, (int)mode, rule);
     return tl_check_launch(what);
 }
 

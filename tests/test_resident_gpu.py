@@ -1,4 +1,4 @@
-"""The device-resident CG / PPCG loops against the host-driven loop.
+"""The device-resident CG / PPCG / Jacobi loops against the host-driven loop.
 
 With tlhip_set_resident(1) (the default) the driver forms alpha and beta on
 the device, evaluates the stop and switch tests there and keeps several
@@ -6,7 +6,9 @@ iterations enqueued ahead of the host; with 0 it takes pw and rrn to the
 host every iteration, as the reference's drivers do (cg_driver.c:78-99,
 ppcg_driver.c:95-131).  Both run the same kernels with the same arguments in
 the same order, so every report, every field and the summary must be the
-same bits.  The cases cover each place the loop can end: convergence, the
+same bits.  The resident Jacobi loop sweeps between u and a partner buffer
+instead of copying u into r first (jacobi_driver.c:51-55): the same values,
+so there too every bit must agree, r included.  The cases cover each place the loop can end: convergence, the
 CG -> Chebyshev / PPCG switch by presteps and by the error switch, max_iters
 (in the CG phase and in the PPCG phase), the relative-residual stop whose
 true-residual check can refuse a candidate, PPCG inner-step counts whose
@@ -24,7 +26,7 @@ from tealeaf_amd import deck as tdeck
 pytestmark = pytest.mark.gpu
 
 
-def _run(hip, text, resident, dims=None, inject=None, rel=0.0):
+def _run(hip, text, resident, dims=None, inject=None, rel=0.0, extra=()):
     from tealeaf_amd.problem import Problem
     hip.tlhip_set_resident(resident)
     try:
@@ -34,20 +36,21 @@ def _run(hip, text, resident, dims=None, inject=None, rel=0.0):
             if rel > 0.0:
                 p.set_solve(0, rel)
             reps = p.run()
-            return reps, p.get("u"), p.get("energy"), p.field_summary()
+            fields = [p.get(f) for f in ("u", "energy") + tuple(extra)]
+            return reps, fields, p.field_summary()
     finally:
         hip.tlhip_set_resident(1)
 
 
-def _same(hip, text, dims=None, inject=None, rel=0.0):
-    a = _run(hip, text, 0, dims, inject, rel)
-    b = _run(hip, text, 1, dims, inject, rel)
+def _same(hip, text, dims=None, inject=None, rel=0.0, extra=()):
+    a = _run(hip, text, 0, dims, inject, rel, extra)
+    b = _run(hip, text, 1, dims, inject, rel, extra)
     assert len(a[0]) == len(b[0])
     for ra, rb in zip(a[0], b[0]):
         assert ra == rb, (ra, rb)
-    np.testing.assert_array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[2], b[2])
-    assert a[3] == b[3]
+    for fa, fb in zip(a[1], b[1]):
+        np.testing.assert_array_equal(fa, fb)
+    assert a[2] == b[2]
     return b[0]
 
 
@@ -121,6 +124,44 @@ def test_2d_blocks(hip):
                                  halo_depth=s.halo_depth)
     reps = _same(hip, text, dims=2, inject=inject)
     assert reps[0]["n_ppcg_outer"] > 0
+
+
+# Jacobi: at 40^2 (eps 1e-8) the oracle stops after 49 and 48 iterations;
+# at 12^3 (eps 1e-8, the 3D test on sqrt(error)) after 171 and 167.
+@pytest.mark.parametrize("dims,text", [
+    (2, tdeck.layered_2d(40, 2, solver=tdeck.JACOBI, eps=1e-8, max_iters=2000)),
+    (2, tdeck.layered_2d(32, 2, solver=tdeck.JACOBI, eps=1e-10, max_iters=4000)),
+    (3, tdeck.cuboid_3d(12, 2, solver=tdeck.JACOBI, eps=1e-8, max_iters=2000)),
+], ids=["2d-40", "2d-32", "3d-12"])
+def test_jacobi(hip, dims, text):
+    reps = _same(hip, text, dims=dims, extra=("r", "u0"))
+    assert all(r["printed_cg_iters"] >= 40 for r in reps)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("max_iters", [1, 2, 5, 50, 51, 52, 101])
+def test_jacobi_max_iters(hip, dims, max_iters):
+    """Runs cut by max_iters: inside the first iterations in flight, just
+    before, at and after an iteration that adds the residual's norm (every
+    50th), whose r the loop must leave as the host-driven loop does."""
+    text = (tdeck.layered_2d(64, 2, solver=tdeck.JACOBI, max_iters=max_iters) if dims == 2 else
+            tdeck.cuboid_3d(12, 2, solver=tdeck.JACOBI, max_iters=max_iters))
+    reps = _same(hip, text, dims=dims, extra=("r",))
+    assert all(r["n_jacobi"] == max_iters and r["printed_cg_iters"] == max_iters for r in reps)
+
+
+def test_jacobi_relative_stop(hip):
+    reps = _same(hip, tdeck.layered_2d(40, 2, solver=tdeck.JACOBI, max_iters=3000), rel=1e-4,
+                 extra=("r",))
+    assert all(r["n_true_residuals"] >= 1 for r in reps)
+
+
+def test_jacobi_blocks(hip):
+    text = tdeck.layered_2d(64, 1, solver=tdeck.JACOBI, eps=1e-9, max_iters=3000)
+    s, _ = tdeck.parse_deck(text, 2)
+    inject = blocks.block_fields((64, 64), 16, seed=4, rho=(0.1, 10.0), energy=(0.1, 25.0),
+                                 halo_depth=s.halo_depth)
+    _same(hip, text, dims=2, inject=inject, extra=("r",))
 
 
 def test_set_resident_returns_zero(hip):
