@@ -423,6 +423,28 @@ int tlhip_problem_create(const tlhip_settings* settings, const tlhip_state* stat
  * redo u = e*rho, the halo prime and store_energy. */
 int tlhip_problem_inject(tlhip_problem* p, const double* density,
         const double* energy0);
+/* Device fields of a chunk that the caller allocated and initialised (the
+ * hip_cdna4 suite's run_kernel_initialise, run_set_chunk_data and
+ * run_set_chunk_state, then initialise.c's halo prime and store_energy).
+ * kz and z_area are used in 3D only.  u_alt and sd_alt are the partners of u
+ * and sd (the suite's chunk extension). */
+typedef struct {
+    double *density0, *density, *energy0, *energy, *u, *u0, *p, *r, *mi, *w;
+    double *kx, *ky, *kz, *sd, *u_alt, *sd_alt;
+    double *volume, *x_area, *y_area, *z_area;
+} tlhip_chunk_fields;
+/* A problem over an existing single chunk whose faces are all external, for
+ * the suite's diffuse_overload (the reference's DIFFUSE_OVERLOAD hook,
+ * 2d/main.c:31-36, 3d/main.c:30-35), which runs the reference's timestep
+ * loop with the native driver (device-resident loops, fused sweeps) in place
+ * of 2d/drivers/.  No counterpart in the reference's kernel interface.  The
+ * fields stay the caller's: tlhip_problem_destroy does not free them, and
+ * after each tlhip_problem_step (and tlhip_problem_adjoint) u, sd, r and p
+ * hold their values at the caller's pointers again (u_alt and sd_alt are
+ * scratch).  The problem allocates the partners of r and p that the 2D
+ * driver needs. */
+int tlhip_problem_adopt(const tlhip_settings* settings, const tlhip_chunk_fields* fields,
+        tlhip_problem** out);
 /* One timestep (tt 0-based), as 2d/diffuse.c:22-80. */
 int tlhip_problem_step(tlhip_problem* p, int tt, tlhip_step_report* report);
 /* vol, mass, ie, temp of the current state (2d/drivers/field_summary_driver.c). */

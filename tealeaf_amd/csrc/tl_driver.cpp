@@ -125,6 +125,10 @@ struct tlhip_problem {
     int neighbours[6] = {-1, -1, -1, -1, -1, -1};
     bool fields[6] = {};
     tlhip_step_report* rep = nullptr;
+    // A problem over a chunk the caller allocated (tlhip_problem_adopt): the
+    // caller's buffers of u, sd, r and p, which the partners swap with.
+    bool adopted = false;
+    double* home[4] = {};
 
     // How the solve runs (tlhip_problem_set_solve).
     int precision = 0;       // 0: FP64; 1: FP32 sweeps + FP64 residual replacement
@@ -1181,15 +1185,38 @@ struct tlhip_problem {
         return (id >= 0 && id < 12) ? f[id] : nullptr;
     }
 
+    // An adopted chunk's u, sd, r and p back at the caller's pointers: a
+    // field whose partner holds it is copied home and the pair swaps back.
+    int settle()
+    {
+        if (!adopted) return 0;
+        double** pairs[4][2] = {{&u, &u_alt}, {&sd, &sd_alt}, {&r, &r_alt}, {&p, &p_alt}};
+        for (int i = 0; i < 4; ++i) {
+            double*& a = *pairs[i][0];
+            double*& b = *pairs[i][1];
+            if (a == home[i]) continue;
+            if (b != home[i])
+                return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_step: a field left its buffer pair");
+            TRY(tlhip_copy_device(home[i], a, sizeof(double) * (size_t)n));
+            std::swap(a, b);
+        }
+        return 0;
+    }
+
     void release()
     {
+        float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b, p32b};
+        for (float* a : all32)
+            if (a) (void)tlhip_free(a);
+        if (adopted) {      // the caller's fields stay; only the partners of r and p are ours
+            for (double* a : {r == home[2] ? r_alt : r, p == home[3] ? p_alt : p})
+                if (a) (void)tlhip_free(a);
+            return;
+        }
         double* all[] = {density0, density, energy0, energy, u, u0, p, r, mi, w, kx, ky, kz,
                          sd, u_alt, sd_alt, r_alt, p_alt, volume, x_area, y_area, z_area, cell_x,
                          cell_y, cell_z, vertex_x, vertex_y, vertex_z};
         for (double* a : all)
-            if (a) (void)tlhip_free(a);
-        float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b, p32b};
-        for (float* a : all32)
             if (a) (void)tlhip_free(a);
     }
 };
@@ -1247,21 +1274,23 @@ extern "C" int tlhip_problem_adjoint(tlhip_problem* P, int steps, const double* 
         return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_adjoint: null argument");
     *loss = 0.0;
     TRY(P->adjoint(steps, u_target, loss, grad_log_density, reports));
+    TRY(P->settle());
     return tlhip_synchronize();
 }
 
-extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* states,
-                                    int num_states, tlhip_problem** out)
+namespace {
+int check_settings(const tlhip_settings* s, const char* what)
 {
-    if (!s || !states || !out || num_states < 1)
-        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: invalid arguments");
-    if (s->dims != 2 && s->dims != 3)
-        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: dims must be 2 or 3");
+    if (s->dims != 2 && s->dims != 3) return tl_fail(TLHIP_ERR_ARG, "%s: dims must be 2 or 3", what);
     if (s->x_cells < 1 || s->y_cells < 1 || (s->dims == 3 && s->z_cells < 1) || s->halo_depth < 1 ||
         s->max_iters < 1)
-        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: invalid sizes");
-    *out = nullptr;
-    TRY(tlhip_init(-1));
+        return tl_fail(TLHIP_ERR_ARG, "%s: invalid sizes", what);
+    return 0;
+}
+
+// Sizes as 2d/chunk.c:7-8 and parse_config.c:188-191 form them.
+tlhip_problem* new_problem(const tlhip_settings* s)
+{
     auto* P = new tlhip_problem();
     P->s = *s;
     P->dims = s->dims;
@@ -1273,25 +1302,6 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
     P->dx = (s->xmax - s->xmin) / (double)s->x_cells;
     P->dy = (s->ymax - s->ymin) / (double)s->y_cells;
     P->dz = s->dims == 3 ? (s->zmax - s->zmin) / (double)s->z_cells : 0.0;
-    int rc = 0;
-    const long n = P->n;
-    double** big[] = {&P->density0, &P->density, &P->energy0, &P->energy, &P->u, &P->u0, &P->p,
-                      &P->r, &P->mi, &P->w, &P->kx, &P->ky, &P->sd, &P->u_alt, &P->sd_alt,
-                      &P->volume, &P->x_area, &P->y_area};
-    for (double** b : big) *b = dalloc(n, &rc);
-    if (P->dims == 3) {
-        P->kz = dalloc(n, &rc);
-        P->z_area = dalloc(n, &rc);
-    } else {
-        P->p_alt = dalloc(n, &rc);      // partner of p: fused calc_p + calc_w, fused Chebyshev
-        if (s->solver == 3) P->r_alt = dalloc(n, &rc);   // partner of r for the fused PPCG loop
-    }
-    P->cell_x = dalloc(P->x + 1, &rc);
-    P->cell_y = dalloc(P->y + 1, &rc);
-    P->cell_z = dalloc(P->z + 1, &rc);
-    P->vertex_x = dalloc(P->x + 1, &rc);
-    P->vertex_y = dalloc(P->y + 1, &rc);
-    P->vertex_z = dalloc(P->z + 1, &rc);
     // The reference sizes these max_iters (2d/c_kernels/omp3/kernel_initialise.c:68)
     // while PPCG writes ppcg_inner_steps Chebyshev coefficients into them; a
     // deck with more inner steps than max_iters overruns the reference's
@@ -1301,6 +1311,44 @@ extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* 
     P->cg_betas.assign(mi, 0.0);
     P->cheby_alphas.assign(mi, 0.0);
     P->cheby_betas.assign(mi, 0.0);
+    return P;
+}
+
+// Partners the 2D driver's fused paths write into.
+void alloc_partners(tlhip_problem* P, int* rc)
+{
+    if (P->dims != 2) return;
+    P->p_alt = dalloc(P->n, rc);      // partner of p: fused calc_p + calc_w, fused Chebyshev
+    if (P->s.solver == 3) P->r_alt = dalloc(P->n, rc);   // partner of r for the fused PPCG loop
+}
+}  // namespace
+
+extern "C" int tlhip_problem_create(const tlhip_settings* s, const tlhip_state* states,
+                                    int num_states, tlhip_problem** out)
+{
+    if (!s || !states || !out || num_states < 1)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_create: invalid arguments");
+    TRY(check_settings(s, "tlhip_problem_create"));
+    *out = nullptr;
+    TRY(tlhip_init(-1));
+    auto* P = new_problem(s);
+    int rc = 0;
+    const long n = P->n;
+    double** big[] = {&P->density0, &P->density, &P->energy0, &P->energy, &P->u, &P->u0, &P->p,
+                      &P->r, &P->mi, &P->w, &P->kx, &P->ky, &P->sd, &P->u_alt, &P->sd_alt,
+                      &P->volume, &P->x_area, &P->y_area};
+    for (double** b : big) *b = dalloc(n, &rc);
+    if (P->dims == 3) {
+        P->kz = dalloc(n, &rc);
+        P->z_area = dalloc(n, &rc);
+    }
+    alloc_partners(P, &rc);
+    P->cell_x = dalloc(P->x + 1, &rc);
+    P->cell_y = dalloc(P->y + 1, &rc);
+    P->cell_z = dalloc(P->z + 1, &rc);
+    P->vertex_x = dalloc(P->x + 1, &rc);
+    P->vertex_y = dalloc(P->y + 1, &rc);
+    P->vertex_z = dalloc(P->z + 1, &rc);
     if (!rc) {
         if (P->dims == 2) {
             rc = tlhip2d_set_chunk_data(P->x, P->y, P->hd, 0, 0, s->xmin, s->ymin, P->dx, P->dy,
@@ -1361,10 +1409,45 @@ extern "C" int tlhip_problem_inject(tlhip_problem* P, const double* density, con
     return tlhip_synchronize();
 }
 
+extern "C" int tlhip_problem_adopt(const tlhip_settings* s, const tlhip_chunk_fields* f,
+                                   tlhip_problem** out)
+{
+    const char* what = "tlhip_problem_adopt";
+    if (!s || !f || !out) return tl_fail(TLHIP_ERR_ARG, "%s: null argument", what);
+    TRY(check_settings(s, what));
+    const bool d3 = s->dims == 3;
+    const double* need[] = {f->density0, f->density, f->energy0, f->energy, f->u, f->u0, f->p,
+                            f->r, f->mi, f->w, f->kx, f->ky, f->sd, f->u_alt, f->sd_alt,
+                            f->volume, f->x_area, f->y_area, d3 ? f->kz : f->kx,
+                            d3 ? f->z_area : f->x_area};
+    for (const double* a : need)
+        if (!a) return tl_fail(TLHIP_ERR_ARG, "%s: a field of the chunk is NULL", what);
+    *out = nullptr;
+    TRY(tlhip_init(-1));
+    auto* P = new_problem(s);
+    P->adopted = true;
+    P->density0 = f->density0, P->density = f->density, P->energy0 = f->energy0;
+    P->energy = f->energy, P->u = f->u, P->u0 = f->u0, P->p = f->p, P->r = f->r, P->mi = f->mi;
+    P->w = f->w, P->kx = f->kx, P->ky = f->ky, P->kz = d3 ? f->kz : nullptr, P->sd = f->sd;
+    P->u_alt = f->u_alt, P->sd_alt = f->sd_alt, P->volume = f->volume;
+    P->x_area = f->x_area, P->y_area = f->y_area, P->z_area = d3 ? f->z_area : nullptr;
+    P->home[0] = P->u, P->home[1] = P->sd, P->home[2] = P->r, P->home[3] = P->p;
+    int rc = 0;
+    alloc_partners(P, &rc);
+    if (rc) {
+        P->release();
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return 0;
+}
+
 extern "C" int tlhip_problem_step(tlhip_problem* P, int tt, tlhip_step_report* report)
 {
     if (!P || !report) return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_step: null argument");
     TRY(P->step(tt, report));
+    TRY(P->settle());
     return tlhip_synchronize();
 }
 

@@ -7,7 +7,12 @@
   hip_cdna4 suite (oracle/_ref/tealeaf{2d,3d}_hip) against the same host
   with the reference omp3 suite (oracle/_ref/tealeaf{2d,3d}_omp3), and the
   reference's stored known answers (tea.problems).
+* The same host built with -DDIFFUSE_OVERLOAD over the suite's
+  diffuse_overload (oracle/_ref/tealeaf{2d,3d}_hip_overload), whose
+  timestep loop is the native driver's.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -83,6 +88,49 @@ DROPIN = [
     ("3d", tdeck.cuboid_3d(25, 3)),
     ("3d", tdeck.cuboid_3d(20, 2, solver=tdeck.PPCG, inner=8, coefficient=2)),
 ]
+
+
+OVERLOAD = DROPIN + [
+    ("2d", tdeck.layered_2d(40, 2, solver=tdeck.JACOBI, eps=1e-8, max_iters=400)),
+    ("2d", tdeck.layered_2d(64, 3, solver=tdeck.PPCG, presteps=20, inner=12, max_iters=2000)),
+    # sqrt(error) < 1e-6: at 1e-8 the 3D Jacobi error is a few ulps of u,
+    # and the count follows rounding (the omp3 host built with FMA
+    # contraction stops after 167 and 176 iterations, without after 171, 167)
+    ("3d", tdeck.cuboid_3d(12, 2, solver=tdeck.JACOBI, eps=1e-6, max_iters=300)),
+    ("3d", tdeck.cuboid_3d(20, 2, solver=tdeck.CHEBY)),
+]
+_ACTUAL = re.compile(r"^Actual\s+(\S+)", re.M)
+
+
+@pytest.mark.parametrize("dim,text", OVERLOAD,
+                         ids=[f"{d}-{i}" for i, (d, _) in enumerate(OVERLOAD)])
+def test_reference_host_runs_native_loop(hip, dim, text):
+    """The reference host built with -DDIFFUSE_OVERLOAD over the suite's
+    diffuse_overload, which runs the timestep loop through the native driver
+    (resident loops, fused sweeps) instead of the reference's drivers.
+    Against the omp3 suite and against the same host running its own
+    drivers over the suite: per-step iteration counts within 1 and the final
+    temperature the reference prints within 1e-12, and the reference's
+    PASSED line where omp3 passes.  (The host's own scalar arithmetic --
+    eigenvalues, Chebyshev coefficients -- is built here with FMA
+    contraction, the native driver's without, so the two agree to rounding,
+    not bit for bit.)"""
+    try:
+        got = ref_run(f"tealeaf{dim}_hip_overload", text)
+        host = ref_run(f"tealeaf{dim}_hip", text)
+        want = ref_run(f"tealeaf{dim}_omp3", text, threads=4)
+    except FileNotFoundError:
+        pytest.skip("reference host binaries not built (oracle/_ref)")
+    assert len(got["steps"]) == len(want["steps"]) == len(host["steps"])
+    for ref in (want, host):
+        for a, b in zip(got["steps"], ref["steps"]):
+            for k, v in b["iters"].items():
+                assert abs(a["iters"][k] - v) <= 1, (k, a["iters"], b["iters"])
+    ta = float(_ACTUAL.search(got["stdout"]).group(1))
+    for tw in (want["summary"][3], float(_ACTUAL.search(host["stdout"]).group(1))):
+        assert rel(ta, tw) < SUMMARY_RTOL, (ta, tw)
+    if "PASSED" in want["stdout"]:
+        assert "PASSED" in got["stdout"]
 
 
 @pytest.mark.parametrize("dim,text", DROPIN, ids=[f"{d}-{i}" for i, (d, _) in enumerate(DROPIN)])
