@@ -54,11 +54,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
 }
 
 // Stencil sweep: a block is kWaves consecutive rows of one 64*V-column
-// strip in tz consecutive planes; a wave one row, a lane V cells.  Each lane
-// reads the field at planes k-1, k, k+1 and rows j-1, j+1 of plane k, kz at k
-// and k+1, kx and ky at k (ky also at row j+1) and the op's inputs directly;
-// what adjacent waves and blocks share is served by L2 (blocks are dealt
-// XCD-contiguously, so each XCD sweeps a run of consecutive planes).
+// strip in tz consecutive planes; a wave one row, a lane V cells.  Marching
+// along z, a lane keeps planes k-1 and k of the field and kz at k from the
+// previous plane and reads the field at plane k+1 and rows j-1, j+1 of
+// plane k, kz at k+1, kx and ky at k (ky also at row j+1) and the op's
+// inputs; what adjacent waves and blocks share is served by L2 (blocks are
+// dealt XCD-contiguously, so each XCD sweeps a run of consecutive planes).
 // x-neighbours come from the adjacent lane by DPP; only the wave's end lanes
 // read them from memory.  Halo values are read as stored.
 template <int V, class Op>
@@ -88,29 +89,40 @@ k_stencil3d(TlGrid3 g, Tile3 t, const double* __restrict__ a, const double* __re
 
     if (j < g.y - g.hd) {   // uniform across the wave
         const long jrow = (long)j * X + c0;
-        const int k1 = min(g.hd + (kb + 1) * t.tz, g.z - g.hd);
-        for (int k = g.hd + kb * t.tz; k < k1; ++k) {
+        const int k0 = g.hd + kb * t.tz;             // < g.z - g.hd: kb < nbk
+        const int k1 = min(k0 + t.tz, g.z - g.hd);
+        // planes k-1 and k of the field and kz at k, carried along z
+        DV<V> a_dn = zdv<V>(), a_c = zdv<V>(), kz_c = zdv<V>();
+        if (ld_ok) {
+            a_dn = ldv<V>(a, (long)(k0 - 1) * XY + jrow);
+            a_c = ldv<V>(a, (long)k0 * XY + jrow);
+            kz_c = ldv<V>(kz, (long)k0 * XY + jrow);
+        }
+        for (int k = k0; k < k1; ++k) {
             const long b = (long)k * XY + jrow;
             Nb3<V> n;
             In4<V> in{};
             double el = 0.0, er = 0.0, ek = 0.0;
+            n.c = a_c;
+            n.dn = a_dn;
+            n.kzc = kz_c;
             if (ld_ok) {
-                n.c = ldv<V>(a, b);
                 n.up = ldv<V>(a, b + XY);
-                n.dn = ldv<V>(a, b - XY);
                 n.no = ldv<V>(a, b + X);
                 n.so = ldv<V>(a, b - X);
                 n.kxc = ldv<V>(kx, b);
                 n.kyc = ldv<V>(ky, b);
                 n.kyn = ldv<V>(ky, b + X);
-                n.kzc = ldv<V>(kz, b);
                 n.kzu = ldv<V>(kz, b + XY);
                 in = op.template load<V>(b);
                 if (lane == 0) el = a[b - 1];
             } else {
-                n.c = n.up = n.dn = n.no = n.so = zdv<V>();
-                n.kxc = n.kyc = n.kyn = n.kzc = n.kzu = zdv<V>();
+                n.up = n.no = n.so = zdv<V>();
+                n.kxc = n.kyc = n.kyn = n.kzu = zdv<V>();
             }
+            a_dn = n.c;
+            a_c = n.up;
+            kz_c = n.kzu;
             if (edge_r) {
                 er = a[b + V];
                 ek = kx[b + V];

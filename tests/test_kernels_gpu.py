@@ -124,6 +124,29 @@ def test_tile_rows_do_not_change_fields(hip):
         assert abs(got["pw"] - base["pw"]) <= SCALAR_RTOL * abs(base["pw"])
 
 
+STENCIL_3D = ("cg_solver_calc_w", "ppcg_inner_iteration", "calculate_residual",
+              "cheby_solver_iterate", "cheby_solver_init", "jacobi_solver_iterate",
+              "cg_solver_init")
+
+
+@pytest.mark.parametrize("planes", [1, 2, 3, 7, 64])
+def test_tile_planes_match_oracle(hip, planes):
+    """The 3D stencil sweeps carry planes k-1, k of the field and kz along z
+    within a tile of `planes` planes: every tile height (one plane, tiles
+    that split z unevenly, one tile for the whole depth) gives the oracle's
+    fields bit for bit."""
+    cases = [c for c in kc.make_cases(3, 19, 11, 13, seed=35) + kc.make_cases(3, 9, 8, 10, hd=1, seed=36)
+             if c.kernel in STENCIL_3D]
+    assert {c.kernel for c in cases} >= set(STENCIL_3D) - {"cg_solver_init"}
+    hip.tlhip_set_tile_planes(planes)
+    try:
+        for case in cases:
+            want = kc.run_host(load_oracle().dll, "or3d_", case)
+            _check(case, kc.run_gpu(case), want)
+    finally:
+        hip.tlhip_set_tile_planes(-1)
+
+
 def test_errors_are_reported_not_fatal(hip):
     from tealeaf_amd.abi import TealeafError
     from tealeaf_amd.device import DeviceArray
