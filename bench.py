@@ -44,6 +44,7 @@ BYTES_PER_CELL = {
     "tlhip2d_cg_calc_p": 24,              # read p, r; write p
     "tlhip2d_cheby_iterate": 64,          # read u, u0, p, kx, ky; write u', p, r
     "tlhip2d_cheby_iterations": 64,       # the same, once per launch of up to 10 iterations
+    "tlhip3d_ppcg_inner_iteration": 72,   # read sd, kx, ky, kz, r, u; write r, u, sd'
 }
 
 
@@ -64,6 +65,9 @@ def parse_args():
     ap.add_argument("--refine-steps", type=int, default=2)
     ap.add_argument("--refine-only", action="store_true",
                     help="run only the FP32-refined vs FP64 comparison and print its JSON")
+    ap.add_argument("--blocks3d", action="store_true",
+                    help="run only the 3D 256^3 seeded block-density PPCG workload and print its JSON")
+    ap.add_argument("--blocks3d-steps", type=int, default=2)
     ap.add_argument("--profile-run", action="store_true",
                     help="counter-collection run: a few launches of each hot kernel at --cells "
                          "and of a copy kernel of known size, then exit")
@@ -304,6 +308,62 @@ def load_traffic(kernel: str, init_share: float = 0.0):
         return None
 
 
+def blocks3d_run(args):
+    """The 3D workload: 256^3 cells on [0,10]^3, recip conductivity
+    (coefficient_inverse_density), 16^3-cell blocks with density
+    exp(U(ln 0.1, ln 100)) and energy U(0.1, 25) from the seeded block
+    generator, dt 0.004, PPCG with 10 inner steps (presteps 20), each step
+    solved to ||u0 - A u|| <= 1e-10 ||u0||.  Reports the wall time per step,
+    the iterations the reference would print, and the per-launch time of
+    the PPCG inner sweep (72 B/cell)."""
+    import torch
+    from tealeaf_amd import blocks
+    from tealeaf_amd import deck as tdeck
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.problem import Problem
+
+    L = lib()
+    L.tlhip_init(0)
+    n = 256
+    text = tdeck.cuboid_3d(n, args.blocks3d_steps, solver=tdeck.PPCG, inner=10, coefficient=2,
+                           dt=0.004, max_iters=20000)
+    s, states = tdeck.parse_deck(text, 3)
+    dens, en = blocks.block_fields((n, n, n), 16, seed=256, rho=(0.1, 100.0), energy=(0.1, 25.0),
+                                   halo_depth=s.halo_depth)
+    timed = "tlhip3d_ppcg_inner_iteration"
+    steps = []
+    with Problem(s, states) as prob:
+        prob.inject(dens, en)
+        prob.set_solve(0, 1e-10)
+        L.tlhip_kernel_timer_start(timed.encode())
+        for tt in range(args.blocks3d_steps):
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rep = prob.step(tt)
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            steps.append({"seconds": dt, "printed_cg": rep["printed_cg_iters"],
+                          "printed_ppcg": rep["printed_inner_iters"],
+                          "ppcg_inner": rep["n_ppcg_inner"], "rel_residual": rep["rel_residual"]})
+            print(f"[bench] 3d blocks step {tt}: {dt:.3f} s, {rep['printed_cg_iters']} CG + "
+                  f"{rep['printed_inner_iters']} PPCG, rel residual {rep['rel_residual']:.2e}",
+                  file=sys.stderr, flush=True)
+        ms, nl = ctypes.c_double(), ctypes.c_long()
+        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
+    total = sum(st["seconds"] for st in steps)
+    iters = sum(st["printed_cg"] + st["printed_ppcg"] for st in steps)
+    avg_ms = ms.value / max(nl.value, 1)
+    gbs = BYTES_PER_CELL[timed] * n ** 3 / (avg_ms * 1e-3) / 1e9 if nl.value else None
+    return {"workload": "3D 256^3 [0,10]^3, recip conductivity, 16^3-cell blocks (density "
+                        "log-uniform 0.1..100, energy U(0.1,25), seed 256), dt 0.004, PPCG 10 inner "
+                        "(presteps 20), relative residual 1e-10, FP64",
+            "steps": steps, "seconds_per_step": total / len(steps), "iters_per_s": iters / total,
+            "kernel": {timed: {"launches": nl.value, "avg_ms": avg_ms, "gbs": gbs,
+                               "frac": gbs / HBM_PEAK_GBS if gbs else None}}}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -315,6 +375,9 @@ def main():
         return
     if args.refine_only:
         print(json.dumps({"refine": refine_run(args)}), flush=True)
+        return
+    if args.blocks3d:
+        print(json.dumps({"blocks3d": blocks3d_run(args)}), flush=True)
         return
     res = gpu_run(args)
     refine = refine_run(args) if args.refine_cells > 0 else None

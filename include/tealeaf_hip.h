@@ -365,8 +365,8 @@ typedef struct tlhip_problem tlhip_problem;
  * Results are identical either way.  Default off: at 4096^2 on MI355X the
  * fused pass (48 B/cell at 4.4 TB/s) is no faster than the two. */
 int tlhip_set_cg_fusion(int on);
-/* Whether the driver runs CG and Jacobi iterations (2D, 3D) and fused PPCG
- * outer iterations (2D) device-resident: alpha = rro/pw, beta = rrn/rro, the
+/* Whether the driver runs CG, Jacobi and PPCG outer iterations (2D, 3D)
+ * device-resident: alpha = rro/pw, beta = rrn/rro, the
  * Jacobi error and the stop and switch tests of cg_driver.c /
  * cheby_driver.c / ppcg_driver.c / jacobi_driver.c are evaluated on the
  * device, the host keeps several iterations enqueued and reads one report

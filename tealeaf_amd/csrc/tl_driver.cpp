@@ -338,7 +338,7 @@ struct tlhip_problem {
     bool resident_cg() const { return g_resident && !can_defer_p() && interior(); }
     bool resident_ppcg() const
     {
-        return resident_cg() && fused_ppcg() && s.ppcg_inner_steps > 0;
+        return resident_cg() && (fused_ppcg() || dims == 3) && s.ppcg_inner_steps > 0;
     }
 
     // The stop test of stop() and, with sw, the switch_now test of the next
@@ -387,9 +387,11 @@ struct tlhip_problem {
         return halo_update(1);
     }
 
-    // ppcg_main_step's fused path (2D), then its halo update of p.
+    // ppcg_main_step's fused path (2D) or its step-by-step path (3D), then
+    // its halo update of p.
     int ppcg_body(const tl::loop::StopRule& rule)
     {
+        if (dims == 3) return ppcg_body3(rule);
         TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
         int done = 0;
         while (done < s.ppcg_inner_steps) {
@@ -429,6 +431,29 @@ struct tlhip_problem {
             if (dims == 2) TRY(tl::loop2d::norm_err(x, y, hd, r, rule));
             else TRY(tl::loop3d::norm_err(x, y, z, hd, r, rule));
         }
+        return halo_update(1);
+    }
+
+    // 3D: calc_w, calc_ur (its rrn unused), ppcg_init, the inner steps each
+    // after a depth-1 halo update of sd, the 2-norm of r, calc_p
+    // (3d/drivers/ppcg_solver_driver.c:88-140).
+    int ppcg_body3(const tl::loop::StopRule& rule)
+    {
+        TRY(tl::loop3d::calc_w(x, y, z, hd, p, w, kx, ky, kz));
+        TRY(tl::loop3d::calc_ur_only(x, y, z, hd, u, p, r, w));
+        TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
+        reset_fields();
+        fields[F_SD] = true;
+        for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
+            TRY(halo_update(1));
+            TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
+                                             kx, ky, kz, sd, sd_alt));
+            std::swap(sd, sd_alt);
+        }
+        reset_fields();
+        fields[F_P] = true;
+        TRY(tl::loop3d::norm(x, y, z, hd, r, rule));
+        TRY(tl::loop3d::calc_p(x, y, z, hd, p, r));
         return halo_update(1);
     }
 

@@ -250,8 +250,9 @@ k_jacobi_init3(TlGrid3 g, int coef, double rx, double ry, double rz,
 
 // Halo ring (every cell outside the interior box) src -> dst.
 __global__ void __launch_bounds__(kBlock)
-k_copy_ring3d(TlGrid3 g, const double* __restrict__ src, double* __restrict__ dst)
+k_copy_ring3d(TlGrid3 g, const double* __restrict__ src, double* __restrict__ dst, TlGate gate)
 {
+    if (gate.closed()) return;
     const long xy = (long)g.x * g.y;
     const int nz = g.z - 2 * g.hd, ny = g.y - 2 * g.hd;
     const long nplane = 2L * g.hd * xy;                 // hd planes front and back
@@ -503,7 +504,7 @@ int copy_ring(const TlGrid3& g, const double* src, double* dst, const char* what
     const long n = (long)g.x * g.y * g.z;
     const long inner = (long)(g.x - 2 * g.hd) * (g.y - 2 * g.hd) * (g.z - 2 * g.hd);
     hipLaunchKernelGGL(k_copy_ring3d, dim3(flat_blocks(n - inner)), dim3(kBlock), 0, tl_stream(),
-                       g, src, dst);
+                       g, src, dst, tl_gate());
     return tl_check_launch(what);
 }
 
@@ -923,6 +924,34 @@ int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, 
     OpCalcUr op{u, p, r, w, 0.0, loop::alpha()};
     unsigned nb = 0;
     TL_TRY(launch_interior3(g, pick_v(g, u, p, r, w), op, what, &nb));
+    return loop::finish_beta(nb, rule, what);
+}
+
+// cg_calc_ur of a PPCG outer iteration, whose rrn the driver does not use
+// (3d/drivers/ppcg_solver_driver.c:107-123): no finisher, the partials are
+// left unread.
+int calc_ur_only(int x, int y, int z, int hd, double* u, const double* p, double* r,
+                 const double* w)
+{
+    const char* what = "tlhip3d_cg_solver_calc_ur";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(u && p && r && w, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    OpCalcUr op{u, p, r, w, 0.0, loop::alpha()};
+    unsigned nb = 0;
+    return launch_interior3(g, pick_v(g, u, p, r, w), op, what, &nb);
+}
+
+int norm(int x, int y, int z, int hd, const double* buf, const loop::StopRule& rule)
+{
+    const char* what = "tlhip3d_calculate_2norm";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(buf, what, "null argument");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    unsigned nb = 0;
+    TL_TRY(launch_interior3(g, pick_v(g, buf), OpNorm{buf}, what, &nb));
     return loop::finish_beta(nb, rule, what);
 }
 

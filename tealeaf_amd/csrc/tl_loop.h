@@ -101,6 +101,11 @@ int calc_w(int x, int y, int z, int hd, const double* p, double* w, const double
 int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, const double* w,
             const loop::StopRule& rule);
 int calc_p(int x, int y, int z, int hd, double* p, const double* r);
+// cg_calc_ur with alpha from the device and no finisher (PPCG's outer step,
+// whose rrn the driver replaces by the 2-norm after the inner loop).
+int calc_ur_only(int x, int y, int z, int hd, double* u, const double* p, double* r,
+                 const double* w);
+int norm(int x, int y, int z, int hd, const double* buf, const loop::StopRule& rule);
 int jacobi(int x, int y, int z, int hd, const double* kx, const double* ky, const double* kz,
            const double* u0, const double* u_in, double* u_out, loop::ErrMode mode,
            const loop::StopRule& rule);

@@ -8,12 +8,13 @@ ppcg_driver.c:95-131).  Both run the same kernels with the same arguments in
 the same order, so every report, every field and the summary must be the
 same bits.  The resident Jacobi loop sweeps between u and a partner buffer
 instead of copying u into r first (jacobi_driver.c:51-55): the same values,
-so there too every bit must agree, r included.  The cases cover each place the loop can end: convergence, the
-CG -> Chebyshev / PPCG switch by presteps and by the error switch, max_iters
-(in the CG phase and in the PPCG phase), the relative-residual stop whose
-true-residual check can refuse a candidate, PPCG inner-step counts whose
-fused launches leave the ping-pong buffers swapped, 3D (sqrt of the error
-in the stop test), and seeded block densities.  The parity of the resident
+so there too every bit must agree, r included.  The cases cover each place
+the loop can end: convergence, the CG -> Chebyshev / PPCG switch by presteps
+and by the error switch, max_iters (in the CG phase and in the PPCG phase),
+the relative-residual stop whose true-residual check can refuse a
+candidate, PPCG inner-step counts whose fused launches leave the ping-pong
+buffers swapped, 3D (sqrt of the error in the stop test; PPCG's outer loop
+resident around step-by-step inner sweeps), and seeded block densities.  The parity of the resident
 path with the reference itself is held by the other driver tests, which run
 with the default.
 """
@@ -106,6 +107,23 @@ def test_relative_residual_stop(hip, solver):
 @pytest.mark.parametrize("solver", [tdeck.CG, tdeck.CHEBY, tdeck.PPCG])
 def test_3d(hip, solver):
     _same(hip, tdeck.cuboid_3d(20, 2, solver=solver, presteps=15, max_iters=2000), dims=3)
+
+
+@pytest.mark.parametrize("max_iters", [5, 14, 20])
+def test_3d_ppcg_max_iters(hip, max_iters):
+    """3D PPCG runs resident in its outer loop too (step by step inside).
+    At 24^3 (presteps 10) the oracle needs 11 CG + 16 PPCG iterations per
+    step, so the cut falls inside the CG phase (5) and inside the PPCG
+    phase (14, 20)."""
+    reps = _same(hip, tdeck.cuboid_3d(24, 2, solver=tdeck.PPCG, presteps=10, inner=10,
+                                      max_iters=max_iters), dims=3)
+    assert all(r["printed_cg_iters"] + r["printed_inner_iters"] >= max_iters - 1 for r in reps)
+
+
+def test_3d_ppcg_relative_stop(hip):
+    reps = _same(hip, tdeck.cuboid_3d(24, 2, solver=tdeck.PPCG, presteps=10, inner=10,
+                                      max_iters=2000), dims=3, rel=1e-9)
+    assert all(r["n_true_residuals"] >= 1 and r["n_ppcg_outer"] > 0 for r in reps)
 
 
 def test_3d_blocks_recip(hip):
