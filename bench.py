@@ -103,7 +103,7 @@ def gpu_run(args):
     total_iters = 0
     kernel = {}
     for s in solvers:
-        timed = dominant(args) if s == "ppcg" else ("tlhip2d_cg_calc_w" if s == "cg" else "tlhip2d_cheby_iterations")
+        timed = dominant(args) if s == "ppcg" else ("tlhip2d_cg_calc_pw" if s == "cg" else "tlhip2d_cheby_iterations")
         L.tlhip_synchronize()
         torch.cuda.synchronize()
         L.tlhip_kernel_timer_start(timed.encode())

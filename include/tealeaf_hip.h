@@ -362,8 +362,8 @@ typedef struct tlhip_problem tlhip_problem;
 
 /* Whether the driver (2D) runs a CG/PPCG step's calc_p fused into the next
  * step's calc_w (tlhip2d_cg_calc_pw) or as calc_p + halo update + calc_w.
- * Results are identical either way.  Default off: at 4096^2 on MI355X the
- * fused pass (48 B/cell at 4.4 TB/s) is no faster than the two. */
+ * Results are identical either way.  Default on: at 4096^2 on MI355X the
+ * fused pass (48 B/cell, 0.162 ms) takes 9% less than the two (0.177 ms). */
 int tlhip_set_cg_fusion(int on);
 /* Whether the driver runs CG, Jacobi and PPCG outer iterations (2D, 3D)
  * device-resident: alpha = rro/pw, beta = rrn/rro, the

@@ -47,7 +47,7 @@ enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 int g_ppcg_fusion = 10;
 // Chebyshev iterations per fused launch (2D, FP64, reference stop test).
 int g_cheby_fusion = 10;
-int g_cg_fusion = 0;     // fuse calc_p into the next calc_w (2D); measured no faster
+int g_cg_fusion = 1;     // fuse calc_p into the next calc_w (2D, tlhip2d_cg_calc_pw)
 int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 
 double* dalloc(long n, int* rc)
@@ -335,7 +335,7 @@ struct tlhip_problem {
     {
         return x > 2 * hd && y > 2 * hd && (dims == 2 || z > 2 * hd);
     }
-    bool resident_cg() const { return g_resident && !can_defer_p() && interior(); }
+    bool resident_cg() const { return g_resident && interior(); }
     bool resident_ppcg() const
     {
         return resident_cg() && (fused_ppcg() || dims == 3) && s.ppcg_inner_steps > 0;
@@ -373,9 +373,24 @@ struct tlhip_problem {
         return 0;
     }
 
+    // calc_w, or with a calc_p pending, calc_p(beta on the device) fused
+    // into it (as calc_w_pending); alpha is finished on the device.
+    int calc_w_resident()
+    {
+        if (!pending_p) return tl::loop2d::calc_w(x, y, hd, p, w, kx, ky);
+        pending_p = false;
+        TRY(tl::loop2d::calc_pw(x, y, hd, p, p_alt, r, w, kx, ky));
+        std::swap(p, p_alt);
+        return 0;
+    }
+
     int cg_body(const tl::loop::StopRule& rule)
     {
-        if (dims == 2) {
+        if (dims == 2 && can_defer_p()) {
+            TRY(calc_w_resident());
+            TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
+            pending_p = true;    // calc_p waits for the next calc_w (beta stays on the device)
+        } else if (dims == 2) {
             TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
             TRY(tl::loop2d::calc_p(x, y, hd, p, r));
@@ -392,7 +407,8 @@ struct tlhip_problem {
     int ppcg_body(const tl::loop::StopRule& rule)
     {
         if (dims == 3) return ppcg_body3(rule);
-        TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
+        if (can_defer_p()) TRY(calc_w_resident());
+        else TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
         int done = 0;
         while (done < s.ppcg_inner_steps) {
             const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
@@ -409,7 +425,8 @@ struct tlhip_problem {
         reset_fields();
         fields[F_P] = true;
         TRY(tl::loop2d::norm(x, y, hd, r, rule));
-        TRY(tl::loop2d::calc_p(x, y, hd, p, r));
+        if (can_defer_p()) pending_p = true;
+        else TRY(tl::loop2d::calc_p(x, y, hd, p, r));
         return halo_update(1);
     }
 
@@ -458,7 +475,7 @@ struct tlhip_problem {
     }
 
     struct Ptrs {
-        double *u, *u_alt, *r, *r_alt, *sd, *sd_alt;
+        double *u, *u_alt, *r, *r_alt, *sd, *sd_alt, *p, *p_alt;
     };
 
     // Iterations t0, t0+1, ... of `body`, up to kAhead beyond the one whose
@@ -487,7 +504,7 @@ struct tlhip_problem {
             while (next < s.max_iters && next <= chk + tl::loop::kAhead) {
                 tl::loop::set_iter(next);
                 TRY(body(rule, next));
-                snap[next % kSnap] = Ptrs{u, u_alt, r, r_alt, sd, sd_alt};
+                snap[next % kSnap] = Ptrs{u, u_alt, r, r_alt, sd, sd_alt, p, p_alt};
                 ++next;
             }
             TRY(tl::loop::wait(chk, &rp));
@@ -497,7 +514,13 @@ struct tlhip_problem {
         }
         const Ptrs& q = snap[chk % kSnap];
         u = q.u, u_alt = q.u_alt, r = q.r, r_alt = q.r_alt, sd = q.sd, sd_alt = q.sd_alt;
+        p = q.p, p_alt = q.p_alt;
         if (cg) TRY(tl::loop::logs(t0, chk - t0 + 1, &cg_alphas[t0], &cg_betas[t0]));
+        if (pending_p) {
+            // the last iteration's calc_p waits with its beta (host copy for flush_p)
+            double a;
+            TRY(tl::loop::logs(chk, 1, &a, &pending_beta));
+        }
         *last = chk;
         *out = rp;
         *rro = rp.rrn;

@@ -579,26 +579,32 @@ k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
 // computed at every cell the stencil reads and stored to a second buffer.
 // Both are the reference's expressions in its order, so p_out and w equal
 // cg_calc_p, a depth-1 halo update of p and cg_calc_w bit for bit; the dot
-// product differs only by summation order.  Layout as the single-step
-// stencils: a block is kWaves rows of one strip of 64*N columns (strips
-// start on a multiple of N), a wave one row; rows j-1, j+1 and the end
-// lanes' x neighbours are read directly (L2 serves the overlap).
+// product differs only by summation order.  Each wave marches up ty rows of
+// its own strip of 64*N columns (strips start on a multiple of N; a block
+// is kWaves strips side by side): the new p of rows j-1, j, j+1 and ky of
+// rows j, j+1 stay in registers, so a row costs one load each of p, r, kx
+// and ky, and the new p is formed once per cell; only the wave's end lanes
+// read x neighbours from memory.  beta is the argument or, in a resident
+// loop, the loop's device value.
 template <class T, bool UPDATE>
 __global__ void __launch_bounds__(kBlock)
-k_calc_pw(PitchGrid g, int cw0, int nbx, int nby, int ty, T beta, const T* __restrict__ p_in,
-          T* __restrict__ p_out, const T* __restrict__ r, T* __restrict__ w,
-          const T* __restrict__ kx, const T* __restrict__ ky, TlReduce rd)
+k_calc_pw(PitchGrid g, int cw0, int ns, int nbx, int nby, int ty, T beta_arg,
+          const double* __restrict__ beta_dev, const T* __restrict__ p_in, T* __restrict__ p_out,
+          const T* __restrict__ r, T* __restrict__ w, const T* __restrict__ kx,
+          const T* __restrict__ ky, TlReduce rd, TlGate gate)
 {
+    if (gate.closed()) return;
+    const T beta = UPDATE && beta_dev ? (T)*beta_dev : beta_arg;
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
     const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
     const int tile = xcd_tile_of(blockIdx.x, nbx * nby);
-    const int sx = tile % nbx, by = tile / nbx;
+    const int sx = (tile % nbx) * kWaves + wave, by = tile / nbx;
     const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
     const int c = cw0 + sx * (TL_WAVE * N) + N * lane;
-    const bool ld_ok = c + N <= g.X;
+    const bool ld_ok = sx < ns && c + N <= g.X;
     const bool all_out = c >= xin0 && c + N <= xin1;
-    const bool any_out = c + N > xin0 && c < xin1;
+    const bool any_out = sx < ns && c + N > xin0 && c < xin1;
     const long X = g.X;
     auto ld = [&](const T* __restrict__ f, long i) -> C {
         return ld_ok ? ldc(f, i) : zeros<T>();
@@ -617,40 +623,48 @@ k_calc_pw(PitchGrid g, int cw0, int nbx, int nby, int ty, T beta, const T* __res
         return p_in[i];
     };
     double acc[1] = {0.0};
-    const int jend = min(yin0 + (by + 1) * ty, yin1);
-    for (int j = yin0 + by * ty + wave; j < jend; j += kWaves) {   // uniform per wave
-        const long i0 = (long)j * X + c;
-        const C cen = pnew(i0);
-        // y neighbours outside the interior mirror the cell
-        const C up = j == yin1 - 1 ? cen : pnew(i0 + X);
-        const C dn = j == yin0 ? cen : pnew(i0 - X);
-        const C kxc = ld(kx, i0), kyc = ld(ky, i0), kyu = ld(ky, i0 + X);
-        T el = T(0), er = T(0), ek = T(0);
-        if (lane == 0 && c >= 1) el = pnew1(i0 - 1);
-        if (lane == TL_WAVE - 1 && c + N < g.X) {
-            er = pnew1(i0 + N);
-            ek = kx[i0 + N];
-        }
-        const C av = apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, el, er, ek);
-        if (any_out) {
-            if (all_out) {
-                stc(w, i0, av);
-                if constexpr (UPDATE) stc(p_out, i0, cen);
-#pragma unroll
-                for (int k = 0; k < N; ++k) acc[0] += (double)av.v[k] * (double)cen.v[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < N; ++k)
-                    if (c + k >= xin0 && c + k < xin1) {
-                        w[i0 + k] = av.v[k];
-                        if constexpr (UPDATE) p_out[i0 + k] = cen.v[k];
-                        acc[0] += (double)av.v[k] * (double)cen.v[k];
-                    }
+    const int j0 = yin0 + by * ty;
+    const int j1 = min(j0 + ty, yin1);
+    if (sx < ns && j0 < j1) {                    // uniform per wave
+        long i0 = (long)j0 * X + c;
+        C cen = pnew(i0);
+        C dn = j0 == yin0 ? cen : pnew(i0 - X);  // y neighbours outside the interior mirror
+        C kyc = ld(ky, i0);
+        for (int j = j0; j < j1; ++j, i0 += X) {
+            const C up = j == yin1 - 1 ? cen : pnew(i0 + X);
+            const C kxc = ld(kx, i0), kyu = ld(ky, i0 + X);
+            T el = T(0), er = T(0), ek = T(0);
+            if (lane == 0 && c >= 1) el = pnew1(i0 - 1);
+            if (lane == TL_WAVE - 1 && c + N < g.X) {
+                er = pnew1(i0 + N);
+                ek = kx[i0 + N];
             }
+            const C av = apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, el, er, ek);
+            if (any_out) {
+                if (all_out) {
+                    stc(w, i0, av);
+                    if constexpr (UPDATE) stc(p_out, i0, cen);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) acc[0] += (double)av.v[k] * (double)cen.v[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+                        if (c + k >= xin0 && c + k < xin1) {
+                            w[i0 + k] = av.v[k];
+                            if constexpr (UPDATE) p_out[i0 + k] = cen.v[k];
+                            acc[0] += (double)av.v[k] * (double)cen.v[k];
+                        }
+                }
+            }
+            dn = cen;
+            cen = up;
+            kyc = kyu;
         }
     }
     tl_reduce_partial<1, kBlock>(acc, rd);
 }
+
+constexpr int kPwRows = 32;   // rows per wave of k_calc_pw (measured best at 4096^2)
 
 // Everything outside the interior of up to three fields, src -> dst.
 template <class T>
@@ -944,9 +958,12 @@ int cheby_iterations(int x, int y, int hd, int count, const double* alphas, cons
     }
 }
 
+// Checks, the ring copy p_in -> p_out (the halo the reference's p keeps)
+// and the k_calc_pw launch under the current gate; *nb: its partials.
 template <class T>
-int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
-            double* pw, const T* p_in, T* p_out, const T* r, T* w, const T* kx, const T* ky)
+int launch_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
+              const double* beta_dev, const T* p_in, T* p_out, const T* r, T* w, const T* kx,
+              const T* ky, unsigned* nb)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
@@ -955,40 +972,50 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
         return tl_fail(TLHIP_ERR_ARG,
                        "%s: needs a row pitch that is a multiple of %d and >= x (x=%d pitch=%d "
                        "y=%d hd=%d)", what, N, x, pitch, y, hd);
-    TL_REQUIRE(pw && p_in && w && kx && ky && (!update || (p_out && r)), what, "null argument");
+    TL_REQUIRE(p_in && w && kx && ky && (!update || (p_out && r)), what, "null argument");
     TL_REQUIRE(w != p_in && (!update || (p_out != p_in && p_out != w && p_out != r)), what,
                "p_out and w must be distinct from the inputs");
     TL_REQUIRE(all_aligned(p_in, p_out, r, w, kx, ky), what, "fields must be 16-byte aligned");
     const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
     const int cw0 = hd / N * N;
-    const int nbx = (int)tl_cdiv(hd + g.nx - cw0, TL_WAVE * N);
-    int ty = tl_tile_rows_override() > 0 ? (int)tl_cdiv(tl_tile_rows_override(), kWaves) * kWaves
-                                         : kWaves;
+    const int ns = (int)tl_cdiv(hd + g.nx - cw0, TL_WAVE * N);
+    const int nbx = (int)tl_cdiv(ns, kWaves);
+    int ty = tl_tile_rows_override() > 0 ? tl_tile_rows_override() : kPwRows;
     while ((long)nbx * tl_cdiv(g.ny, ty) > TL_MAX_RED_BLOCKS) ty *= 2;
     const int nby = (int)tl_cdiv(g.ny, ty);
-    const unsigned nb = (unsigned)(nbx * nby);
-    if (nbx > TL_MAX_RED_BLOCKS) return tl_fail(TLHIP_ERR_ARG, "%s: rows too long", what);
+    *nb = (unsigned)(nbx * nby);
     if (update) {
         Ring3<T> rg{};
         rg.src[0] = p_in;
         rg.dst[0] = p_out;
         rg.n = 1;
         const long ring = (long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny;
-        // ungated: calc_pw returns pw to the host, so it never runs inside a
-        // resident loop
         hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
-                           tl_stream(), g, y, rg, TlGate{});
+                           tl_stream(), g, y, rg, tl_gate());
         TL_TRY(tl_check_launch(what));
     }
     const int slot = tl_timer_begin(what);
     if (update)
-        hipLaunchKernelGGL((k_calc_pw<T, true>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, cw0,
-                           nbx, nby, ty, (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+        hipLaunchKernelGGL((k_calc_pw<T, true>), dim3(*nb), dim3(kBlock), 0, tl_stream(), g, cw0,
+                           ns, nbx, nby, ty, (T)beta, beta_dev, p_in, p_out, r, w, kx, ky,
+                           tl_reduce_slots(), tl_gate());
     else
-        hipLaunchKernelGGL((k_calc_pw<T, false>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, cw0,
-                           nbx, nby, ty, (T)beta, p_in, p_out, r, w, kx, ky, tl_reduce_slots());
+        hipLaunchKernelGGL((k_calc_pw<T, false>), dim3(*nb), dim3(kBlock), 0, tl_stream(), g, cw0,
+                           ns, nbx, nby, ty, (T)beta, beta_dev, p_in, p_out, r, w, kx, ky,
+                           tl_reduce_slots(), tl_gate());
     tl_timer_end(slot);
-    TL_TRY(tl_check_launch(what));
+    return tl_check_launch(what);
+}
+
+// The entry points' form: pw finished and fetched to the host (+=).
+template <class T>
+int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
+            double* pw, const T* p_in, T* p_out, const T* r, T* w, const T* kx, const T* ky)
+{
+    TL_REQUIRE(pw, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_pw<T>(what, x, y, hd, pitch, update, beta, nullptr, p_in, p_out, r, w, kx, ky,
+                        &nb));
     TL_TRY(tl_finish_reduce(1, nb, what));
     return fetch_add(pw, what);
 }
@@ -1010,6 +1037,18 @@ int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const doub
                                     betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
                                     r_out, u_in, u_out, kx, ky, 0.0, p, w,
                                     p ? loop::alpha() : nullptr);
+}
+
+// calc_p with the loop's beta fused into calc_w, pw finished into alpha.
+int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const double* r, double* w,
+            const double* kx, const double* ky)
+{
+    const char* what = "tlhip2d_cg_calc_pw";
+    if (x % 2) return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x (got %d)", what, x);
+    unsigned nb = 0;
+    TL_TRY(launch_pw<double>(what, x, y, hd, x, true, 0.0, loop::beta(), p_in, p_out, r, w, kx,
+                             ky, &nb));
+    return loop::finish_alpha(nb, what);
 }
 
 }  // namespace loop2d

@@ -81,6 +81,10 @@ int calc_w(int x, int y, int hd, const double* p, double* w, const double* kx, c
 int calc_ur(int x, int y, int hd, double* u, const double* p, double* r, const double* w,
             const loop::StopRule& rule);
 int calc_p(int x, int y, int hd, double* p, const double* r);
+// calc_p with the loop's beta fused into the next calc_w (p_in -> p_out, w),
+// pw finished into alpha (tlhip2d_cg_calc_pw's pass).
+int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const double* r, double* w,
+            const double* kx, const double* ky);
 int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
 // One Jacobi sweep from u_in into u_out (run_jacobi_iterate without the copy
 // of u into r: the sweep reads u_in where the reference reads that copy).

@@ -281,11 +281,14 @@ def test_fused_with_calc_ur_matches_oracle(hip, oracle, grid, count):
         v.free()
 
 
+@pytest.mark.parametrize("rows", [-1, 1, 5])
 @pytest.mark.parametrize("grid", [(250, 70, 2), (74, 29, 1), (300, 21, 3), (6, 5, 2)],
                          ids=["250x70", "74x29h1", "300x21h3", "6x5"])
-def test_calc_pw_matches_calc_p_halo_calc_w(hip, oracle, grid):
+def test_calc_pw_matches_calc_p_halo_calc_w(hip, oracle, grid, rows):
     """tlhip2d_cg_calc_pw against the oracle's cg_calc_p, depth-1 halo update
-    of p and cg_calc_w: p_out and w bit-exact, pw to reduction order."""
+    of p and cg_calc_w: p_out and w bit-exact, pw to reduction order; with
+    the default rows per wave and with 1 and 5 (tiles that end inside the
+    grid, so the y window restarts at every tile)."""
     import ctypes
     from tealeaf_amd.device import DeviceArray
     nx, ny, hd = grid
@@ -308,8 +311,12 @@ def test_calc_pw_matches_calc_p_halo_calc_w(hip, oracle, grid):
     d = {k: DeviceArray.from_host(v) for k, v in dict(p=p, r=r, kx=kx, ky=ky).items()}
     p_out, w = DeviceArray.from_host(np.full(n, np.nan)), DeviceArray.from_host(np.full(n, np.nan))
     pw = ctypes.c_double(0.0)
-    hip.tlhip2d_cg_calc_pw(x, y, hd, beta, ctypes.addressof(pw), d["p"].ptr, p_out.ptr, d["r"].ptr,
-                           w.ptr, d["kx"].ptr, d["ky"].ptr)
+    hip.tlhip_set_tile_rows(rows)
+    try:
+        hip.tlhip2d_cg_calc_pw(x, y, hd, beta, ctypes.addressof(pw), d["p"].ptr, p_out.ptr,
+                               d["r"].ptr, w.ptr, d["kx"].ptr, d["ky"].ptr)
+    finally:
+        hip.tlhip_set_tile_rows(-1)
     gp, gw = p_out.download(), w.download()
     np.testing.assert_array_equal(_interior(gp, x, y, hd), _interior(p_ref, x, y, hd))
     np.testing.assert_array_equal(_interior(gw, x, y, hd), _interior(w_ref, x, y, hd))
