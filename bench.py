@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X published HBM3E bandwidth
 # Algorithmic bytes per interior cell per call, FP64 (each distinct array
-# read once + each array written once; see DESIGN.md "Kernels").
+# read once + each array written once).
 BYTES_PER_CELL = {
     "tlhip2d_ppcg_inner_iteration": 64,   # read sd, kx, ky, r, u; write r, u, sd'
     "tlhip2d_ppcg_inner_iterations": 64,  # the same arrays once for S fused inner steps
@@ -368,7 +368,8 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
-              "is a design, not a build (DESIGN.md).", file=sys.stderr)
+              "(one chunk per GPU, halo faces and two scalar all-reduces per CG iteration) "
+              "is a design, not a build.", file=sys.stderr)
         sys.exit(2)
     if args.profile_run:
         profile_run(args)
