@@ -69,6 +69,9 @@ void* tlhip_get_stream(void);
  * the automatic choice. */
 int tlhip_set_tile_rows(int rows);
 int tlhip_set_tile_planes(int planes);
+/* Most blocks a pointwise sweep (calc_ur, calc_p, 2-norm, ...) launches;
+ * <= 0 restores the default (4096). */
+int tlhip_set_interior_blocks(int blocks);
 
 /* Device memory, zero-filled (replaces the allocate_buffer calls of
  * run_kernel_initialise, 2d/kernel_interface.h:17 / 3d/kernel_interface.h:13,

@@ -62,6 +62,8 @@ int tl_fail(int code, const char* fmt, ...);
 int tl_check_launch(const char* what);
 // Output rows per tile set by tlhip_set_tile_rows (<= 0: the kernel's default).
 int tl_tile_rows_override();
+// Most blocks a pointwise sweep launches (tlhip_set_interior_blocks).
+int tl_interior_blocks();
 // Wait for the stream and copy `n` published reduction values out.
 int tl_fetch(int n, double* out, const char* what);
 int tl_ensure_init();

@@ -440,6 +440,19 @@ bool interior_empty(const TlGrid2& g) { return g.x - 2 * g.hd <= 0 || g.y - 2 * 
 
 int tl_tile_rows_override() { return g_ty_override; }
 
+namespace {
+int g_interior_blocks = 4096;
+}
+int tl_interior_blocks() { return g_interior_blocks; }
+
+extern "C" int tlhip_set_interior_blocks(int blocks)
+{
+    if (blocks > TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_set_interior_blocks: at most %d", TL_MAX_RED_BLOCKS);
+    g_interior_blocks = blocks > 0 ? blocks : 4096;
+    return 0;
+}
+
 extern "C" int tlhip_set_tile_rows(int rows)
 {
     g_ty_override = rows;

@@ -454,7 +454,8 @@ static int launch_interior(const TlRows& g, int nx, int V, const Op& op, const c
     const long nseg = (long)spr * g.nrows;
     // ~16 waves per CU, each thread owning two segments at a time.
     const long want = (nseg + 1) / 2;
-    const unsigned nb = (unsigned)(want < 4096 ? want : 4096);
+    const long cap = tl_interior_blocks();
+    const unsigned nb = (unsigned)(want < cap ? want : cap);
     const int slot = tl_timer_begin(what);
     if (V == 2)
         hipLaunchKernelGGL((k_interior<2, Op>), dim3(nb), dim3(kBlock), 0, tl_stream(), g, spr,
