@@ -262,6 +262,15 @@ int tlhip3d_cg_solver_init(int x, int y, int z, int halo_depth,
 int tlhip3d_cg_solver_calc_w(int x, int y, int z, int halo_depth, double* pw,
         const double* p, double* w, const double* kx, const double* ky,
         const double* kz);
+/* No single counterpart: one CG step's calc_p (3d/kernel_interface.h:36
+ * run_cg_solver_calc_p, called at 3d/drivers/cg_solver_driver.c:116), the
+ * depth-1 halo update of p and the next step's calc_w (:32, called at
+ * cg_solver_driver.c:79) in one pass, p_in -> p_out (the halo ring copied)
+ * and w, bit-exact with the three; pw accumulated (+=).  The native driver
+ * does not use it: inside a 3D solve it measured no faster than the two. */
+int tlhip3d_cg_solver_calc_pw(int x, int y, int z, int halo_depth, double beta, double* pw,
+        const double* p_in, double* p_out, const double* r, double* w, const double* kx,
+        const double* ky, const double* kz);
 /* 3d/kernel_interface.h:34 run_cg_solver_calc_ur */
 int tlhip3d_cg_solver_calc_ur(int x, int y, int z, int halo_depth,
         double alpha, double* rrn, double* u, const double* p, double* r,

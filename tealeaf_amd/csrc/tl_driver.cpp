@@ -224,6 +224,10 @@ struct tlhip_problem {
     // one pass (p ping-pongs with p_alt); the reference's halo update of p in
     // between is the kernel's mirror.  The results are the reference's bit
     // for bit.
+    // 2D only: in 3D the fused pass (tlhip3d_cg_solver_calc_pw) reads p and r
+    // at six neighbours each and measured no faster inside a solve (128^3 CG
+    // 0.058 ms per iteration either way, 160^3 0.125 against 0.101, 256^3
+    // 0.232 against 0.217 ms per pass).
     bool can_defer_p() const { return g_cg_fusion && dims == 2 && x % 2 == 0 && p_alt; }
 
     int calc_w_pending(double* pw)
@@ -386,7 +390,7 @@ struct tlhip_problem {
 
     int cg_body(const tl::loop::StopRule& rule)
     {
-        if (dims == 2 && can_defer_p()) {
+        if (can_defer_p()) {
             TRY(calc_w_resident());
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
             pending_p = true;    // calc_p waits for the next calc_w (beta stays on the device)

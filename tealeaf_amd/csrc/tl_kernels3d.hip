@@ -248,6 +248,106 @@ k_jacobi_init3(TlGrid3 g, int coef, double rx, double ry, double rz,
     }
 }
 
+// calc_p fused into the next calc_w: p' = beta p + r (cg_solver.c:172-193)
+// formed at every point the 7-point stencil reads, with the depth-1 mirror
+// built in (a neighbour outside the interior takes the cell's own p'), then
+// w = A p' and pw += w p' (cg_solver.c:105-135); p' goes to a second
+// buffer.  The same expressions in the same order as cg_calc_p, a depth-1
+// halo update of p and cg_calc_w, so p' and w are theirs bit for bit.
+// Layout and z carry as k_stencil3d.
+template <int V>
+__global__ void __launch_bounds__(kBlock)
+k_calc_pw3d(TlGrid3 g, Tile3 t, double beta, const double* __restrict__ p, const double* __restrict__ r, double* __restrict__ p_out,
+            double* __restrict__ w, const double* __restrict__ kx, const double* __restrict__ ky,
+            const double* __restrict__ kz, TlReduce rd, TlGate gate)
+{
+    if (gate.closed()) return;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile(blockIdx.x, t.ns * t.nbj * t.nbk);
+    const int sx = tile % t.ns;
+    const int rest = tile / t.ns;
+    const int jb = rest % t.nbj, kb = rest / t.nbj;
+    const int j = g.hd + jb * kWaves + wave;
+    const int xend = g.x - g.hd;
+    const int c0 = g.hd + sx * (TL_WAVE * V) + lane * V;
+    int nval = xend - c0;
+    nval = nval < 0 ? 0 : (nval > V ? V : nval);
+    const bool ld_ok = c0 + V <= g.x;
+    const bool edge_r = lane == TL_WAVE - 1 && c0 + V < g.x;
+    const long X = g.x, XY = (long)g.x * g.y;
+    const int ylast = g.y - g.hd - 1, zfirst = g.hd, zlast = g.z - g.hd - 1;
+    auto pn = [&](long i) -> DV<V> {
+        DV<V> a = ldv<V>(p, i);
+        const DV<V> b = ldv<V>(r, i);
+#pragma unroll
+        for (int v = 0; v < V; ++v) a.v[v] = beta * a.v[v] + b.v[v];
+        return a;
+    };
+    double acc[1] = {0.0};
+
+    if (j < g.y - g.hd) {   // uniform across the wave
+        const long jrow = (long)j * X + c0;
+        const int k0 = g.hd + kb * t.tz;             // < g.z - g.hd: kb < nbk
+        const int k1 = min(k0 + t.tz, g.z - g.hd);
+        DV<V> a_dn = zdv<V>(), a_c = zdv<V>(), kz_c = zdv<V>();
+        if (ld_ok) {
+            a_c = pn((long)k0 * XY + jrow);
+            a_dn = k0 == zfirst ? a_c : pn((long)(k0 - 1) * XY + jrow);
+            kz_c = ldv<V>(kz, (long)k0 * XY + jrow);
+        }
+        for (int k = k0; k < k1; ++k) {
+            const long b = (long)k * XY + jrow;
+            Nb3<V> n;
+            double el = 0.0, er = 0.0, ek = 0.0;
+            n.c = a_c;
+            n.dn = a_dn;
+            n.kzc = kz_c;
+            if (ld_ok) {
+                n.up = k == zlast ? a_c : pn(b + XY);
+                n.no = j == ylast ? a_c : pn(b + X);
+                n.so = j == g.hd ? a_c : pn(b - X);
+                n.kxc = ldv<V>(kx, b);
+                n.kyc = ldv<V>(ky, b);
+                n.kyn = ldv<V>(ky, b + X);
+                n.kzu = ldv<V>(kz, b + XY);
+                if (lane == 0) el = beta * p[b - 1] + r[b - 1];
+            } else {
+                n.up = n.no = n.so = zdv<V>();
+                n.kxc = n.kyc = n.kyn = n.kzu = zdv<V>();
+            }
+            if (edge_r) {
+                er = beta * p[b + V] + r[b + V];
+                ek = kx[b + V];
+            }
+            const double al = tl_lane_prev(n.c.v[V - 1], el);
+            const double ar = tl_lane_next(n.c.v[0], er);
+            const double kr = tl_lane_next(n.kxc.v[0], ek);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                n.l.v[v] = v == 0 ? al : n.c.v[v - 1];
+                n.r.v[v] = v == V - 1 ? ar : n.c.v[v + 1];
+                n.kxr.v[v] = v == V - 1 ? kr : n.kxc.v[v + 1];
+                if (c0 + v == g.hd) n.l.v[v] = n.c.v[v];          // x mirror
+                if (c0 + v == xend - 1) n.r.v[v] = n.c.v[v];
+            }
+            if (nval > 0) {
+                DV<V> o;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    o.v[v] = apply_a(n, v);
+                    if (v < nval) acc[0] += o.v[v] * n.c.v[v];
+                }
+                stv<V>(w, b, o, nval);
+                stv<V>(p_out, b, n.c, nval);
+            }
+            a_dn = n.c;
+            a_c = n.up;
+            kz_c = n.kzu;
+        }
+    }
+    tl_reduce_partial<1, kBlock>(acc, rd);
+}
+
 // Halo ring (every cell outside the interior box) src -> dst.
 __global__ void __launch_bounds__(kBlock)
 k_copy_ring3d(TlGrid3 g, const double* __restrict__ src, double* __restrict__ dst, TlGate gate)
@@ -508,6 +608,35 @@ int copy_ring(const TlGrid3& g, const double* src, double* dst, const char* what
     return tl_check_launch(what);
 }
 
+// Ring copy p -> p_out and k_calc_pw3d under the current gate; *nb: its
+// partials.
+int launch_pw3d(const TlGrid3& g, double beta, const double* p,
+                double* p_out, const double* r, double* w, const double* kx, const double* ky,
+                const double* kz, const char* what, unsigned* nb)
+{
+    if (!(p && p_out && r && w && kx && ky && kz))
+        return tl_fail(TLHIP_ERR_ARG, "%s: null argument", what);
+    if (p_out == p || p_out == w || p_out == r || w == p)
+        return tl_fail(TLHIP_ERR_ARG, "%s: p_out and w must be distinct from the inputs", what);
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    if ((long)tl_cdiv(g.x - 2 * g.hd, TL_WAVE * 2) * tl_cdiv(g.y - 2 * g.hd, kWaves) >
+        TL_MAX_RED_BLOCKS)
+        return tl_fail(TLHIP_ERR_ARG, "%s: planes too large (%d x %d)", what, g.x, g.y);
+    TL_TRY(copy_ring(g, p, p_out, what));
+    const int V = pick_v(g, p, p_out, r, w, kx, ky, kz);
+    const Tile3 t = make_tile(g, V);
+    *nb = (unsigned)(t.ns * t.nbj * t.nbk);
+    const int slot = tl_timer_begin(what);
+    if (V == 2)
+        hipLaunchKernelGGL(k_calc_pw3d<2>, dim3(*nb), dim3(kBlock), 0, tl_stream(), g, t, beta,
+                           p, r, p_out, w, kx, ky, kz, tl_reduce_slots(), tl_gate());
+    else
+        hipLaunchKernelGGL(k_calc_pw3d<1>, dim3(*nb), dim3(kBlock), 0, tl_stream(), g, t, beta,
+                           p, r, p_out, w, kx, ky, kz, tl_reduce_slots(), tl_gate());
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
 #define REQUIRE TL_REQUIRE
 
 }  // namespace
@@ -702,6 +831,22 @@ extern "C" int tlhip3d_cg_solver_calc_w(int x, int y, int z, int hd, double* pw,
     if (interior_empty(g)) return 0;
     OpCalcW op{w};
     TL_TRY(launch_stencil(g, pick_v(g, p, w, kx, ky, kz), p, kx, ky, kz, op, what));
+    return fetch_add(pw, what);
+}
+
+extern "C" int tlhip3d_cg_solver_calc_pw(int x, int y, int z, int hd, double beta, double* pw,
+                                         const double* p_in, double* p_out, const double* r,
+                                         double* w, const double* kx, const double* ky,
+                                         const double* kz)
+{
+    const char* what = "tlhip3d_cg_solver_calc_pw";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(pw, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_pw3d(g, beta, p_in, p_out, r, w, kx, ky, kz, what, &nb));
+    TL_TRY(tl_finish_reduce(1, nb, what));
     return fetch_add(pw, what);
 }
 

@@ -325,3 +325,54 @@ def test_calc_pw_matches_calc_p_halo_calc_w(hip, oracle, grid, rows):
     assert abs(pw.value - pw_ref[0]) <= 1e-13 * max(terms, 1e-300)
     for v in list(d.values()) + [p_out, w]:
         v.free()
+
+
+@pytest.mark.parametrize("planes", [-1, 3])
+@pytest.mark.parametrize("grid", [(20, 13, 9, 2), (17, 10, 12, 1), (9, 7, 5, 3), (130, 6, 5, 2)],
+                         ids=["20x13x9", "17x10x12h1", "9x7x5h3", "130x6x5"])
+def test_calc_pw_3d_matches_calc_p_halo_calc_w(hip, oracle, grid, planes):
+    """tlhip3d_cg_solver_calc_pw against the oracle's cg_solver_calc_p, a
+    depth-1 halo update of p and cg_solver_calc_w: p_out and w bit-exact
+    (the mirror on all six faces, odd widths with one cell per lane, tiles
+    of 3 planes), the halo ring of p copied, pw to reduction order."""
+    import ctypes
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, nz, hd = grid
+    x, y, z = nx + 2 * hd, ny + 2 * hd, nz + 2 * hd
+    n = x * y * z
+    rng = np.random.default_rng(1900 + nx + ny)
+    p, r = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    kx, ky, kz = (rng.uniform(0.05, 0.3, n) for _ in range(3))
+    beta = 0.43
+    pw_ref = np.zeros(1)
+    p_ref, w_ref = p.copy(), np.zeros(n)
+    oracle.or3d_cg_solver_calc_p(x, y, z, hd, beta, ptr(p_ref), ptr(r))
+    nbr = np.full(6, -1, dtype=np.int32)
+    flags = np.zeros(6, dtype=np.bool_)
+    flags[4] = True                                   # p
+    dummy = np.zeros(n)
+    oracle.or3d_local_halos(x, y, z, 1, hd, ptr(nbr), ptr(flags), ptr(dummy), ptr(dummy),
+                            ptr(dummy), ptr(dummy), ptr(p_ref), ptr(dummy))
+    oracle.or3d_cg_solver_calc_w(x, y, z, hd, ptr(pw_ref), ptr(p_ref), ptr(w_ref), ptr(kx),
+                                 ptr(ky), ptr(kz))
+    d = {k: DeviceArray.from_host(v) for k, v in dict(p=p, r=r, kx=kx, ky=ky, kz=kz).items()}
+    p_out, w = DeviceArray.from_host(np.full(n, np.nan)), DeviceArray.from_host(np.full(n, np.nan))
+    pw = ctypes.c_double(0.0)
+    hip.tlhip_set_tile_planes(planes)
+    try:
+        hip.tlhip3d_cg_solver_calc_pw(x, y, z, hd, beta, ctypes.addressof(pw), d["p"].ptr,
+                                      p_out.ptr, d["r"].ptr, w.ptr, d["kx"].ptr, d["ky"].ptr,
+                                      d["kz"].ptr)
+    finally:
+        hip.tlhip_set_tile_planes(-1)
+    inner = (slice(hd, z - hd), slice(hd, y - hd), slice(hd, x - hd))
+    gp, gw = p_out.download().reshape(z, y, x), w.download().reshape(z, y, x)
+    np.testing.assert_array_equal(gp[inner], p_ref.reshape(z, y, x)[inner])
+    np.testing.assert_array_equal(gw[inner], w_ref.reshape(z, y, x)[inner])
+    ring = np.ones((z, y, x), dtype=bool)
+    ring[inner] = False
+    np.testing.assert_array_equal(gp[ring], p.reshape(z, y, x)[ring])
+    terms = np.sum(np.abs(w_ref.reshape(z, y, x)[inner] * p_ref.reshape(z, y, x)[inner]))
+    assert abs(pw.value - pw_ref[0]) <= 1e-13 * max(terms, 1e-300)
+    for v in list(d.values()) + [p_out, w]:
+        v.free()
