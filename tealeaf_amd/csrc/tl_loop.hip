@@ -176,15 +176,35 @@ void end() { g_loop.active = false; }
 const double* alpha() { return &g_loop.st->alpha; }
 const double* beta() { return &g_loop.st->beta; }
 
-int finish_alpha(unsigned nblocks, const char* what)
+namespace {
+// A finisher may be enqueued: a loop is armed, its iteration is in range and
+// the partials fit the reduction slots.
+int check_finisher(unsigned nblocks, const char* what)
 {
-    Host& h = g_loop;
+    const Host& h = g_loop;
     if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
     if (h.iter < 0 || h.iter >= h.cap)
         return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
     if (nblocks > TL_MAX_RED_BLOCKS)
         return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
                        TL_MAX_RED_BLOCKS);
+    return 0;
+}
+
+// Arm iteration j's report slot.  Its previous report (iteration j - kRing)
+// was read before this iteration could be enqueued.
+void arm_slot(int j)
+{
+    volatile unsigned long long* b = slot_bits(j);
+    b[0] = b[1] = b[2] = TL_MAIL_EMPTY;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+}
+}  // namespace
+
+int finish_alpha(unsigned nblocks, const char* what)
+{
+    Host& h = g_loop;
+    TL_TRY(check_finisher(nblocks, what));
     hipLaunchKernelGGL(k_loop_alpha, dim3(1), dim3(256), 0, tl_stream(), nblocks,
                        tl_reduce_slots().partials, h.st, h.alog, h.iter);
     return tl_check_launch(what);
@@ -193,17 +213,8 @@ int finish_alpha(unsigned nblocks, const char* what)
 int finish_beta(unsigned nblocks, const StopRule& rule, const char* what)
 {
     Host& h = g_loop;
-    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
-    if (h.iter < 0 || h.iter >= h.cap)
-        return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
-    if (nblocks > TL_MAX_RED_BLOCKS)
-        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
-                       TL_MAX_RED_BLOCKS);
-    // The slot's previous report (iteration iter - kRing) was read before
-    // this iteration could be enqueued.
-    volatile unsigned long long* b = slot_bits(h.iter);
-    b[0] = b[1] = b[2] = TL_MAIL_EMPTY;
-    std::atomic_thread_fence(std::memory_order_seq_cst);
+    TL_TRY(check_finisher(nblocks, what));
+    arm_slot(h.iter);
     hipLaunchKernelGGL(k_loop_beta, dim3(1), dim3(256), 0, tl_stream(), nblocks,
                        tl_reduce_slots().partials, h.st, h.blog, h.ring, h.iter, rule);
     return tl_check_launch(what);
@@ -212,17 +223,8 @@ int finish_beta(unsigned nblocks, const StopRule& rule, const char* what)
 int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char* what)
 {
     Host& h = g_loop;
-    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
-    if (h.iter < 0 || h.iter >= h.cap)
-        return tl_fail(TLHIP_ERR_ARG, "%s: iteration %d outside the loop's %d", what, h.iter, h.cap);
-    if (nblocks > TL_MAX_RED_BLOCKS)
-        return tl_fail(TLHIP_ERR_ARG, "%s: %u reduction blocks exceed %d", what, nblocks,
-                       TL_MAX_RED_BLOCKS);
-    if (mode != kErrSet) {
-        volatile unsigned long long* b = slot_bits(h.iter);
-        b[0] = b[1] = b[2] = TL_MAIL_EMPTY;
-        std::atomic_thread_fence(std::memory_order_seq_cst);
-    }
+    TL_TRY(check_finisher(nblocks, what));
+    if (mode != kErrSet) arm_slot(h.iter);     // kErrSet does not report
     hipLaunchKernelGGL(k_loop_err, dim3(1), dim3(256), 0, tl_stream(), nblocks,
                        tl_reduce_slots().partials, h.st, h.ring, h.iter, (int)mode, rule);
     return tl_check_launch(what);
