@@ -389,6 +389,14 @@ int tlhip_set_cg_fusion(int on);
  * kernels of an iteration.  Default on.  No counterpart in the reference:
  * its drivers take pw and rrn to the host (2d/drivers/cg_driver.c:78-99). */
 int tlhip_set_resident(int on);
+/* Whether the resident loops on small grids (up to 2.5M cells with halo,
+ * where an iteration is a few launches of a few microseconds) replay their
+ * iterations from a HIP graph: after the first iteration of a run, two
+ * iterations are captured once and replayed with one host call per two
+ * iterations.  Results are identical either way.  Default off: on MI355X
+ * (ROCm 7.2) the replays measured 5-9% slower than launch-by-launch
+ * enqueueing.  No counterpart in the reference. */
+int tlhip_set_graphs(int on);
 /* Steps per fused PPCG launch used by the driver (1..10; 0 or 1 = one step
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 10: the
  * reference deck's 10 inner steps in one launch (FP64, and FP32 in the

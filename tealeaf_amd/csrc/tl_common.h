@@ -49,7 +49,11 @@ struct TlReduce {
 struct TlGate {
     const int* last;   // null: always run
     int iter;
-    __device__ __forceinline__ bool closed() const { return last && iter > *last; }
+    const int* base = nullptr;   // graph replays: iter counts from *base
+    __device__ __forceinline__ bool closed() const
+    {
+        return last && iter + (base ? *base : 0) > *last;
+    }
 };
 // The gate that launches issued now should carry (null outside a loop).
 TlGate tl_gate();
@@ -70,6 +74,8 @@ int tl_ensure_init();
 // Kernel timer hooks: begin returns a slot (or -1 when `what` is not being
 // timed); end records the closing event for that slot.
 int tl_timer_begin(const char* what);
+// Whether the kernel timer is recording (tlhip_kernel_timer_start).
+bool tl_timer_active();
 void tl_timer_end(int slot);
 
 #define TL_TRY(expr)                      \

@@ -68,6 +68,18 @@ int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char*
 int wait(int j, Report* out);
 // alpha and beta of iterations j0 .. j0+n-1 into host arrays.
 int logs(int j0, int n, double* alphas, double* betas);
+// Graph replays of a periodic body (the same launches every `period`
+// iterations, field pointers back where they started): capture_begin()
+// records the launches issued until capture_end(period) into a graph whose
+// iteration numbers count from a device base; set_base(j0) puts the base
+// at the first replayed iteration, and replay(j) arms the reports of
+// iterations j .. j+period-1 and launches the graph, which ends by moving
+// the base on by `period`.  drop_graph() frees it.
+int capture_begin();
+int capture_end(int period);
+int set_base(int j0);
+int replay(int j);
+void drop_graph();
 // Free the loop's device and pinned memory (tlhip_finalise).
 void release();
 

@@ -19,7 +19,7 @@ struct State {
     double rro, alpha, beta;
     double err;          // the Jacobi error of the current iteration
     int last;            // last iteration to run; INT_MAX while the loop goes on
-    int pad;
+    int base;            // graph replays: the iteration their first one is
 };
 
 // One iteration's report in pinned host memory.  The host arms the three
@@ -41,6 +41,9 @@ struct Host {
     Slot* ring = nullptr;          // pinned, host-mapped
     bool active = false;
     int iter = 0;
+    bool capturing = false;        // launches go into a graph (iter is relative)
+    hipGraphExec_t exec = nullptr;
+    int period = 0;
 };
 
 Host g_loop;
@@ -50,7 +53,13 @@ __global__ void __launch_bounds__(64) k_loop_begin(State* st, double rro)
     if (threadIdx.x == 0) {
         st->rro = rro;
         st->last = INT_MAX;
+        st->base = 0;
     }
+}
+
+__global__ void __launch_bounds__(64) k_loop_base(State* st, int j0, int add)
+{
+    if (threadIdx.x == 0) st->base = add ? st->base + j0 : j0;
 }
 
 // The stop rule and the report of iteration j (thread 0 of a finisher).
@@ -71,8 +80,9 @@ __device__ void test_and_report(State* st, Slot* ring, int j, double rrn, const 
 // alpha = rro / pw of iteration j (cg_driver.c:88-90, ppcg_driver.c:102-104)
 __global__ void __launch_bounds__(256)
 k_loop_alpha(unsigned nblocks, const double* __restrict__ partials, State* st, double* alog,
-             int j)
+             int j, int rel)
 {
+    if (rel) j += st->base;
     if (j > st->last) return;                 // uniform: the whole block returns
     double pw[1];
     tl_sum_partials<1>(nblocks, partials, pw);
@@ -88,8 +98,9 @@ k_loop_alpha(unsigned nblocks, const double* __restrict__ partials, State* st, d
 // ppcg_driver.c); the report goes to ring slot j % kRing.
 __global__ void __launch_bounds__(256)
 k_loop_beta(unsigned nblocks, const double* __restrict__ partials, State* st, double* blog,
-            Slot* ring, int j, StopRule rule)
+            Slot* ring, int j, int rel, StopRule rule)
 {
+    if (rel) j += st->base;
     if (j > st->last) return;
     double tot[1];
     tl_sum_partials<1>(nblocks, partials, tot);
@@ -106,8 +117,9 @@ k_loop_beta(unsigned nblocks, const double* __restrict__ partials, State* st, do
 // residual norm of every 50th iteration, added (calculate_2norm's +=).
 __global__ void __launch_bounds__(256)
 k_loop_err(unsigned nblocks, const double* __restrict__ partials, State* st, Slot* ring, int j,
-           int mode, StopRule rule)
+           int rel, int mode, StopRule rule)
 {
+    if (rel) j += st->base;
     if (j > st->last) return;
     double tot[1];
     tl_sum_partials<1>(nblocks, partials, tot);
@@ -206,7 +218,7 @@ int finish_alpha(unsigned nblocks, const char* what)
     Host& h = g_loop;
     TL_TRY(check_finisher(nblocks, what));
     hipLaunchKernelGGL(k_loop_alpha, dim3(1), dim3(256), 0, tl_stream(), nblocks,
-                       tl_reduce_slots().partials, h.st, h.alog, h.iter);
+                       tl_reduce_slots().partials, h.st, h.alog, h.iter, (int)h.capturing);
     return tl_check_launch(what);
 }
 
@@ -214,9 +226,10 @@ int finish_beta(unsigned nblocks, const StopRule& rule, const char* what)
 {
     Host& h = g_loop;
     TL_TRY(check_finisher(nblocks, what));
-    arm_slot(h.iter);
+    if (!h.capturing) arm_slot(h.iter);       // a replay arms its own
     hipLaunchKernelGGL(k_loop_beta, dim3(1), dim3(256), 0, tl_stream(), nblocks,
-                       tl_reduce_slots().partials, h.st, h.blog, h.ring, h.iter, rule);
+                       tl_reduce_slots().partials, h.st, h.blog, h.ring, h.iter,
+                       (int)h.capturing, rule);
     return tl_check_launch(what);
 }
 
@@ -224,9 +237,10 @@ int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char*
 {
     Host& h = g_loop;
     TL_TRY(check_finisher(nblocks, what));
-    if (mode != kErrSet) arm_slot(h.iter);     // kErrSet does not report
+    if (mode != kErrSet && !h.capturing) arm_slot(h.iter);     // kErrSet does not report
     hipLaunchKernelGGL(k_loop_err, dim3(1), dim3(256), 0, tl_stream(), nblocks,
-                       tl_reduce_slots().partials, h.st, h.ring, h.iter, (int)mode, rule);
+                       tl_reduce_slots().partials, h.st, h.ring, h.iter, (int)h.capturing,
+                       (int)mode, rule);
     return tl_check_launch(what);
 }
 
@@ -264,6 +278,72 @@ int wait(int j, Report* out)
     return 0;
 }
 
+int capture_begin()
+{
+    const char* what = "tl::loop::capture_begin";
+    Host& h = g_loop;
+    if (!h.active) return tl_fail(TLHIP_ERR_ARG, "%s: no resident loop is armed", what);
+    drop_graph();
+    LOOP_HIP(hipStreamBeginCapture(tl_stream(), hipStreamCaptureModeRelaxed), what);
+    h.capturing = true;
+    return 0;
+}
+
+int capture_end(int period)
+{
+    const char* what = "tl::loop::capture_end";
+    Host& h = g_loop;
+    if (!h.capturing) return tl_fail(TLHIP_ERR_ARG, "%s: no capture in progress", what);
+    hipLaunchKernelGGL(k_loop_base, dim3(1), dim3(64), 0, tl_stream(), h.st, period, 1);
+    const hipError_t le = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(tl_stream(), &graph);
+    h.capturing = false;
+    if (le != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(le, what);
+    }
+    if (e != hipSuccess) return hip_fail(e, what);
+    const hipError_t ie = hipGraphInstantiateWithFlags(&h.exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        h.exec = nullptr;
+        return hip_fail(ie, what);
+    }
+    h.period = period;
+    return 0;
+}
+
+int set_base(int j0)
+{
+    hipLaunchKernelGGL(k_loop_base, dim3(1), dim3(64), 0, tl_stream(), g_loop.st, j0, 0);
+    return tl_check_launch("tl::loop::set_base");
+}
+
+int replay(int j)
+{
+    const char* what = "tl::loop::replay";
+    Host& h = g_loop;
+    if (!h.exec) return tl_fail(TLHIP_ERR_ARG, "%s: no graph", what);
+    if (j < 0 || j + h.period > h.cap)
+        return tl_fail(TLHIP_ERR_ARG, "%s: iterations %d.. outside the loop's %d", what, j, h.cap);
+    for (int q = 0; q < h.period; ++q) arm_slot(j + q);
+    LOOP_HIP(hipGraphLaunch(h.exec, tl_stream()), what);
+    return 0;
+}
+
+void drop_graph()
+{
+    Host& h = g_loop;
+    if (h.exec) {
+        // replays of it may still be queued (gated iterations past the stop)
+        (void)hipStreamSynchronize(tl_stream());
+        (void)hipGraphExecDestroy(h.exec);
+    }
+    h.exec = nullptr;
+    h.period = 0;
+}
+
 int logs(int j0, int n, double* alphas, double* betas)
 {
     const char* what = "tl::loop::logs";
@@ -281,6 +361,7 @@ int logs(int j0, int n, double* alphas, double* betas)
 void release()
 {
     Host& h = g_loop;
+    drop_graph();
     if (h.st) (void)hipFree(h.st);
     if (h.alog) (void)hipFree(h.alog);
     if (h.blog) (void)hipFree(h.blog);
@@ -295,5 +376,6 @@ TlGate tl_gate()
 {
     const tl::loop::Host& h = tl::loop::g_loop;
     if (!h.active) return TlGate{nullptr, 0};
+    if (h.capturing) return TlGate{&h.st->last, h.iter, &h.st->base};
     return TlGate{&h.st->last, h.iter};
 }

@@ -241,6 +241,8 @@ extern "C" int tlhip_copy_device(void* dst, const void* src, size_t bytes)
 }
 
 // ------------------------------------------------------------ kernel timer
+bool tl_timer_active() { return g_timer.on != 0; }
+
 int tl_timer_begin(const char* what)
 {
     if (!g_timer.on || strcmp(what, g_timer.name) != 0) return -1;

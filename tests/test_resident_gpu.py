@@ -182,6 +182,36 @@ def test_jacobi_blocks(hip):
     _same(hip, text, dims=2, inject=inject, extra=("r",))
 
 
+@pytest.mark.parametrize("dims,text", [
+    (2, tdeck.layered_2d(64, 2, solver=tdeck.CG, max_iters=3000)),
+    (2, tdeck.layered_2d(64, 2, solver=tdeck.CHEBY, presteps=20, max_iters=3000)),
+    (2, tdeck.layered_2d(64, 2, solver=tdeck.PPCG, presteps=20, inner=10, max_iters=3000)),
+    (2, tdeck.layered_2d(64, 1, solver=tdeck.PPCG, presteps=20, inner=12, max_iters=41)),
+    (3, tdeck.cuboid_3d(20, 2, solver=tdeck.CG, max_iters=2000)),
+    (3, tdeck.cuboid_3d(20, 2, solver=tdeck.PPCG, presteps=15, max_iters=2000)),
+], ids=["2d-cg", "2d-cheby", "2d-ppcg", "2d-ppcg-inner12-cut41", "3d-cg", "3d-ppcg"])
+def test_graph_replays_match_eager(hip, dims, text):
+    """Small grids can replay pairs of resident iterations from a HIP
+    graph (tlhip_set_graphs; off by default); otherwise the same iterations
+    are enqueued launch by launch.  Every report, u, energy, r and the summary
+    must be the same bits; each solve replays the graph hundreds of times
+    with the host's waits and the loop's own eager launches in between."""
+    runs = []
+    for on in (0, 1):
+        hip.tlhip_set_graphs(on)
+        try:
+            runs.append(_run(hip, text, 1, dims, extra=("r", "p")))
+        finally:
+            hip.tlhip_set_graphs(0)
+    a, b = runs
+    assert len(a[0]) == len(b[0])
+    for ra, rb in zip(a[0], b[0]):
+        assert ra == rb, (ra, rb)
+    for fa, fb in zip(a[1], b[1]):
+        np.testing.assert_array_equal(fa, fb)
+    assert a[2] == b[2]
+
+
 def test_set_resident_returns_zero(hip):
     assert hip.dll.tlhip_set_resident(0) == 0
     assert hip.dll.tlhip_set_resident(1) == 0
