@@ -403,6 +403,7 @@ struct tlhip_problem {
             TRY(calc_w_resident());
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
             pending_p = true;    // calc_p waits for the next calc_w (beta stays on the device)
+            return 0;            // u and p are read pointwise or mirrored: halo at the end
         } else if (dims == 2) {
             TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
@@ -438,8 +439,11 @@ struct tlhip_problem {
         reset_fields();
         fields[F_P] = true;
         TRY(tl::loop2d::norm(x, y, hd, r, rule));
-        if (can_defer_p()) pending_p = true;
-        else TRY(tl::loop2d::calc_p(x, y, hd, p, r));
+        if (can_defer_p()) {
+            pending_p = true;
+            return 0;            // p is read pointwise or mirrored: halo at the end
+        }
+        TRY(tl::loop2d::calc_p(x, y, hd, p, r));
         return halo_update(1);
     }
 
@@ -542,7 +546,7 @@ struct tlhip_problem {
 
     template <class Body>
     int run_resident(int t0, int sw, bool cg, Body&& body, double* rro, double* error, int* last,
-                     tl::loop::Report* out, bool periodic = true)
+                     tl::loop::Report* out, bool periodic = true, bool halo_after = false)
     {
         if (t0 < 0 || t0 >= s.max_iters)
             return tl_fail(TLHIP_ERR_ARG, "resident loop: no iteration to run from %d", t0);
@@ -593,6 +597,12 @@ struct tlhip_problem {
         const Ptrs& q = snap[chk % kSnap];
         u = q.u, u_alt = q.u_alt, r = q.r, r_alt = q.r_alt, sd = q.sd, sd_alt = q.sd_alt;
         p = q.p, p_alt = q.p_alt;
+        if (halo_after) {
+            // the depth-1 halo update that each iteration of the reference
+            // ends with, once: the values depend only on the last iteration
+            tl::loop::end();
+            TRY(halo_update(1));
+        }
         if (cg) TRY(tl::loop::logs(t0, chk - t0 + 1, &cg_alphas[t0], &cg_betas[t0]));
         if (pending_p) {
             // the last iteration's calc_p waits with its beta (host copy for flush_p)
@@ -632,7 +642,7 @@ struct tlhip_problem {
             tl::loop::Report rp;
             TRY(run_resident(from, sw, true,
                              [&](const tl::loop::StopRule& rule, int) { return cg_body(rule); }, rro,
-                             error, &last, &rp));
+                             error, &last, &rp, true, can_defer_p()));
             rep->n_cg_steps += last - from + 1;
             *tt = last;
             TRY(confirm(rp, done));
@@ -919,7 +929,7 @@ struct tlhip_problem {
                                      [&](const tl::loop::StopRule& rule, int) {
                                          return ppcg_body(rule);
                                      },
-                                     &rro, error, &last, &rp));
+                                     &rro, error, &last, &rp, true, dims == 2 && can_defer_p()));
                     const int ran = last - tt + 1;
                     rep->n_ppcg_outer += ran;
                     rep->n_ppcg_inner += ran * s.ppcg_inner_steps;

@@ -1034,12 +1034,13 @@ int cheby_iterations(int x, int y, int hd, int count, const double* alphas, cons
     }
 }
 
-// Checks, the ring copy p_in -> p_out (the halo the reference's p keeps)
-// and the k_calc_pw launch under the current gate; *nb: its partials.
+// Checks, the ring copy p_in -> p_out (the halo the reference's p keeps;
+// skipped with ring = false) and the k_calc_pw launch under the current
+// gate; *nb: its partials.
 template <class T>
 int launch_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
               const double* beta_dev, const T* p_in, T* p_out, const T* r, T* w, const T* kx,
-              const T* ky, unsigned* nb)
+              const T* ky, unsigned* nb, bool ring = true)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
@@ -1062,7 +1063,7 @@ int launch_pw(const char* what, int x, int y, int hd, int pitch, bool update, do
     while ((long)nbx * tl_cdiv(g.ny, ty) > TL_MAX_RED_BLOCKS) ty *= 2;
     const int nby = (int)tl_cdiv(g.ny, ty);
     *nb = (unsigned)(nbx * nby);
-    if (update) {
+    if (update && ring) {
         Ring3<T> rg{};
         rg.src[0] = p_in;
         rg.dst[0] = p_out;
@@ -1130,6 +1131,9 @@ int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const doub
 }
 
 // calc_p with the loop's beta fused into calc_w, pw finished into alpha.
+// p_out's halo is left as it is: in the resident loop nothing reads p's
+// halo until the loop's closing halo update refreshes it (the fused passes
+// mirror, the deeper rings of p and its partner are the same zeros).
 int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const double* r, double* w,
             const double* kx, const double* ky)
 {
@@ -1137,7 +1141,7 @@ int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const doubl
     if (x % 2) return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x (got %d)", what, x);
     unsigned nb = 0;
     TL_TRY(launch_pw<double>(what, x, y, hd, x, true, 0.0, loop::beta(), p_in, p_out, r, w, kx,
-                             ky, &nb));
+                             ky, &nb, false));
     return loop::finish_alpha(nb, what);
 }
 
