@@ -61,6 +61,17 @@ def test_2d_layered(hip, solver):
     assert all(r["printed_cg_iters"] > 0 for r in reps)
 
 
+@pytest.mark.parametrize("solver", [tdeck.CG, tdeck.CHEBY, tdeck.PPCG, tdeck.JACOBI])
+def test_2d_odd_width(hip, solver):
+    """An odd row length (77 + 4 halo cells): one cell per lane, no fused
+    calc_p + calc_w and no fused PPCG or Chebyshev loop, so the resident CG
+    body runs calc_w, calc_ur, calc_p and a halo update per iteration, and
+    the PPCG outer loop stays host-driven."""
+    eps = 1e-8 if solver == tdeck.JACOBI else 1e-15
+    _same(hip, tdeck.layered_2d(77, 2, solver=solver, presteps=20, eps=eps, max_iters=3000),
+          extra=("r",))
+
+
 @pytest.mark.parametrize("solver", [tdeck.CHEBY, tdeck.PPCG])
 def test_2d_error_switch(hip, solver):
     _same(hip, tdeck.layered_2d(80, 2, solver=solver, max_iters=3000, epslim=1e-3))
