@@ -52,10 +52,11 @@ int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 // Replay the resident loop's iterations from a HIP graph on small grids
 // (up to kGraphCells cells with halo, where an iteration is a handful of
 // launches of a few microseconds).  Off by default: on MI355X with ROCm 7.2
-// the replays measured 5-9% slower per iteration than launch-by-launch
-// enqueueing (512^2 CG 0.0478 against 0.0455 ms, 1024^2 CG 0.0625 against
-// 0.0584, 128^3 CG 0.0617 against 0.0580): the loop is bound by each
-// kernel's dispatch and drain on the device, not by the host.
+// the replays measured 5-9% slower than launch-by-launch enqueueing (wall
+// time per iteration of a timestep, setup included: 512^2 CG 0.0478 against
+// 0.0455 ms, 1024^2 CG 0.0625 against 0.0584, 128^3 CG 0.0617 against
+// 0.0580): the loop is bound by each kernel's dispatch and drain on the
+// device, not by the host.
 int g_graphs = 0;
 constexpr long kGraphCells = 2500000;
 
