@@ -70,6 +70,9 @@ int tl_tile_rows_override();
 int tl_interior_blocks();
 // Wait for the stream and copy `n` published reduction values out.
 int tl_fetch(int n, double* out, const char* what);
+// Longest a host wait for a device result may take before it fails
+// (seconds; tlhip_set_wait_limit).
+double tl_wait_limit_s();
 int tl_ensure_init();
 // Kernel timer hooks: begin returns a slot (or -1 when `what` is not being
 // timed); end records the closing event for that slot.

@@ -247,20 +247,34 @@ int finish_err(unsigned nblocks, ErrMode mode, const StopRule& rule, const char*
 // Spin on the slot.  Every millisecond the stream is queried: an error
 // surfaces there, and an idle stream with no report means the iteration
 // was never run (its gate was closed), which the callers rule out by
-// waiting only for iterations after the last one that reported.
+// waiting only for iterations after the last one that reported.  A stream
+// still busy after the wait limit (tlhip_set_wait_limit) fails the call
+// with the iteration it waited for and the last report it saw, rather
+// than spinning for ever on a kernel that does not finish.
 int wait(int j, Report* out)
 {
     const char* what = "tl::loop::wait";
+    static int last_j = -1;
+    static Report last{};
     volatile unsigned long long* b = slot_bits(j);
     if (!slot_ready(b)) {
-        auto t0 = std::chrono::steady_clock::now();
+        const auto start = std::chrono::steady_clock::now();
+        auto t0 = start;
         for (unsigned spin = 1; !slot_ready(b); ++spin) {
             if ((spin & 1023u) != 0) continue;
             const auto now = std::chrono::steady_clock::now();
             if (now - t0 < std::chrono::milliseconds(1)) continue;
             t0 = now;
             const hipError_t e = hipStreamQuery(tl_stream());
-            if (e == hipErrorNotReady) continue;
+            if (e == hipErrorNotReady) {
+                const double waited = std::chrono::duration<double>(now - start).count();
+                if (waited > tl_wait_limit_s())
+                    return tl_fail(TLHIP_ERR_HIP,
+                                   "%s: iteration %d has not reported after %.0f s (wait limit; "
+                                   "last report: iteration %d, rrn %.17g, error %.17g)",
+                                   what, j, waited, last_j, last.rrn, last.err);
+                continue;
+            }
             if (e != hipSuccess) return hip_fail(e, what);
             if (!slot_ready(b))
                 return tl_fail(TLHIP_ERR_HIP, "%s: iteration %d did not report", what, j);
@@ -275,6 +289,8 @@ int wait(int j, Report* out)
     out->rrn = v[0];
     out->err = v[1];
     out->flags = (int)v[2];
+    last_j = j;
+    last = *out;
     return 0;
 }
 

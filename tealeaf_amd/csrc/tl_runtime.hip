@@ -41,6 +41,8 @@ struct KernelTimer {
 
 Runtime g_rt;
 KernelTimer g_timer;
+// Longest the host waits for a result the device owes it (tlhip_set_wait_limit).
+double g_wait_limit_s = 300.0;
 thread_local char g_err[512] = "";
 
 }  // namespace
@@ -111,8 +113,9 @@ int tl_check_launch(const char* what)
 // and the drivers wait for two scalars per CG iteration.  Every finisher
 // is fetched before the next is launched, so a slot is rewritten only
 // after its last value was read.  A finisher that does not report within
-// 2 ms falls back to synchronising the stream, which also surfaces any
-// launch error.
+// 2 ms falls back to polling the stream, which also surfaces any launch
+// error; a stream still busy after the wait limit is reported as an error
+// instead of blocking the caller for ever.
 static volatile unsigned long long* mail_slot(unsigned long long seq)
 {
     return (volatile unsigned long long*)(g_rt.red.mailbox +
@@ -126,20 +129,31 @@ static bool mail_ready(volatile unsigned long long* m, int n)
     return true;
 }
 
+double tl_wait_limit_s() { return g_wait_limit_s; }
+
 int tl_fetch(int n, double* out, const char* what)
 {
     volatile unsigned long long* m = mail_slot(g_rt.seq);
     if (!mail_ready(m, n)) {
         const auto t0 = std::chrono::steady_clock::now();
         for (unsigned spin = 1; !mail_ready(m, n); ++spin) {
-            if ((spin & 1023u) == 0 &&
-                std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
-                hipError_t e = hipStreamSynchronize(g_rt.stream);
-                if (e != hipSuccess) return hip_fail(e, what);
-                if (!mail_ready(m, n))
-                    return tl_fail(TLHIP_ERR_HIP, "%s: reduction result not published", what);
-                break;
+            if ((spin & 1023u) != 0) continue;
+            const double waited =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (waited < 2e-3) continue;
+            const hipError_t e = hipStreamQuery(g_rt.stream);
+            if (e == hipErrorNotReady) {
+                if (waited > g_wait_limit_s)
+                    return tl_fail(TLHIP_ERR_HIP,
+                                   "%s: reduction result not published after %.0f s (stream "
+                                   "still busy; wait limit %.0f s, tlhip_set_wait_limit)",
+                                   what, waited, g_wait_limit_s);
+                continue;
             }
+            if (e != hipSuccess) return hip_fail(e, what);
+            if (!mail_ready(m, n))
+                return tl_fail(TLHIP_ERR_HIP, "%s: reduction result not published", what);
+            break;
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
@@ -147,6 +161,13 @@ int tl_fetch(int n, double* out, const char* what)
         const unsigned long long bits = m[i];
         memcpy(&out[i], &bits, sizeof(double));
     }
+    return 0;
+}
+
+extern "C" int tlhip_set_wait_limit(double seconds)
+{
+    if (seconds != seconds) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_wait_limit: NaN");
+    g_wait_limit_s = seconds > 0.0 ? seconds : 300.0;
     return 0;
 }
 
