@@ -62,6 +62,12 @@ int tlhip_finalise(void);
 int tlhip_device(void);
 const char* tlhip_last_error(void);
 int tlhip_synchronize(void);
+/* Longest a host-side wait for a device result (a reduction scalar, a
+ * resident loop's iteration report) may take before the call fails with
+ * TLHIP_ERR_HIP and a message naming what it waited for; <= 0 restores
+ * the default of 300 s.  No reference counterpart: the reference's
+ * drivers block in their device synchronisation. */
+int tlhip_set_wait_limit(double seconds);
 /* Run the suite's work on a caller-owned stream (e.g. a framework's). */
 int tlhip_set_stream(void* stream);
 void* tlhip_get_stream(void);
@@ -214,6 +220,15 @@ int tlhip2d_cheby_iterations(int x, int y, int halo_depth, int count,
  * once, from the kernel's occupancy and the CU count) and prefetch distance
  * in rows (0-3; < 0: default 1; the Chebyshev loop uses 0 or 1). */
 int tlhip_set_fused_tuning(int tile_rows, int prefetch);
+/* Which kernel the fused PPCG loop launches: 0 (default) keeps each
+ * row's diagonal, kx and ky in a per-wave LDS ring (prefetch fixed at one
+ * row); 1 keeps them in register windows (the prefetch setting applies).
+ * Results are bit-identical. */
+int tlhip_set_fused_variant(int variant);
+/* Tuning of the LDS-ring PPCG kernel: its tiles (one wave each) number
+ * about fill x the waves the device holds at once (<= 0: default 0.95,
+ * measured best at 4096^2 among 0.88, 0.95 and 1.0). */
+int tlhip_set_ring_fill(double fill);
 /* 2d/kernel_interface.h:64 run_copy_u */
 int tlhip2d_copy_u(int x, int y, int halo_depth, double* u0, const double* u);
 /* 2d/kernel_interface.h:66 run_calculate_residual */

@@ -27,32 +27,42 @@
 // the fly from r, and r, u at level 0 may first take cg_calc_ur's update
 // u += alpha p, r -= alpha w (cg.c:116-143; the PPCG outer step that
 // precedes the inner loop), also on the fly.
-#include "tl_lanes.h"
+#include "tl_fused.h"
 #include "tl_mixed.h"
 #include "tl_loop.h"
-#include "tl_ops.h"
 #include "tealeaf_hip.h"
 
 #include <algorithm>
 #include <type_traits>
 
 using namespace tl;
+using namespace tl::fused;
+
+namespace tl {
+namespace fused {
+
+int g_fused_ty = -1;       // output rows per tile (<= 0: default)
+int g_fused_pf = -1;       // row prefetch (< 0: default)
+int g_fused_variant = 0;   // 0: LDS-ring PPCG kernel; 1: register windows
+
+// Compute units of the current device (cached; the runtime uses one device).
+int device_cus()
+{
+    static int cus = 0;
+    if (cus <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+}  // namespace fused
+}  // namespace tl
 
 namespace {
-
-constexpr int kMaxFused = 10;
-
-// Raw buffer descriptors (gfx9 family): word 3 of a descriptor with no
-// format conversion, the largest range a descriptor is given here, and the
-// offset that marks a lane's load as outside it (returns zeros).
-constexpr int kBufferWord3 = 0x00020000;
-constexpr unsigned kBufferRange = 0xFFFFFF00u;
-constexpr unsigned kBufferOutside = 0xFFFFFFF0u;
-
-template <class T, int S>
-struct PpcgCoefs {
-    T alpha[S], beta[S];
-};
 
 // Level 0 of the loop: MODE 0 reads sd; 1 forms sd = r/theta (ppcg_init);
 // 2 first applies cg_calc_ur to r and u, then forms sd = r/theta.
@@ -780,43 +790,6 @@ k_copy_rings2d(PitchGrid g, int rows, Ring3<T> rg, TlGate gate)
     }
 }
 
-int g_fused_ty = -1;   // output rows per tile (<= 0: default)
-int g_fused_pf = -1;   // row prefetch (< 0: default)
-
-template <class T>
-struct FusedArgs {
-    PitchGrid g;
-    const double* al;
-    const double* be;
-    double theta, alpha_ur;
-    const double* alpha_dev;   // null: alpha_ur
-    const T* sd_in;
-    T* sd_out;
-    const T* r_in;
-    T* r_out;
-    const T* u_in;
-    T* u_out;
-    const T* kx;
-    const T* ky;
-    const T* p;
-    const T* w;
-    const char* what;
-};
-
-// Compute units of the current device (cached; the runtime uses one device).
-int device_cus()
-{
-    static int cus = 0;
-    if (cus <= 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            cus <= 0)
-            cus = 256;
-    }
-    return cus;
-}
-
 // Default tile height: as many row tiles as fill one round of the blocks
 // the device holds at once.  A wave marches its strip down the tile, and
 // the S-1 rows above and below it are recomputed by the neighbouring
@@ -888,6 +861,7 @@ int launch_fused(const FusedArgs<T>& a)
 template <class T, int MODE>
 int dispatch_fused(int count, const FusedArgs<T>& a)
 {
+    if (g_fused_variant == 0) return launch_ppcg_ring<T>(count, MODE, a);
     switch (count) {
     case 1: return launch_fused<T, 1, MODE>(a);
     case 2: return launch_fused<T, 2, MODE>(a);
@@ -1182,6 +1156,14 @@ extern "C" int tlhip_set_fused_tuning(int tile_rows, int prefetch)
 {
     g_fused_ty = tile_rows;
     g_fused_pf = prefetch;
+    return 0;
+}
+
+extern "C" int tlhip_set_fused_variant(int variant)
+{
+    if (variant < 0 || variant > 1)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_set_fused_variant: 0 or 1 (got %d)", variant);
+    g_fused_variant = variant;
     return 0;
 }
 
