@@ -57,17 +57,24 @@ def parse_args():
     ap.add_argument("--solvers", default="cg,cheby,ppcg")
     ap.add_argument("--ppcg-fusion", type=int, default=10,
                     help="PPCG inner steps per fused launch (1 = one launch per step)")
-    ap.add_argument("--cpu-iters", type=int, default=300,
-                    help="CG iterations of the bounded CPU reference sample")
+    ap.add_argument("--cpu-cg-iters", type=int, default=1020,
+                    help="CG phase of the bounded CPU reference samples (must reach the switch "
+                         "to Chebyshev/PPCG: 1018 iterations at 4096^2, presteps 30)")
+    ap.add_argument("--cpu-cheby-iters", type=int, default=230,
+                    help="Chebyshev iterations of the CPU sample after the switch")
+    ap.add_argument("--cpu-ppcg-outer", type=int, default=20,
+                    help="PPCG outer iterations of the CPU sample after the switch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--refine-cells", type=int, default=8192,
                     help="grid of the FP32-refined vs FP64 time-to-1e-10 comparison (0 = skip)")
-    ap.add_argument("--refine-steps", type=int, default=2)
+    ap.add_argument("--refine-steps", type=int, default=5)
     ap.add_argument("--refine-only", action="store_true",
                     help="run only the FP32-refined vs FP64 comparison and print its JSON")
     ap.add_argument("--blocks3d", action="store_true",
                     help="run only the 3D 256^3 seeded block-density PPCG workload and print its JSON")
     ap.add_argument("--blocks3d-steps", type=int, default=2)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint (forward+backward) timing")
     ap.add_argument("--profile-run", action="store_true",
                     help="counter-collection run: a few launches of each hot kernel at --cells "
                          "and of a copy kernel of known size, then exit")
@@ -258,54 +265,148 @@ def dominant(args) -> str:
     return "tlhip2d_ppcg_inner_iterations" if args.ppcg_fusion > 1 else "tlhip2d_ppcg_inner_iteration"
 
 
-def cpu_baseline(cells: int, iters: int):
+def cpu_info() -> dict:
+    """CPU model, the physical cores in this process's affinity mask, and
+    the threads the CPU reference may use: those cores, capped by the CPU
+    share the job is given (OMP_NUM_THREADS when set, as on a one-GPU
+    machine shared with other jobs)."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        topo = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((topo.joinpath("physical_package_id").read_text().strip(),
+                       topo.joinpath("core_id").read_text().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    physical = len(cores) or len(cpus)
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = min(physical, int(share)) if share and share.isdigit() and int(share) > 0 else physical
+    return {"cpu_model": model, "logical_cpus_in_mask": len(cpus),
+            "physical_cores_in_mask": physical, "omp_num_threads_env": share, "threads": threads}
+
+
+_PROF = re.compile(r"^(run_\w+|eigenvalue_driver_initialise)\s+(\d+)\s+([0-9.]+)\s*$", re.M)
+
+
+def _ref_profile(exe: Path, text: str, threads: int) -> dict:
+    """Run the reference host + omp3 binary on a deck; return its printed
+    iteration lines, Wallclock and -DENABLE_PROFILING kernel table."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as d:
+        Path(d, "tea.in").write_text(text)
+        Path(d, "tea.problems").write_text((ROOT / "tests" / "golden" / "tea_problems_2d.txt").read_text())
+        p = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, env=env, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError(f"{exe.name} exited {p.returncode}: {p.stdout[-1000:]}")
+    iters = {k: int(v) for k, v in re.findall(r"^(CG|Cheby|PPCG):\s+(\d+) iterations", p.stdout, re.M)}
+    prof = {name: (int(calls), float(sec)) for name, calls, sec in _PROF.findall(p.stdout)}
+    m_w = re.search(r"Wallclock:\s+([0-9.]+)s", p.stdout)
+    return {"iters": iters, "profile": prof, "wallclock": float(m_w.group(1)) if m_w else None}
+
+
+def cpu_baseline(cells: int, cg_iters: int, cheby_iters: int, ppcg_outer: int):
     """The reference omp3 CPU path (reference host + omp3 suite, built from
-    source into oracle/_ref) on a bounded sample: one CG timestep of the same
-    deck stopped after `iters` iterations."""
+    source into oracle/_ref) on bounded samples of the benchmark deck, per
+    solver, timed by the reference's own -DENABLE_PROFILING kernel table:
+
+      * a Chebyshev-deck timestep (presteps 30) cut at max_iters =
+        cg_iters + cheby_iters: its CG phase (cg_calc_w + cg_calc_ur +
+        cg_calc_p + their halo updates) gives CG iterations/s, its
+        Chebyshev phase (cheby_iterate + 2-norms + halo updates) Chebyshev
+        iterations/s;
+      * a PPCG-deck timestep (presteps 30, 10 inner) cut at
+        cg_iters + ppcg_outer: its PPCG phase (outer steps' calc_w, calc_ur,
+        calc_p, 2-norm, ppcg_init, the inner sweeps and halo updates) gives
+        PPCG outer iterations/s, counted as the reference prints them.
+
+    cg_iters must reach the switch to Chebyshev/PPCG (1018 CG iterations at
+    4096^2 with presteps 30); the halo updates are shared out by call count."""
     exe = ROOT / "oracle" / "_ref" / "tealeaf2d_omp3"
     if not exe.exists():
         return None
     from tealeaf_amd import deck as tdeck
-    text = tdeck.layered_2d(cells, 1, solver=tdeck.CG, eps=1e-15, max_iters=iters)
-    cores = min(os.cpu_count() or 1, 16)
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    with tempfile.TemporaryDirectory() as d:
-        Path(d, "tea.in").write_text(text)
-        Path(d, "tea.problems").write_text((ROOT / "tests" / "golden" / "tea_problems_2d.txt").read_text())
-        p = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, env=env, timeout=600)
-    m_it = re.search(r"CG:\s+(\d+) iterations", p.stdout)
-    m_w = re.search(r"Wallclock:\s+([0-9.]+)s", p.stdout)
-    if not (m_it and m_w):
+    info = cpu_info()
+    th = info["threads"]
+    t0 = time.perf_counter()
+    ch = _ref_profile(exe, tdeck.layered_2d(cells, 1, solver=tdeck.CHEBY, eps=1e-15,
+                                            max_iters=cg_iters + cheby_iters, presteps=30), th)
+    pp = _ref_profile(exe, tdeck.layered_2d(cells, 1, solver=tdeck.PPCG, eps=1e-15,
+                                            max_iters=cg_iters + ppcg_outer, presteps=30,
+                                            inner=10), th)
+    wall = time.perf_counter() - t0
+
+    def t(prof, *names):
+        return sum(prof.get(n, (0, 0.0))[1] for n in names)
+
+    def calls(prof, n):
+        return prof.get(n, (0, 0.0))[0]
+
+    out = {}
+    P = ch["profile"]
+    n_cg = calls(P, "run_cg_calc_w")            # CG steps (the printed line counts them too)
+    n_ch = calls(P, "run_cheby_iterate")
+    halo = t(P, "run_local_halos") / max(calls(P, "run_local_halos"), 1)
+    if n_cg:
+        t_cg = t(P, "run_cg_calc_w", "run_cg_calc_ur", "run_cg_calc_p") + n_cg * halo
+        out["cg"] = {"iterations": n_cg, "seconds": t_cg, "iters_per_s": n_cg / t_cg}
+    if n_ch:
+        t_ch = t(P, "run_cheby_iterate", "run_calculate_2norm") + n_ch * halo
+        out["cheby"] = {"iterations": n_ch, "seconds": t_ch, "iters_per_s": n_ch / t_ch}
+    Q = pp["profile"]
+    n_out = pp["iters"].get("PPCG", 0)
+    if n_out:
+        n_cgq = pp["iters"].get("CG", 0)
+        # the outer steps are cg_calc_w/ur/p calls beyond the CG phase
+        frac = n_out / max(calls(Q, "run_cg_calc_w"), 1)
+        halo_q = t(Q, "run_local_halos") / max(calls(Q, "run_local_halos"), 1)
+        n_halo_ppcg = calls(Q, "run_local_halos") - n_cgq - 1
+        t_pp = (frac * t(Q, "run_cg_calc_w", "run_cg_calc_ur", "run_cg_calc_p")
+                + t(Q, "run_ppcg_inner_iteration", "run_ppcg_init", "run_calculate_2norm",
+                    "run_calculate_residual") + max(n_halo_ppcg, 0) * halo_q)
+        out["ppcg"] = {"iterations": n_out, "inner_sweeps": calls(Q, "run_ppcg_inner_iteration"),
+                       "seconds": t_pp, "iters_per_s": n_out / t_pp}
+    if not out:
         return None
-    printed = int(m_it.group(1))
-    # cg_driver.c prints the loop index: max_iters when the cap ends the loop,
-    # else the index of the converging iteration.
-    n_it = iters if printed >= iters else printed + 1
-    wall = float(m_w.group(1))
-    return {"value": n_it / wall, "unit": "iters/s", "cores": cores, "kind": "reference",
+    return {"unit": "iters/s", "cores": th, "kind": "reference", **info,
+            "by_solver": out, "sample_wall_s": wall,
             "sample": f"reference host + omp3 suite built from source (oracle/_ref/tealeaf2d_omp3), "
-                      f"{cells}^2 layered deck, 1 CG timestep stopped at {n_it} iterations, "
-                      f"timed by the reference's own Wallclock ({wall:.2f} s), "
-                      f"OMP_NUM_THREADS={cores}"}
+                      f"{cells}^2 layered deck, one timestep as Chebyshev (presteps 30) cut at "
+                      f"{cg_iters + cheby_iters} iterations and one as PPCG (presteps 30, 10 inner) "
+                      f"cut at {cg_iters + ppcg_outer}; per-solver time from the reference's own "
+                      f"ENABLE_PROFILING kernel table, OMP_NUM_THREADS={th}, OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores"}
 
 
 def load_traffic(kernel: str, init_share: float = 0.0):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size).
-    For the fused PPCG loop, launches that start from r (ppcg_init folded in)
-    and launches that continue from sd are weighted by their share."""
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
+    and where they come from (file, commit the counters were taken at).
+    For the fused PPCG loop, launches that start from r (ppcg_init folded
+    in) and launches that continue from sd are weighted by their share."""
     path = ROOT / "profiles" / "ppcg_4096_traffic.json"
     if not path.exists():
-        return None
+        return None, None
     try:
-        kernels = json.loads(path.read_text())["kernels"]
+        doc = json.loads(path.read_text())
+        kernels = doc["kernels"]
+        src = {"file": str(path.relative_to(ROOT)), "commit": doc.get("commit"),
+               "date": doc.get("date")}
         if kernel == "tlhip2d_ppcg_inner_iterations":
             a = kernels["tlhip2d_ppcg_inner_iterations (from r)"]["bytes_per_launch"]
             b = kernels["tlhip2d_ppcg_inner_iterations"]["bytes_per_launch"]
-            return init_share * a + (1.0 - init_share) * b
-        return kernels[kernel]["bytes_per_launch"]
+            return init_share * a + (1.0 - init_share) * b, src
+        return kernels[kernel]["bytes_per_launch"], src
     except Exception:
-        return None
+        return None, None
 
 
 def blocks3d_run(args):
@@ -364,6 +465,66 @@ def blocks3d_run(args):
                                "frac": gbs / HBM_PEAK_GBS if gbs else None}}}
 
 
+def adjoint_run(args):
+    """The adjoint workload: loss = sum over the interior of
+    (u_N - u_target)^2 after N = 5 implicit steps (CG, eps 1e-12), u_target
+    the uniform-density (1.0) solve perturbed by seeded N(0, 0.01) noise;
+    the gradient with respect to every cell's log-density, with per-step u
+    checkpoints (tlhip_problem_adjoint: 5 forward solves, then per step back
+    one adjoint CG solve and the face-gradient kernel).  2D: the layered
+    deck at 2048^2; 3D: the cuboid deck at 128^3.  Times forward+backward,
+    and the forward solves alone for comparison."""
+    import numpy as np
+    import torch
+    from tealeaf_amd import deck as tdeck
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.problem import Problem
+
+    L = lib()
+    L.tlhip_init(0)
+    steps = 5
+    out = {}
+    for dims, n in ((2, 2048), (3, 128)):
+        text = (tdeck.layered_2d(n, steps, eps=1e-12, max_iters=20000) if dims == 2 else
+                tdeck.cuboid_3d(n, steps, eps=1e-12, max_iters=20000))
+        s, states = tdeck.parse_deck(text, dims)
+        with Problem(s, states) as p:
+            dens, e0 = p.get("density"), p.get("energy0")
+        with Problem(s, states) as p:
+            p.inject(np.ones_like(dens), e0)
+            p.run(steps)
+            u_uni = p.get("u")
+        hd = s.halo_depth
+        inner = tuple(slice(hd, m - hd) for m in s.shape())
+        rng = np.random.default_rng(5)
+        target = np.zeros(s.shape())
+        target[inner] = u_uni[inner] + 0.01 * rng.standard_normal(u_uni[inner].shape)
+        with Problem(s, states) as p:
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = p.run(steps)
+            L.tlhip_synchronize()
+            t_fwd = time.perf_counter() - t0
+        with Problem(s, states) as p:
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss, grad, areps = p.adjoint(steps, target)
+            L.tlhip_synchronize()
+            t_adj = time.perf_counter() - t0
+        key = f"{dims}d_{n}"
+        out[key] = {"forward_backward_s": t_adj, "forward_only_s": t_fwd,
+                    "backward_over_forward": (t_adj - t_fwd) / t_fwd,
+                    "forward_cg_iters": [r["printed_cg_iters"] for r in reps],
+                    "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
+        print(f"[bench] adjoint {key}: forward+backward {t_adj:.3f} s (forward {t_fwd:.3f} s)",
+              file=sys.stderr, flush=True)
+    return {"workload": "loss sum (u_5 - u_target)^2 after 5 CG steps (eps 1e-12), gradient "
+                        "w.r.t. per-cell log-density; 2D layered deck 2048^2, 3D cuboid deck 128^3",
+            **out}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -382,6 +543,7 @@ def main():
         return
     res = gpu_run(args)
     refine = refine_run(args) if args.refine_cells > 0 else None
+    adjoint = None if args.no_adjoint else adjoint_run(args)
     value = res["iters"] / res["time"]
     n_int = args.cells * args.cells
     dom = dominant(args)
@@ -391,24 +553,54 @@ def main():
         sw = res["by_solver"]["ppcg"]["sweeps"]
         t = k["avg_ms"] * 1e-3
         if dom == "tlhip2d_ppcg_inner_iterations":
-            # the survey's per-unit figures: 64 B/cell per inner step, plus the
-            # cg_calc_ur (48 B/cell) and ppcg_init (16 B/cell) that each outer
-            # iteration's first launch does
-            alg_cell = (64.0 * sw["ppcg_inner"] + (48.0 + 16.0) * sw["ppcg_outer"]) / k["launches"]
-            init_share = sw["ppcg_outer"] / k["launches"]
+            # Compulsory HBM bytes of one launch: every array it reads once and
+            # every array it writes once.  A launch that starts an outer
+            # iteration (cg_calc_ur + ppcg_init + the inner steps) reads r, w,
+            # u, p, kx, ky and writes sd', r', u' (72 B/cell); one that
+            # continues from sd reads sd, kx, ky, r, u and writes the same
+            # (64 B/cell).
+            init_share = min(1.0, sw["ppcg_outer"] / k["launches"])
+            min_cell = 64.0 + 8.0 * init_share
+            # the bytes the same steps would move unfused: 64 B/cell per inner
+            # step plus cg_calc_ur (48) and ppcg_init (16) per outer iteration
+            unfused_cell = (64.0 * sw["ppcg_inner"] + (48.0 + 16.0) * sw["ppcg_outer"]) / k["launches"]
         else:
-            alg_cell, init_share = 64.0, 0.0
-        alg = alg_cell * n_int
-        achieved = alg / t / 1e9
-        traffic = load_traffic(dom, init_share)
+            init_share, min_cell, unfused_cell = 0.0, 64.0, 64.0
+        achieved = min_cell * n_int / t / 1e9
+        traffic, tsrc = load_traffic(dom, init_share)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                    "algorithmic_bytes": alg, "avg_kernel_ms": k["avg_ms"],
-                    "launches": k["launches"],
+                    "traffic_source": tsrc,
+                    "bytes_per_cell": min_cell, "algorithmic_bytes": min_cell * n_int,
+                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"],
                     "inner_steps_per_launch": sw["ppcg_inner"] / k["launches"],
-                    # what the launch itself moves through HBM (counters), and at what rate
+                    # what the launch moves through HBM by the counters, and at what rate
                     "hbm_gbs": traffic / t / 1e9 if traffic else None,
-                    "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None}
+                    "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
+                    # the rate an unfused sequence of the same steps would need (not a
+                    # fraction of any peak: one launch does several steps' work)
+                    "unfused_equivalent_gbs": unfused_cell * n_int / t / 1e9}
+    cpu = None if args.no_cpu else cpu_baseline(args.cells, args.cpu_cg_iters,
+                                                 args.cpu_cheby_iters, args.cpu_ppcg_outer)
+    if cpu:
+        # the CPU time for the same iteration mix as the timed GPU steps
+        rate = {kind: v["iters_per_s"] for kind, v in cpu["by_solver"].items()}
+        kind_of = {"cg": "cg", "cheby": "cheby", "ppcg": "ppcg"}
+        cpu_total, ratio = 0.0, {}
+        ok = True
+        for sname, b in res["by_solver"].items():
+            n_cg = sum(st["CG"] for st in b["per_step"])
+            n_in = sum(st["inner"] for st in b["per_step"])
+            if "cg" not in rate or (n_in and kind_of[sname] not in rate):
+                ok = False
+                continue
+            t_cpu = n_cg / rate["cg"] + (n_in / rate[kind_of[sname]] if n_in else 0.0)
+            cpu_total += t_cpu
+            ratio[sname] = t_cpu / b["seconds"]
+        cpu["gpu_speedup_by_solver"] = ratio
+        cpu["value"] = res["iters"] / cpu_total if ok and cpu_total > 0 else None
+        cpu["value_note"] = ("the timed GPU steps' iteration mix (CG, Chebyshev and PPCG "
+                             "iterations as printed) at the per-solver CPU rates")
     line = {
         "metric": "solver iters/s + HBM GB/s, 2D 4096²; FP32-refined vs FP64 time-to-1e-10",
         "value": value,
@@ -427,8 +619,9 @@ def main():
                                "(10 inner, presteps 30); one step = one timestep of each",
                    "cells": n_int, "ppcg_fusion": args.ppcg_fusion, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
         "roofline": roofline,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cells, args.cpu_iters),
+        "cpu_baseline": cpu,
         "refine": refine,
+        "adjoint": adjoint,
         "by_solver": res["by_solver"],
         "kernels": res["kernels"],
     }
