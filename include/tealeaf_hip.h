@@ -133,6 +133,20 @@ int tlhip2d_cg_init(int x, int y, int halo_depth, int coefficient, double rx,
 /* 2d/kernel_interface.h:38 run_cg_calc_w */
 int tlhip2d_cg_calc_w(int x, int y, int halo_depth, double* pw,
         const double* p, double* w, const double* kx, const double* ky);
+/* No counterpart in the reference: tlhip2d_cg_calc_w for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  The
+ * call allocates nothing: its per-block partial sums go to a fixed device
+ * array of 65536 doubles (512 KiB, held by each GPU the library runs on
+ * until the process ends; the sweep's rows per block grow so that every
+ * accepted grid fits) and the dot product is added (+=) into *pw_dev, a
+ * device double, on the suite's stream; w and pw_dev equal, bit for bit,
+ * what tlhip2d_cg_calc_w gives.  tlhip_init must have been called (it
+ * records the array's address).  Rejected: grids over 2^31 - 1 cells and
+ * rows over 65536 strips of 64 lanes (more than 4 million cells).  Calls
+ * that overlap on one GPU (other streams or threads) share the array and
+ * race. */
+int tlhip2d_cg_calc_w_noalloc(int x, int y, int halo_depth, double* pw_dev,
+        const double* p, double* w, const double* kx, const double* ky);
 /* 2d/kernel_interface.h:40 run_cg_calc_ur */
 int tlhip2d_cg_calc_ur(int x, int y, int halo_depth, double alpha,
         double* rrn, double* u, const double* p, double* r, const double* w);

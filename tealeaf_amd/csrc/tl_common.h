@@ -74,6 +74,9 @@ int tl_fetch(int n, double* out, const char* what);
 // (seconds; tlhip_set_wait_limit).
 double tl_wait_limit_s();
 int tl_ensure_init();
+// Records, for the device current at initialisation, the address of the
+// allocation-free entry point's scratch array (tl_kernels2d.hip).
+int tl_noalloc_setup(int device);
 // Kernel timer hooks: begin returns a slot (or -1 when `what` is not being
 // timed); end records the closing event for that slot.
 int tl_timer_begin(const char* what);

@@ -909,3 +909,88 @@ int norm_err(int x, int y, int hd, const double* buf, const loop::StopRule& rule
 
 }  // namespace loop2d
 }  // namespace tl
+
+// ------------------------------------------------- allocation-free cg_calc_w
+// tlhip2d_cg_calc_w_noalloc: cg_calc_w (w = A p, pw += sum w*p) for callers
+// that cannot allocate -- code under HIP graph capture, tight serving loops.
+// The stencil sweep is the same kernel as tlhip2d_cg_calc_w; its per-block
+// partial sums go to `scratch`, a file-scope device array, instead of the
+// runtime's allocated reduction slots, and a one-block finisher adds the
+// total, summed in the same order as the standard finisher, into a device
+// double the caller passes.  The call itself allocates nothing: no
+// hipMalloc/hipFree, no pinned mailbox, no timer events, no runtime set-up
+// (tlhip_init must have run; it records the array's device address).
+// Every call on a GPU shares that GPU's array; its contents are the last
+// call's partials, each fully rewritten before it is read, so nothing needs
+// clearing.  Calls that overlap on one GPU (other streams, other threads)
+// race on the array.
+namespace {
+
+// One partial per stencil block: make_tile keeps the block count of every
+// grid that passes check_grid and the row-length check within
+// TL_MAX_RED_BLOCKS (it grows the rows per block), so this holds them all.
+constexpr int kNoAllocPartials = TL_MAX_RED_BLOCKS;   // 512 KiB
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+double* g_scratch[kNoAllocDevices] = {};    // per device: the array's device address
+
+__global__ void __launch_bounds__(256)
+k_finish_into(unsigned nblocks, const double* __restrict__ partials, double* __restrict__ out)
+{
+    double tot[1];
+    tl_sum_partials<1>(nblocks, partials, tot);
+    if (threadIdx.x == 0) out[0] += tot[0];
+}
+
+}  // namespace
+
+// Called by tlhip_init with `device` current: the array's address on it.
+int tl_noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices)
+        return tl_fail(TLHIP_ERR_ARG, "device %d: at most %d devices", device, kNoAllocDevices);
+    if (g_scratch[device]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return tl_fail(TLHIP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[device] = static_cast<double*>(ptr);
+    return 0;
+}
+
+extern "C" int tlhip2d_cg_calc_w_noalloc(int x, int y, int hd, double* pw_dev, const double* p,
+                                         double* w, const double* kx, const double* ky)
+{
+    const char* what = "tlhip2d_cg_calc_w_noalloc";
+    const int dev = tlhip_device();
+    if (dev < 0 || dev >= kNoAllocDevices || !g_scratch[dev])
+        return tl_fail(TLHIP_ERR_ARG, "%s: call tlhip_init first (this entry point sets up "
+                       "nothing itself)", what);
+    const TlGrid2 g{x, y, hd};
+    TL_TRY(check_grid(g, what));
+    REQUIRE(pw_dev && p && w && kx && ky, what, "null argument");
+    REQUIRE(p != w, what, "p and w must be distinct buffers");
+    if (interior_empty(g)) return 0;
+    const int V = pick_v(g, p, w, kx, ky);
+    if (tl_cdiv(g.x - 2 * g.hd, TL_WAVE * V) > kNoAllocPartials)
+        return tl_fail(TLHIP_ERR_ARG, "%s: rows too long (%d cells): more than %d strips", what,
+                       x, kNoAllocPartials);
+    const Tile2 t = make_tile(g, V);
+    const long nb = (long)t.nbx * t.nby;
+    if (nb > kNoAllocPartials)
+        return tl_fail(TLHIP_ERR_ARG, "%s: %dx%d needs %ld reduction partials, more than the %d "
+                       "its scratch array holds", what, x, y, nb, kNoAllocPartials);
+    TlReduce rd{};
+    rd.partials = g_scratch[dev];
+    const OpCalcW op{w};
+    if (V == 2)
+        hipLaunchKernelGGL((k_stencil2d<2, OpCalcW>), dim3((unsigned)nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, p, kx, ky, op, rd, TlGate{});
+    else
+        hipLaunchKernelGGL((k_stencil2d<1, OpCalcW>), dim3((unsigned)nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, p, kx, ky, op, rd, TlGate{});
+    TL_TRY(tl_check_launch(what));
+    hipLaunchKernelGGL(k_finish_into, dim3(1), dim3(256), 0, tl_stream(), (unsigned)nb,
+                       rd.partials, pw_dev);
+    return tl_check_launch(what);
+}

@@ -80,6 +80,7 @@ extern "C" int tlhip_init(int device)
     if (count < 1) return tl_fail(TLHIP_ERR_HIP, "tlhip_init: no HIP device");
     const int dev = device < 0 ? 0 : device % count;
     HIP_TRY(hipSetDevice(dev), "hipSetDevice");
+    TL_TRY(tl_noalloc_setup(dev));
     // Default priority on purpose: streams of mixed priority are not used.
     HIP_TRY(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking), "hipStreamCreate");
     g_rt.own_stream = 1;
