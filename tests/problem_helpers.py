@@ -75,7 +75,40 @@ def ref_run(binary: str, text: str, timeout: float = 600.0, threads: int | None 
                       "error": float(e.group(1)) if e else None})
     m = re.search(r"TL_FIELD_SUMMARY (\S+) (\S+) (\S+) (\S+)", p.stdout)
     summary = [float(v) for v in m.groups()] if m else None
-    return {"steps": steps, "summary": summary, "stdout": p.stdout}
+    m = re.search(r"TL_REFORDER_SUMMARY (\S+) (\S+) (\S+) (\S+)", p.stdout)
+    refsum = [float(v) for v in m.groups()] if m else None
+    return {"steps": steps, "summary": summary, "refsum": refsum, "stdout": p.stdout}
+
+
+def _cell_terms(s: tdeck.Settings, density, energy0, u):
+    """The per-cell terms of 2d/c_kernels/omp3/field_summary.c:31-38 (3D
+    3d/c_kernels/omp3/field_summary.c:33-40) over the interior, in the
+    reference's loop order: cell volume (dx*dy[*dz], set_chunk_data.c:49),
+    mass = volume*density, mass*energy0, mass*u -- each an IEEE product."""
+    import numpy as np
+    hd = s.halo_depth
+    inner = tuple(slice(hd, n - hd) for n in s.shape())
+    d = np.asarray(density).reshape(s.shape())[inner].ravel()
+    e0 = np.asarray(energy0).reshape(s.shape())[inner].ravel()
+    uu = np.asarray(u).reshape(s.shape())[inner].ravel()
+    vol = s.dx * s.dy * s.dz if s.dims == 3 else s.dx * s.dy
+    v = np.full(d.size, vol)
+    mass = v * d
+    return v, mass, mass * e0, mass * uu
+
+
+def reference_order_summary(s: tdeck.Settings, density, energy0, u) -> list[float]:
+    """vol, mass, ie, temp summed as the reference's field_summary sums
+    them: one cell after another into a running double, in its loop order
+    (numpy's add.accumulate is that sequential loop)."""
+    import numpy as np
+    return [float(np.add.accumulate(t)[-1]) for t in _cell_terms(s, density, energy0, u)]
+
+
+def exact_summary(s: tdeck.Settings, density, energy0, u) -> list[float]:
+    """The same four sums correctly rounded (math.fsum)."""
+    import math
+    return [math.fsum(t.tolist()) for t in _cell_terms(s, density, energy0, u)]
 
 
 def printed_iters(rep: dict, solver: int) -> dict:
