@@ -10,9 +10,11 @@ all three; the three chunks stay resident in HBM throughout.
 value = solver iterations per second over the timed steps, counting
 iterations as the reference host prints them (CG + Chebyshev + PPCG lines).
 The JSON line also carries the per-solver rates, the roofline of the
-dominant kernel (the temporally blocked PPCG inner loop, timed live with HIP events on
-the suite's stream), and the reference omp3 CPU path timed on this host on a
-bounded sample of the same deck.
+dominant kernel (of calc_pw, calc_ur, the temporally blocked PPCG loop and
+the temporally blocked Chebyshev loop, timed live with HIP events on the
+suite's stream in every leg, the one with the most total time; the fused
+PPCG loop's own roofline beside it), and the reference omp3 CPU path timed
+on this host on a bounded sample of the same deck.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 4096]
 """
@@ -75,6 +77,11 @@ def parse_args():
     ap.add_argument("--blocks3d-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint (forward+backward) timing")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="build the HBM-traffic summary under profiles/ from the two rocprofv3 "
+                         "--pmc passes over --profile-run, and exit (no GPU)")
+    ap.add_argument("--traffic-commit", default="",
+                    help="commit the counter passes ran at, stamped into the summary")
     ap.add_argument("--profile-run", action="store_true",
                     help="counter-collection run: a few launches of each hot kernel at --cells "
                          "and of a copy kernel of known size, then exit")
@@ -109,26 +116,32 @@ def gpu_run(args):
     total_time = 0.0
     total_iters = 0
     kernel = {}
+    # the kernels that hold most of the time, timed in every leg
+    timed = ["tlhip2d_cg_calc_pw", "tlhip2d_cg_calc_ur", dominant(args), "tlhip2d_cheby_iterations"]
     for s in solvers:
-        timed = dominant(args) if s == "ppcg" else ("tlhip2d_cg_calc_pw" if s == "cg" else "tlhip2d_cheby_iterations")
         L.tlhip_synchronize()
         torch.cuda.synchronize()
-        L.tlhip_kernel_timer_start(timed.encode())
+        L.tlhip_kernel_timer_start(",".join(timed).encode())
         t0 = time.perf_counter()
         reps = [probs[s].step(tt) for tt in range(args.warmup, total_steps)]
         L.tlhip_synchronize()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        ms, nl = ctypes.c_double(), ctypes.c_long()
-        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
+        for name in timed:
+            ms, nl = ctypes.c_double(), ctypes.c_long()
+            L.tlhip_kernel_timer_read_entry(name.encode(), ctypes.addressof(ms),
+                                            ctypes.addressof(nl))
+            k = kernel.setdefault(name, {"launches": 0, "total_ms": 0.0, "by_solver": {}})
+            k["launches"] += nl.value
+            k["total_ms"] += ms.value
+            if nl.value:
+                k["by_solver"][s] = {"launches": nl.value, "ms": ms.value}
+        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))   # stop
         printed = sum(r["printed_cg_iters"] + r["printed_inner_iters"] for r in reps)
         sweeps = {"cg_steps": sum(r["n_cg_steps"] for r in reps),
                   "cheby_steps": sum(r["n_cheby_steps"] for r in reps),
                   "ppcg_outer": sum(r["n_ppcg_outer"] for r in reps),
                   "ppcg_inner": sum(r["n_ppcg_inner"] for r in reps)}
-        avg_ms = ms.value / max(nl.value, 1)
-        kernel[timed] = {"launches": nl.value, "avg_ms": avg_ms,
-                         "gbs": BYTES_PER_CELL[timed] * n_int / (avg_ms * 1e-3) / 1e9 if nl.value else None}
         by[s] = {"iterations": printed, "seconds": dt, "iters_per_s": printed / dt,
                  "per_step": [{"CG": r["printed_cg_iters"], "inner": r["printed_inner_iters"]}
                               for r in reps], "sweeps": sweeps,
@@ -138,6 +151,11 @@ def gpu_run(args):
         print(f"[bench] {s}: {printed} iterations in {dt:.3f} s", file=sys.stderr, flush=True)
     for p in probs.values():
         p.close()
+    for name, k in kernel.items():
+        k["avg_ms"] = k["total_ms"] / k["launches"] if k["launches"] else None
+        k["gbs"] = (BYTES_PER_CELL[name] * n_int / (k["avg_ms"] * 1e-3) / 1e9
+                    if k["launches"] else None)
+        k["share_of_timed_time"] = k["total_ms"] * 1e-3 / total_time
     return {"by_solver": by, "kernels": kernel, "time": total_time, "iters": total_iters}
 
 
@@ -203,13 +221,22 @@ def refine_run(args):
 
 
 def profile_run(args):
-    """Three launches each of the hot kernels on --cells^2 synthetic fields,
-    and of a 1 GiB device copy (8 B read + 8 B written per element) whose
-    FETCH_SIZE / WRITE_SIZE calibrate the counters; for rocprofv3 --pmc."""
+    """Counter-collection run (rocprofv3 --pmc): three launches each of every
+    hot kernel on synthetic fields -- 2D at --cells^2 (the fused PPCG and
+    Chebyshev loops, calc_w, calc_pw, calc_ur, calc_p, the 2-norm,
+    field_summary, cg_init and the single-step PPCG sweep; the FP32 fused
+    loop), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
+    2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
+    2D PPCG solve at 2048^2 for the FP32 CG kernels, and a 1 GiB device copy
+    (8 B read + 8 B written per element) whose FETCH_SIZE / WRITE_SIZE
+    calibrate the counters."""
     import ctypes as C
     import numpy as np
+    from tealeaf_amd import blocks
+    from tealeaf_amd import deck as tdeck
     from tealeaf_amd.abi import lib
     from tealeaf_amd.device import DeviceArray
+    from tealeaf_amd.problem import Problem
 
     L = lib()
     L.tlhip_init(0)
@@ -221,6 +248,9 @@ def profile_run(args):
            for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
     for k in ("kx", "ky"):
         f64[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, N))
+    f64["rho"] = DeviceArray.from_host(rng.uniform(0.5, 2.0, N))
+    f64["e"] = DeviceArray.from_host(rng.uniform(0.1, 1.0, N))
+    f64["vol"] = DeviceArray.from_host(np.full(N, 1e-4))
     f32 = {k: DeviceArray.from_host(rng.uniform(-1, 1, P * y).astype(np.float32))
            for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
     for k in ("kx", "ky"):
@@ -230,6 +260,8 @@ def profile_run(args):
     nb = np.full(4, -1, dtype=np.int32)
     al, be = np.full(10, 0.7), np.full(10, 0.2)
     pw = C.c_double()
+    s4 = [C.c_double() for _ in range(4)]
+    a4 = [C.addressof(v) for v in s4]
     for _ in range(3):
         L.tlhip_stream_copy(dst.ptr, src.ptr, C.c_size_t(ncopy))
         # the driver's launch: cg_calc_ur + ppcg_init + 10 inner steps (MODE 2)
@@ -257,8 +289,59 @@ def profile_run(args):
                                        f64["ky"].ptr, f64["sd"].ptr, f64["sd2"].ptr)
         L.tlhip2d_cg_calc_w(x, y, hd, C.addressof(pw), f64["p"].ptr, f64["w"].ptr, f64["kx"].ptr,
                             f64["ky"].ptr)
+        L.tlhip2d_cg_calc_ur(x, y, hd, 1e-3, C.addressof(pw), f64["u"].ptr, f64["p"].ptr,
+                             f64["r"].ptr, f64["w"].ptr)
+        L.tlhip2d_cg_calc_p(x, y, hd, 0.5, f64["p"].ptr, f64["r"].ptr)
+        L.tlhip2d_calculate_2norm(x, y, hd, f64["r"].ptr, C.addressof(pw))
+        L.tlhip2d_field_summary(x, y, hd, f64["vol"].ptr, f64["rho"].ptr, f64["e"].ptr,
+                                f64["u"].ptr, *a4)
+        L.tlhip2d_cg_init(x, y, hd, 1, 0.01, 0.01, C.addressof(pw), f64["rho"].ptr, f64["e"].ptr,
+                          f64["u2"].ptr, f64["p"].ptr, f64["r2"].ptr, f64["w"].ptr,
+                          f64["kx"].ptr, f64["ky"].ptr)
     L.tlhip_synchronize()
-    print(json.dumps({"profile_run": True, "cells": n, "copy_elements": ncopy}), flush=True)
+    for v in list(f64.values()) + list(f32.values()):
+        v.free()
+    # 3D at 256^3
+    m = 256
+    X3 = m + 2 * hd
+    N3 = X3 ** 3
+    g = {k: DeviceArray.from_host(rng.uniform(-1, 1, N3)) for k in ("u", "r", "sd", "sd2", "p", "w")}
+    for k in ("kx", "ky", "kz"):
+        g[k] = DeviceArray.from_host(rng.uniform(0.02, 0.1, N3))
+    g["rho"] = DeviceArray.from_host(rng.uniform(0.5, 2.0, N3))
+    g["e"] = DeviceArray.from_host(rng.uniform(0.1, 1.0, N3))
+    g["vol"] = DeviceArray.from_host(np.full(N3, 1e-6))
+    for _ in range(3):
+        L.tlhip3d_cg_solver_calc_w(X3, X3, X3, hd, C.addressof(pw), g["p"].ptr, g["w"].ptr,
+                                   g["kx"].ptr, g["ky"].ptr, g["kz"].ptr)
+        L.tlhip3d_cg_solver_calc_ur(X3, X3, X3, hd, 1e-3, C.addressof(pw), g["u"].ptr, g["p"].ptr,
+                                    g["r"].ptr, g["w"].ptr)
+        L.tlhip3d_cg_solver_calc_p(X3, X3, X3, hd, 0.5, g["p"].ptr, g["r"].ptr)
+        L.tlhip3d_ppcg_inner_iteration(X3, X3, X3, hd, 0.7, 0.2, g["u"].ptr, g["r"].ptr,
+                                       g["kx"].ptr, g["ky"].ptr, g["kz"].ptr, g["sd"].ptr,
+                                       g["sd2"].ptr)
+        L.tlhip3d_calculate_2norm(X3, X3, X3, hd, g["r"].ptr, C.addressof(pw))
+        L.tlhip3d_field_summary(X3, X3, X3, hd, g["vol"].ptr, g["rho"].ptr, g["e"].ptr,
+                                g["u"].ptr, *a4)
+        L.tlhip3d_cg_solver_init(X3, X3, X3, hd, 1, 0.01, 0.01, 0.01, C.addressof(pw),
+                                 g["rho"].ptr, g["e"].ptr, g["u"].ptr, g["p"].ptr, g["r"].ptr,
+                                 g["w"].ptr, g["kx"].ptr, g["ky"].ptr, g["kz"].ptr)
+    L.tlhip_synchronize()
+    for v in g.values():
+        v.free()
+    # one FP32-refined PPCG timestep at 2048^2: the FP32 CG kernels
+    text = tdeck.layered_2d(2048, 1, solver=tdeck.PPCG, eps=1e-15, max_iters=20000, inner=10,
+                            presteps=30, epslim=1e30)
+    st, states = tdeck.parse_deck(text, 2)
+    dens, en = blocks.block_fields((2048, 2048), 64, seed=1001, rho=(1.0, 1e3),
+                                   energy=(0.1, 25.0), halo_depth=st.halo_depth)
+    with Problem(st, states) as prob:
+        prob.inject(dens, en)
+        prob.set_solve(1, 1e-10)
+        prob.step(0)
+    L.tlhip_synchronize()
+    print(json.dumps({"profile_run": True, "cells": n, "cells3d": m,
+                      "copy_elements": ncopy}), flush=True)
 
 
 def dominant(args) -> str:
@@ -392,7 +475,7 @@ def load_traffic(kernel: str, init_share: float = 0.0):
     and where they come from (file, commit the counters were taken at).
     For the fused PPCG loop, launches that start from r (ppcg_init folded
     in) and launches that continue from sd are weighted by their share."""
-    path = ROOT / "profiles" / "ppcg_4096_traffic.json"
+    path = ROOT / TRAFFIC_JSON
     if not path.exists():
         return None, None
     try:
@@ -407,6 +490,192 @@ def load_traffic(kernel: str, init_share: float = 0.0):
         return kernels[kernel]["bytes_per_launch"], src
     except Exception:
         return None, None
+
+
+TRAFFIC_JSON = "profiles/hbm_traffic_2d4096_3d256.json"
+
+# The main kernel(s) of each entry point --profile-run launches, in launch
+# order: (entry point, HIP kernel name fragments, compulsory bytes per
+# interior cell = each array the call reads once + each it writes once).
+_TRAFFIC_2D = [
+    ("tlhip2d_ppcg_inner_iterations (from r)", ["k_ppcg_ring2d<double, 10, 2>"], 72,
+     "cg_calc_ur + ppcg_init + 10 inner steps: reads r, w, u, p, kx, ky; writes sd, r, u"),
+    ("tlhip2d_ppcg_inner_iterations", ["k_ppcg_ring2d<double, 10, 0>"], 64,
+     "10 inner steps from sd: reads sd, kx, ky, r, u; writes sd, r, u"),
+    ("tlhip2d_cg_calc_pw", ["k_calc_pw<double, true>"], 48,
+     "calc_p fused into calc_w: reads p, r, kx, ky; writes p', w"),
+    ("tlhip2d_cheby_iterations", ["k_cheby_fused2d<double, 10"], 64,
+     "10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
+    ("tlhip2d_ppcg_inner_iterations_f32 (from r)", ["k_ppcg_ring2d<float, 10, 2>"], 36,
+     "FP32 fields, cg_calc_ur + ppcg_init + 10 inner steps"),
+    ("tlhip2d_ppcg_inner_iteration", ["k_stencil2d<2, tl::OpPpcgInner>"], 64,
+     "one unfused inner step: reads sd, kx, ky, r, u; writes sd', r, u"),
+    ("tlhip2d_cg_calc_w", ["k_stencil2d<2, tl::OpCalcW>"], 32, "reads p, kx, ky; writes w"),
+    ("tlhip2d_cg_calc_ur", ["k_interior<2, tl::OpCalcUr>"], 48, "reads u, p, r, w; writes u, r"),
+    ("tlhip2d_cg_calc_p", ["k_interior<2, tl::OpCalcP>"], 24, "reads p, r; writes p"),
+    ("tlhip2d_calculate_2norm", ["k_interior<2, tl::OpNorm>"], 8, "reads r"),
+    ("tlhip2d_field_summary", ["k_interior<2, tl::OpFieldSummary>"], 32,
+     "reads volume, density, energy, u"),
+    ("tlhip2d_cg_init", ["k_cg_init_fields(", "k_stencil2d<2, tl::OpCgInit>"], 64,
+     "reads density, energy; writes u, p, r, w, kx, ky (two passes: the coefficients, then "
+     "the stencil)"),
+]
+_TRAFFIC_3D = [
+    ("tlhip3d_cg_solver_calc_w", ["k_stencil3d<2, tl::OpCalcW>"], 40,
+     "reads p, kx, ky, kz; writes w"),
+    ("tlhip3d_cg_solver_calc_ur", ["k_interior<2, tl::OpCalcUr>"], 48,
+     "reads u, p, r, w; writes u, r"),
+    ("tlhip3d_cg_solver_calc_p", ["k_interior<2, tl::OpCalcP>"], 24, "reads p, r; writes p"),
+    ("tlhip3d_ppcg_inner_iteration", ["k_stencil3d<2, tl::OpPpcgInner>"], 72,
+     "reads sd, kx, ky, kz, r, u; writes sd', r, u"),
+    ("tlhip3d_calculate_2norm", ["k_interior<2, tl::OpNorm>"], 8, "reads r"),
+    ("tlhip3d_field_summary", ["k_interior<2, tl::OpFieldSummary>"], 32,
+     "reads volume, density, energy, u"),
+    ("tlhip3d_cg_solver_init", ["k_cg_init_fields3(", "k_stencil3d<2, tl::OpCgInit>"], 72,
+     "reads density, energy; writes u, p, r, w, kx, ky, kz (two passes)"),
+]
+# FP32 kernels of the FP32-refined PPCG timestep at 2048^2 (many launches
+# each): name fragment -> (call, bytes per cell)
+_TRAFFIC_MIXED = [
+    ("tlhip2d_ppcg_inner_iterations_f32 (from r), 2048^2", "k_ppcg_ring2d<float, 10, 2>", 36,
+     "FP32 cg_calc_ur + ppcg_init + 10 inner steps"),
+    ("FP32 calc_pw", "k_calc_pw_rows<float, true>", 24, "reads p, r, kx, ky; writes p', w (FP32)"),
+    ("FP32 calc_w", "k_calc_pw_rows<float, false>", 16, "reads p, kx, ky; writes w (FP32)"),
+    ("FP32 calc_ur", "k_groups<(anonymous namespace)::OpUr32>", 24,
+     "reads d, p, r, w; writes d, r (FP32)"),
+    ("FP32 2-norm", "k_groups<(anonymous namespace)::OpNorm32>", 4, "reads r (FP32)"),
+    ("FP64 residual into FP32", "k_groups<(anonymous namespace)::OpResidual64>", 36,
+     "reads u, u0, kx, ky (FP64); writes r (FP32)"),
+    ("u += d", "k_groups<(anonymous namespace)::OpPromote>", 24,
+     "reads u (FP64), d (FP32); writes u, d"),
+]
+
+
+def write_traffic(fetch_csv: str, write_csv: str, cells: int = 4096, cells3d: int = 256,
+                  cells_mixed: int = 2048) -> dict:
+    """Builds the HBM-traffic summary from the FETCH_SIZE and WRITE_SIZE
+    passes (rocprofv3 --pmc ... --output-format csv) over `bench.py
+    --profile-run`: bytes per launch of every kernel it launches, scaled by
+    the 1 GiB device copy (1 GiB read + 1 GiB written) that opens each 2D
+    round, against the compulsory bytes of the call."""
+    import csv
+
+    def rows(path, counter):
+        out = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    out[int(r["Dispatch_Id"])] = (r["Kernel_Name"], float(r["Counter_Value"]))
+        return out
+
+    fe, wr = rows(fetch_csv, "FETCH_SIZE"), rows(write_csv, "WRITE_SIZE")
+    ids = sorted(set(fe) & set(wr))
+    disp = [(fe[i][0], fe[i][1], wr[i][1]) for i in ids]
+    gib_kib = float(1 << 20)
+    calib = [k for k, d in enumerate(disp) if "k_copy_all" in d[0] and d[2] > 0.9 * gib_kib]
+    if len(calib) < 3:
+        raise SystemExit("write_traffic: the three 1 GiB calibration copies are missing")
+    f_kib = sum(disp[k][1] for k in calib) / len(calib)
+    w_kib = sum(disp[k][2] for k in calib) / len(calib)
+    fs, ws = gib_kib / f_kib, gib_kib / w_kib
+
+    def bytes_of(d):
+        return 1024.0 * (fs * d[1] + ws * d[2])
+
+    def walk(start, table, rounds):
+        got = {name: [] for name, *_ in table}
+        k = start
+        for _ in range(rounds):
+            for name, frags, *_ in table:
+                total, fk, wk = 0.0, 0.0, 0.0
+                for frag in frags:
+                    while k < len(disp) and frag not in disp[k][0]:
+                        k += 1
+                    if k == len(disp):
+                        raise SystemExit(f"write_traffic: {frag} not found")
+                    total += bytes_of(disp[k])
+                    fk += disp[k][1]
+                    wk += disp[k][2]
+                    k += 1
+                got[name].append((total, fk, wk))
+        return got, k
+
+    g2, k = walk(calib[0], _TRAFFIC_2D, 3)
+    g3, k = walk(k, _TRAFFIC_3D, 3)
+    kernels = {}
+
+    def add(table, got, n_cells, size):
+        for name, frags, bpc, note in table:
+            v = got[name]
+            b = sum(t for t, _, _ in v) / len(v)
+            kernels[name] = {
+                "hip_kernels": frags, "cells": n_cells, "launches": len(v), "size": size,
+                "FETCH_SIZE_KiB": sum(f for _, f, _ in v) / len(v),
+                "WRITE_SIZE_KiB": sum(w for _, _, w in v) / len(v),
+                "bytes_per_launch": b, "min_bytes_per_launch": bpc * n_cells,
+                "ratio_to_min": b / (bpc * n_cells), "note": note}
+
+    add(_TRAFFIC_2D, g2, cells * cells, f"{cells}^2")
+    add(_TRAFFIC_3D, g3, cells3d ** 3, f"{cells3d}^3")
+    rest = disp[k:]
+    nm = cells_mixed * cells_mixed
+    for name, frag, bpc, note in _TRAFFIC_MIXED:
+        v = [d for d in rest if frag in d[0]]
+        if not v:
+            continue
+        b = sum(bytes_of(d) for d in v) / len(v)
+        kernels[name] = {
+            "hip_kernels": [frag], "cells": nm, "launches": len(v), "size": f"{cells_mixed}^2",
+            "FETCH_SIZE_KiB": sum(d[1] for d in v) / len(v),
+            "WRITE_SIZE_KiB": sum(d[2] for d in v) / len(v),
+            "bytes_per_launch": b, "min_bytes_per_launch": bpc * nm,
+            "ratio_to_min": b / (bpc * nm), "note": note}
+    return {"calibration": {"kernel": "tl::k_copy_all (tlhip_stream_copy), 1 GiB read + 1 GiB "
+                                      "written", "launches": len(calib),
+                            "FETCH_SIZE_KiB": f_kib, "WRITE_SIZE_KiB": w_kib,
+                            "fetch_scale": fs, "write_scale": ws},
+            "kernels": kernels}
+
+
+def traffic_main(fetch_csv: str, write_csv: str, commit: str) -> None:
+    """--write-traffic: the summary as JSON (read by load_traffic) and as a
+    markdown table beside it."""
+    doc = write_traffic(fetch_csv, write_csv)
+    cmds = ["rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- "
+            "python3 bench.py --profile-run",
+            "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- "
+            "python3 bench.py --profile-run",
+            f"python3 bench.py --write-traffic {fetch_csv} {write_csv} --traffic-commit {commit}"]
+    out = {"workload": "bench.py --profile-run: 2D 4096^2 interior (halo 2), 3D 256^3 interior "
+                       "(halo 2), and one FP32-refined PPCG timestep at 2048^2",
+           "commit": commit, "date": time.strftime("%Y-%m-%d"), "commands": cmds, **doc}
+    path = ROOT / TRAFFIC_JSON
+    path.write_text(json.dumps(out, indent=1) + "\n")
+    c = doc["calibration"]
+    lines = [
+        "# HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE)", "",
+        f"Kernels as of commit {commit}; MI355X, ROCm 7.2. Commands, from the repository root "
+        "(rocprofv3 run from /tmp with TMPDIR=/tmp, one counter per run):", "", "```",
+        *cmds, "```", "",
+        "`bench.py --profile-run` launches each entry point three times on synthetic fields "
+        "(2D at 4096², 3D at 256³), then one FP32-refined PPCG timestep at 2048² (the FP32 "
+        "kernels, many launches each), with a 1 GiB device copy opening each 2D round. "
+        f"The copy reads and writes 1 GiB: FETCH_SIZE reads {c['FETCH_SIZE_KiB']:.0f} KiB "
+        f"(scale {c['fetch_scale']:.3f}), WRITE_SIZE {c['WRITE_SIZE_KiB']:.0f} KiB "
+        f"(scale {c['write_scale']:.3f}). Bytes = 1024 × (fetch scale × FETCH_SIZE + write "
+        "scale × WRITE_SIZE), averaged over the launches. The minimum is the call's "
+        "compulsory traffic: each array it reads once plus each array it writes once, over "
+        "the interior cells. Auxiliary launches (one-block reduction finishers, the ring "
+        "copies of the fused loops' halo rows) are under 0.3% of any row and are not "
+        "included. `" + TRAFFIC_JSON.split("/")[-1] + "` holds the numbers.", "",
+        "| call | size | HIP kernel(s) | launches | measured MB | minimum MB | ratio | what it moves |",
+        "|---|---|---|---|---|---|---|---|"]
+    for name, v in doc["kernels"].items():
+        lines.append(f"| `{name}` | {v['size']} | " + ", ".join(f"`{h.rstrip('(')}`" for h in v["hip_kernels"])
+                     + f" | {v['launches']} | {v['bytes_per_launch'] / 1e6:.1f} | "
+                     f"{v['min_bytes_per_launch'] / 1e6:.1f} | {v['ratio_to_min']:.3f} | {v['note']} |")
+    path.with_suffix(".md").write_text("\n".join(lines) + "\n")
+    print(f"wrote {path.relative_to(ROOT)} and {path.with_suffix('.md').relative_to(ROOT)}")
 
 
 def blocks3d_run(args):
@@ -525,6 +794,44 @@ def adjoint_run(args):
             **out}
 
 
+def kernel_roofline(name: str, k: dict, res: dict, n_int: int) -> dict:
+    """Roofline of one timed entry point: achieved = its compulsory HBM
+    bytes per launch (each array read once + each written once, over the
+    interior cells) / its average launch time from HIP events on the suite's
+    stream; traffic = the counter-measured bytes per launch from the
+    committed rocprofv3 summary."""
+    t = k["avg_ms"] * 1e-3
+    out = {}
+    init_share = 0.0
+    if name == "tlhip2d_ppcg_inner_iterations":
+        sw = res["by_solver"]["ppcg"]["sweeps"]
+        # A launch that starts an outer iteration (cg_calc_ur + ppcg_init +
+        # the inner steps) reads r, w, u, p, kx, ky and writes sd', r', u'
+        # (72 B/cell); one that continues from sd reads sd, kx, ky, r, u and
+        # writes the same (64 B/cell).
+        init_share = min(1.0, sw["ppcg_outer"] / k["launches"])
+        min_cell = 64.0 + 8.0 * init_share
+        # the bytes the same steps would move unfused: 64 B/cell per inner
+        # step plus cg_calc_ur (48) and ppcg_init (16) per outer iteration
+        unfused_cell = (64.0 * sw["ppcg_inner"] + (48.0 + 16.0) * sw["ppcg_outer"]) / k["launches"]
+        out["inner_steps_per_launch"] = sw["ppcg_inner"] / k["launches"]
+        # the rate an unfused sequence of the same steps would need (not a
+        # fraction of any peak: one launch does several steps' work)
+        out["unfused_equivalent_gbs"] = unfused_cell * n_int / t / 1e9
+    else:
+        min_cell = float(BYTES_PER_CELL[name])
+    achieved = min_cell * n_int / t / 1e9
+    traffic, tsrc = load_traffic(name, init_share)
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": tsrc, "bytes_per_cell": min_cell,
+            "algorithmic_bytes": min_cell * n_int, "avg_kernel_ms": k["avg_ms"],
+            "launches": k["launches"], "share_of_timed_time": k["share_of_timed_time"],
+            # what the launch moves through HBM by the counters, and at what rate
+            "hbm_gbs": traffic / t / 1e9 if traffic else None,
+            "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None, **out}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -532,6 +839,9 @@ def main():
               "(one chunk per GPU, halo faces and two scalar all-reduces per CG iteration) "
               "is a design, not a build.", file=sys.stderr)
         sys.exit(2)
+    if args.write_traffic:
+        traffic_main(*args.write_traffic, args.traffic_commit)
+        return
     if args.profile_run:
         profile_run(args)
         return
@@ -546,40 +856,13 @@ def main():
     adjoint = None if args.no_adjoint else adjoint_run(args)
     value = res["iters"] / res["time"]
     n_int = args.cells * args.cells
-    dom = dominant(args)
-    k = res["kernels"].get(dom)
-    roofline = None
-    if k and k["launches"]:
-        sw = res["by_solver"]["ppcg"]["sweeps"]
-        t = k["avg_ms"] * 1e-3
-        if dom == "tlhip2d_ppcg_inner_iterations":
-            # Compulsory HBM bytes of one launch: every array it reads once and
-            # every array it writes once.  A launch that starts an outer
-            # iteration (cg_calc_ur + ppcg_init + the inner steps) reads r, w,
-            # u, p, kx, ky and writes sd', r', u' (72 B/cell); one that
-            # continues from sd reads sd, kx, ky, r, u and writes the same
-            # (64 B/cell).
-            init_share = min(1.0, sw["ppcg_outer"] / k["launches"])
-            min_cell = 64.0 + 8.0 * init_share
-            # the bytes the same steps would move unfused: 64 B/cell per inner
-            # step plus cg_calc_ur (48) and ppcg_init (16) per outer iteration
-            unfused_cell = (64.0 * sw["ppcg_inner"] + (48.0 + 16.0) * sw["ppcg_outer"]) / k["launches"]
-        else:
-            init_share, min_cell, unfused_cell = 0.0, 64.0, 64.0
-        achieved = min_cell * n_int / t / 1e9
-        traffic, tsrc = load_traffic(dom, init_share)
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                    "traffic_source": tsrc,
-                    "bytes_per_cell": min_cell, "algorithmic_bytes": min_cell * n_int,
-                    "avg_kernel_ms": k["avg_ms"], "launches": k["launches"],
-                    "inner_steps_per_launch": sw["ppcg_inner"] / k["launches"],
-                    # what the launch moves through HBM by the counters, and at what rate
-                    "hbm_gbs": traffic / t / 1e9 if traffic else None,
-                    "hbm_frac": traffic / t / 1e9 / HBM_PEAK_GBS if traffic else None,
-                    # the rate an unfused sequence of the same steps would need (not a
-                    # fraction of any peak: one launch does several steps' work)
-                    "unfused_equivalent_gbs": unfused_cell * n_int / t / 1e9}
+    # roofline: the timed kernel with the most total time over the three legs
+    timed = {n: k for n, k in res["kernels"].items() if k["launches"]}
+    dom = max(timed, key=lambda n: timed[n]["total_ms"]) if timed else None
+    roofline = kernel_roofline(dom, timed[dom], res, n_int) if dom else None
+    fused = dominant(args)
+    roofline_ppcg = (kernel_roofline(fused, timed[fused], res, n_int)
+                     if fused in timed and fused != dom else None)
     cpu = None if args.no_cpu else cpu_baseline(args.cells, args.cpu_cg_iters,
                                                  args.cpu_cheby_iters, args.cpu_ppcg_outer)
     if cpu:
@@ -619,6 +902,7 @@ def main():
                                "(10 inner, presteps 30); one step = one timestep of each",
                    "cells": n_int, "ppcg_fusion": args.ppcg_fusion, "solvers": args.solvers, "eps": 1e-15, "max_iters": 10000},
         "roofline": roofline,
+        "roofline_ppcg_fused": roofline_ppcg,
         "cpu_baseline": cpu,
         "refine": refine,
         "adjoint": adjoint,

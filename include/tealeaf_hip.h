@@ -88,10 +88,13 @@ int tlhip_copy_to_device(void* dst, const void* src, size_t bytes);
 int tlhip_copy_to_host(void* dst, const void* src, size_t bytes);
 int tlhip_copy_device(void* dst, const void* src, size_t bytes);
 /* Kernel timer: after start, every launch of the main kernel of the named
- * entry point (e.g. "tlhip2d_ppcg_inner_iteration") is bracketed by HIP
- * events on the suite's stream; read synchronises, returns the summed
- * kernel time and the launch count, and stops timing. */
-int tlhip_kernel_timer_start(const char* entry_point);
+ * entry points (one name, e.g. "tlhip2d_ppcg_inner_iteration", or a
+ * comma-separated list of up to 8) is bracketed by HIP events on the
+ * suite's stream.  read_entry synchronises and returns one entry point's
+ * summed kernel time and launch count so far; read does the same for the
+ * first name and stops timing. */
+int tlhip_kernel_timer_start(const char* entry_points);
+int tlhip_kernel_timer_read_entry(const char* entry_point, double* total_ms, long* launches);
 int tlhip_kernel_timer_read(double* total_ms, long* launches);
 /* A plain streaming copy (n doubles) on the suite's stream, timed under the
  * name "tlhip_stream_copy": the reference point for HBM byte counters. */

@@ -32,11 +32,15 @@ struct Runtime {
 
 // Event pairs around every launch of one entry point's main kernel, for
 // measuring average kernel duration on the suite's own stream.
+// Up to kTimerNames entry points timed at once (tlhip_kernel_timer_start
+// takes a comma-separated list).
+constexpr int kTimerNames = 8;
 struct KernelTimer {
-    char name[128] = "";
+    char name[kTimerNames][64] = {};
+    int nnames = 0;
     int on = 0;
-    std::vector<hipEvent_t> ev;     // 2 per timed launch
-    size_t used = 0;                // launches recorded
+    std::vector<hipEvent_t> ev[kTimerNames];   // 2 per timed launch
+    size_t used[kTimerNames] = {};             // launches recorded
 };
 
 Runtime g_rt;
@@ -267,32 +271,80 @@ bool tl_timer_active() { return g_timer.on != 0; }
 
 int tl_timer_begin(const char* what)
 {
-    if (!g_timer.on || strcmp(what, g_timer.name) != 0) return -1;
-    const size_t k = g_timer.used;
-    while (g_timer.ev.size() < 2 * (k + 1)) {
+    if (!g_timer.on) return -1;
+    int idx = 0;
+    while (idx < g_timer.nnames && strcmp(what, g_timer.name[idx]) != 0) ++idx;
+    if (idx == g_timer.nnames) return -1;
+    std::vector<hipEvent_t>& ev = g_timer.ev[idx];
+    const size_t k = g_timer.used[idx];
+    if (k >= (1u << 24)) return -1;
+    while (ev.size() < 2 * (k + 1)) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return -1;
-        g_timer.ev.push_back(e);
+        ev.push_back(e);
     }
-    if (hipEventRecord(g_timer.ev[2 * k], g_rt.stream) != hipSuccess) return -1;
-    return (int)k;
+    if (hipEventRecord(ev[2 * k], g_rt.stream) != hipSuccess) return -1;
+    return (idx << 24) | (int)k;
 }
 
-void tl_timer_end(int k)
+void tl_timer_end(int code)
 {
-    if (k < 0) return;
-    if (hipEventRecord(g_timer.ev[2 * k + 1], g_rt.stream) == hipSuccess) g_timer.used = k + 1;
+    if (code < 0) return;
+    const int idx = code >> 24;
+    const size_t k = (size_t)(code & 0xFFFFFF);
+    if (hipEventRecord(g_timer.ev[idx][2 * k + 1], g_rt.stream) == hipSuccess)
+        g_timer.used[idx] = k + 1;
 }
 
-extern "C" int tlhip_kernel_timer_start(const char* entry_point)
+extern "C" int tlhip_kernel_timer_start(const char* entry_points)
 {
     TL_TRY(tl_ensure_init());
-    if (!entry_point || strlen(entry_point) >= sizeof(g_timer.name))
-        return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_start: bad name");
-    strcpy(g_timer.name, entry_point);
+    if (!entry_points) return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_start: null name");
+    int n = 0;
+    const char* p = entry_points;
+    while (true) {
+        const char* q = strchr(p, ',');
+        const size_t len = q ? (size_t)(q - p) : strlen(p);
+        if (len == 0 || len >= sizeof(g_timer.name[0]) || n == kTimerNames)
+            return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_start: bad name list '%s'",
+                           entry_points);
+        memcpy(g_timer.name[n], p, len);
+        g_timer.name[n][len] = 0;
+        ++n;
+        if (!q) break;
+        p = q + 1;
+    }
+    g_timer.nnames = n;
+    for (int i = 0; i < kTimerNames; ++i) g_timer.used[i] = 0;
     g_timer.on = 1;
-    g_timer.used = 0;
     return 0;
+}
+
+static int timer_sum(int idx, double* total_ms, long* launches)
+{
+    double sum = 0.0;
+    for (size_t k = 0; k < g_timer.used[idx]; ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, g_timer.ev[idx][2 * k], g_timer.ev[idx][2 * k + 1]),
+                "hipEventElapsedTime");
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (long)g_timer.used[idx];
+    return 0;
+}
+
+extern "C" int tlhip_kernel_timer_read_entry(const char* entry_point, double* total_ms,
+                                             long* launches)
+{
+    TL_TRY(tl_ensure_init());
+    if (!entry_point || !total_ms || !launches)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_read_entry: null argument");
+    HIP_TRY(hipStreamSynchronize(g_rt.stream), "timer sync");
+    for (int i = 0; i < g_timer.nnames; ++i)
+        if (strcmp(entry_point, g_timer.name[i]) == 0) return timer_sum(i, total_ms, launches);
+    return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_read_entry: '%s' is not being timed",
+                   entry_point);
 }
 
 extern "C" int tlhip_kernel_timer_read(double* total_ms, long* launches)
@@ -300,16 +352,14 @@ extern "C" int tlhip_kernel_timer_read(double* total_ms, long* launches)
     TL_TRY(tl_ensure_init());
     if (!total_ms || !launches) return tl_fail(TLHIP_ERR_ARG, "tlhip_kernel_timer_read: null argument");
     HIP_TRY(hipStreamSynchronize(g_rt.stream), "timer sync");
-    double sum = 0.0;
-    for (size_t k = 0; k < g_timer.used; ++k) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, g_timer.ev[2 * k], g_timer.ev[2 * k + 1]), "hipEventElapsedTime");
-        sum += ms;
+    if (g_timer.nnames > 0) {
+        TL_TRY(timer_sum(0, total_ms, launches));
+    } else {
+        *total_ms = 0.0;
+        *launches = 0;
     }
-    *total_ms = sum;
-    *launches = (long)g_timer.used;
     g_timer.on = 0;
-    g_timer.used = 0;
+    for (int i = 0; i < kTimerNames; ++i) g_timer.used[i] = 0;
     return 0;
 }
 
