@@ -189,14 +189,31 @@ def refine_run(args):
             prob.inject(dens, en)
             prob.set_solve(prec, 1e-10)
             times, reps = [], []
+            # the last step also times its main kernels (HIP events)
+            names = (["tlhip2d_mix_calc_pw32", "tlhip2d_ppcg_inner_iterations_f32",
+                      "tlhip2d_mix_norm32"] if prec else
+                     ["tlhip2d_cg_calc_pw", "tlhip2d_ppcg_inner_iterations",
+                      "tlhip2d_calculate_2norm"])
+            kern = {}
             for tt in range(args.refine_steps):
+                last = tt == args.refine_steps - 1
                 L.tlhip_synchronize()
                 torch.cuda.synchronize()
+                if last:
+                    L.tlhip_kernel_timer_start(",".join(names).encode())
                 t0 = time.perf_counter()
                 reps.append(prob.step(tt))
                 L.tlhip_synchronize()
                 torch.cuda.synchronize()
                 times.append(time.perf_counter() - t0)
+                if last:
+                    for nm in names:
+                        ms, nl = ctypes.c_double(), ctypes.c_long()
+                        L.tlhip_kernel_timer_read_entry(nm.encode(), ctypes.addressof(ms),
+                                                        ctypes.addressof(nl))
+                        kern[nm] = {"launches": nl.value, "total_ms": ms.value,
+                                    "avg_ms": ms.value / nl.value if nl.value else None}
+                    L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
                 print(f"[bench] refine contrast {contrast:g} {'fp32' if prec else 'fp64'} step {tt}: "
                       f"{times[-1]:.3f} s, {reps[-1]['n_cg_steps']} CG + {reps[-1]['n_ppcg_outer']} PPCG, "
                       f"rel residual {reps[-1]['rel_residual']:.2e}", file=sys.stderr, flush=True)
@@ -207,7 +224,8 @@ def refine_run(args):
                 "cg_iters": [r["n_cg_steps"] for r in reps],
                 "ppcg_outer": [r["n_ppcg_outer"] for r in reps],
                 "true_residual_evals": [r["n_true_residuals"] for r in reps],
-                "rel_residual": [r["rel_residual"] for r in reps]}
+                "rel_residual": [r["rel_residual"] for r in reps],
+                "last_step_kernels": kern}
         hd = s.halo_depth
         a = u_final[1].reshape(s.shape())[hd:-hd, hd:-hd]
         b = u_final[0].reshape(s.shape())[hd:-hd, hd:-hd]

@@ -202,14 +202,28 @@ int tlhip2d_ppcg_inner_iterations(int x, int y, int halo_depth, int count,
 /* The same loop on FP32 fields (the inner sweeps of the mixed-precision
  * solve): rows of `pitch` floats (a multiple of 4, >= x), interior
  * [halo_depth, x-halo_depth) x [halo_depth, y-halo_depth).  Coefficients
- * and theta are rounded to FP32; every operation is the FP32 one of the
- * reference's expression, in the same order. */
+ * and theta are rounded to FP32.  The reference's expressions in its order,
+ * with each product that feeds an addition fused into one FP32
+ * multiply-add: kxr*rgt + kxc*lft and kyu*up + kyc*dn as fma(a, b, c*d);
+ * A sd = fma(diag, sd, -x terms) - y terms; sd' = fma(alpha, sd, beta*r');
+ * r - alpha w and u + alpha p as one fma each; sd = r * (1/theta). */
 int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int halo_depth, int pitch,
         int count, const double* alphas, const double* betas, double theta,
         int init_from_r, const int* neighbours, const float* sd_in,
         float* sd_out, const float* r_in, float* r_out, const float* u_in,
         float* u_out, const float* kx, const float* ky, double alpha,
         const float* p, const float* w);
+/* No counterpart in the reference (whose PPCG driver calls
+ * calculate_2norm after the inner steps, 2d/drivers/ppcg_driver.c:121-131):
+ * tlhip2d_ppcg_inner_iterations_f32 that also sets *rrn (host) to the
+ * squared 2-norm, accumulated in FP64, of the interior of r_out, formed in
+ * the same launch. */
+int tlhip2d_ppcg_inner_iterations_f32_norm(int x, int y, int halo_depth, int pitch,
+        int count, const double* alphas, const double* betas, double theta,
+        int init_from_r, const int* neighbours, const float* sd_in,
+        float* sd_out, const float* r_in, float* r_out, const float* u_in,
+        float* u_out, const float* kx, const float* ky, double alpha,
+        const float* p, const float* w, double* rrn);
 /* run_cg_calc_p (2d/kernel_interface.h:42), a depth-1 halo update of p and
  * run_cg_calc_w (:38) in one pass, for a chunk whose faces are external:
  * p_out = beta p_in + r, w = A p_out, *pw += sum w p_out.  p_out and w equal
@@ -237,10 +251,10 @@ int tlhip2d_cheby_iterations(int x, int y, int halo_depth, int count,
  * once, from the kernel's occupancy and the CU count) and prefetch distance
  * in rows (0-3; < 0: default 1; the Chebyshev loop uses 0 or 1). */
 int tlhip_set_fused_tuning(int tile_rows, int prefetch);
-/* Which kernel the fused PPCG loop launches: 0 (default) keeps each
+/* Which kernel the FP64 fused PPCG loop launches: 0 (default) keeps each
  * row's diagonal, kx and ky in a per-wave LDS ring (prefetch fixed at one
  * row); 1 keeps them in register windows (the prefetch setting applies).
- * Results are bit-identical. */
+ * Results are bit-identical.  The FP32 loop always runs the ring kernel. */
 int tlhip_set_fused_variant(int variant);
 /* Tuning of the LDS-ring PPCG kernel: its tiles (one wave each) number
  * about fill x the waves the device holds at once (<= 0: default 0.95,

@@ -149,6 +149,7 @@ struct tlhip_problem {
     float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *w32 = nullptr, *r32 = nullptr,
           *d32 = nullptr, *sd32 = nullptr, *sd32b = nullptr, *r32b = nullptr, *d32b = nullptr,
           *p32b = nullptr;
+    double* alpha32 = nullptr;   // device: a PPCG step's cg_calc_ur alpha (rro / pw)
     // A CG/PPCG step's calc_p waits to be fused into the next step's calc_w
     // (tlhip2d_cg_calc_pw); flush_p() applies it where p is read otherwise.
     bool pending_p = false;
@@ -975,6 +976,7 @@ struct tlhip_problem {
         const size_t bytes = sizeof(float) * (size_t)P * (size_t)y;
         float** all[] = {&kx32, &ky32, &p32, &w32, &r32, &d32, &sd32, &sd32b, &r32b, &d32b, &p32b};
         for (float** f : all) TRY(tlhip_alloc(bytes, (void**)f));
+        TRY(tlhip_alloc(sizeof(double), (void**)&alpha32));
         return 0;
     }
 
@@ -984,6 +986,21 @@ struct tlhip_problem {
         if (!pending_p) return tl::mix::calc_w32(x, y, hd, P, pw, p32, w32, kx32, ky32);
         pending_p = false;
         TRY(tl::mix::calc_pw32(x, y, hd, P, pending_beta, pw, p32, p32b, r32, w32, kx32, ky32));
+        std::swap(p32, p32b);
+        return 0;
+    }
+
+    // The same, with alpha = rro / pw formed on the device into alpha32 and
+    // no wait for pw: a PPCG step needs alpha only inside its first fused
+    // launch.
+    int calc_w32_pending_alpha(double rro)
+    {
+        if (!pending_p)
+            return tl::mix::calc_pw32_alpha(x, y, hd, P, false, 0.0, rro, alpha32, p32, nullptr,
+                                            nullptr, w32, kx32, ky32);
+        pending_p = false;
+        TRY(tl::mix::calc_pw32_alpha(x, y, hd, P, true, pending_beta, rro, alpha32, p32, p32b,
+                                     r32, w32, kx32, ky32));
         std::swap(p32, p32b);
         return 0;
     }
@@ -1027,32 +1044,34 @@ struct tlhip_problem {
 
     int mixed_ppcg_step(double* rro, double* rrn)
     {
-        double pw = 0.0;
-        TRY(calc_w32_pending(&pw));
-        const double alpha = *rro / pw;
         if (s.ppcg_inner_steps == 0) {
-            double unused = 0.0;
-            TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, &unused, d32, p32, r32, w32));
+            double pw = 0.0, unused = 0.0;
+            TRY(calc_w32_pending(&pw));
+            TRY(tl::mix::calc_ur32(x, y, hd, P, *rro / pw, &unused, d32, p32, r32, w32));
+        } else {
+            TRY(calc_w32_pending_alpha(*rro));    // alpha stays on the device
         }
         const int per = std::max(1, g_ppcg_fusion);
         int done = 0;
+        *rrn = 0.0;
         while (done < s.ppcg_inner_steps) {
-            // the first launch also applies calc_ur (u is d here)
+            // the first launch also applies calc_ur (u is d here); the last
+            // also forms rrn = r.r
             const int cnt = std::min(per, s.ppcg_inner_steps - done);
-            const bool first = done == 0;
-            TRY(tlhip2d_ppcg_inner_iterations_f32(x, y, hd, P, cnt, &cheby_alphas[done],
-                                                  &cheby_betas[done], theta, first, neighbours,
-                                                  first ? nullptr : sd32, sd32b, r32, r32b, d32,
-                                                  d32b, kx32, ky32, alpha, first ? p32 : nullptr,
-                                                  first ? w32 : nullptr));
+            const bool first = done == 0, last = done + cnt == s.ppcg_inner_steps;
+            TRY(tl::mix::ppcg_inner32_norm(x, y, hd, P, cnt, &cheby_alphas[done],
+                                           &cheby_betas[done], theta, first, neighbours,
+                                           first ? nullptr : sd32, sd32b, r32, r32b, d32, d32b,
+                                           kx32, ky32, 0.0, first ? p32 : nullptr,
+                                           first ? w32 : nullptr, first ? alpha32 : nullptr,
+                                           last ? rrn : nullptr));
             std::swap(sd32, sd32b);
             std::swap(r32, r32b);
             std::swap(d32, d32b);
             done += cnt;
             rep->n_ppcg_inner += cnt;
         }
-        *rrn = 0.0;
-        TRY(tl::mix::norm32(x, y, hd, P, r32, rrn));
+        if (done == 0) TRY(tl::mix::norm32(x, y, hd, P, r32, rrn));
         const double beta = *rrn / *rro;
         pending_p = true;
         pending_beta = beta;
@@ -1345,6 +1364,7 @@ struct tlhip_problem {
         float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b, p32b};
         for (float* a : all32)
             if (a) (void)tlhip_free(a);
+        if (alpha32) (void)tlhip_free(alpha32);
         if (adopted) {      // the caller's fields stay; only the partners of r and p are ours
             for (double* a : {r == home[2] ? r_alt : r, p == home[3] ? p_alt : p})
                 if (a) (void)tlhip_free(a);

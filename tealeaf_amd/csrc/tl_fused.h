@@ -43,6 +43,9 @@ struct FusedArgs {
     const T* p;
     const T* w;
     const char* what;
+    // Non-null: the launch also forms the squared 2-norm of the r it writes
+    // (the interior), fetched to *rrn (FP32 ring kernel only).
+    double* rrn = nullptr;
 };
 
 // Compute units of the current device (cached; the runtime uses one device).

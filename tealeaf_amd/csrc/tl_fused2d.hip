@@ -861,7 +861,9 @@ int launch_fused(const FusedArgs<T>& a)
 template <class T, int MODE>
 int dispatch_fused(int count, const FusedArgs<T>& a)
 {
-    if (g_fused_variant == 0) return launch_ppcg_ring<T>(count, MODE, a);
+    // FP32 always takes the ring kernel (its FP32 arithmetic uses fused
+    // multiply-adds; the register-window kernel's does not)
+    if (g_fused_variant == 0 || std::is_same_v<T, float>) return launch_ppcg_ring<T>(count, MODE, a);
     switch (count) {
     case 1: return launch_fused<T, 1, MODE>(a);
     case 2: return launch_fused<T, 2, MODE>(a);
@@ -882,10 +884,13 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
                      const double* alphas, const double* betas, double theta, int init_from_r,
                      const int* neighbours, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
                      const T* u_in, T* u_out, const T* kx, const T* ky, double alpha_ur,
-                     const T* p, const T* w, const double* alpha_dev = nullptr)
+                     const T* p, const T* w, const double* alpha_dev = nullptr,
+                     double* rrn = nullptr, bool copy_ring = true)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
+    if (rrn && !std::is_same_v<T, float>)
+        return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm is FP32 only", what);
     if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || pitch < x || pitch % N ||
         (long)pitch * y > (1L << 31) - 1)
         return tl_fail(TLHIP_ERR_ARG,
@@ -916,11 +921,13 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
         rg.n = 3;
     }
     const long ring = ((long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny) * rg.n;
-    hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0, tl_stream(), g,
-                       y, rg, tl_gate());
-    TL_TRY(tl_check_launch(what));
+    if (copy_ring) {
+        hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
+                           tl_stream(), g, y, rg, tl_gate());
+        TL_TRY(tl_check_launch(what));
+    }
     const FusedArgs<T> a{g, alphas, betas, theta, alpha_ur, alpha_dev, sd_in, sd_out, r_in, r_out,
-                         u_in, u_out, kx, ky, p, w, what};
+                         u_in, u_out, kx, ky, p, w, what, rrn};
     if (!init_from_r) return dispatch_fused<T, 0>(count, a);
     if (!p) return dispatch_fused<T, 1>(count, a);
     return dispatch_fused<T, 2>(count, a);
@@ -1087,6 +1094,22 @@ int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, doub
 
 }  // namespace
 
+namespace {
+
+// alpha = rro / pw, pw the grid total of the partials (the finisher's
+// order), for a launch that takes alpha from the device.
+__global__ void __launch_bounds__(256)
+k_mix_alpha(unsigned nblocks, const double* __restrict__ partials, double rro,
+            double* __restrict__ alpha, TlGate gate)
+{
+    if (gate.closed()) return;
+    double pw[1];
+    tl_sum_partials<1>(nblocks, partials, pw);
+    if (threadIdx.x == 0) alpha[0] = rro / pw[0];
+}
+
+}  // namespace
+
 namespace tl {
 namespace loop2d {
 
@@ -1137,6 +1160,20 @@ int calc_pw32(int x, int y, int hd, int P, double beta, double* pw, const float*
     *pw = 0.0;
     return calc_pw<float>("tlhip2d_mix_calc_pw32", x, y, hd, P, true, beta, pw, p_in, p_out, r,
                           w, kx, ky);
+}
+
+int calc_pw32_alpha(int x, int y, int hd, int P, bool update, double beta, double rro,
+                    double* alpha_dev, const float* p_in, float* p_out, const float* r, float* w,
+                    const float* kx, const float* ky)
+{
+    const char* what = update ? "tlhip2d_mix_calc_pw32" : "tlhip2d_mix_calc_w32";
+    TL_REQUIRE(alpha_dev, what, "null alpha");
+    unsigned nb = 0;
+    TL_TRY(launch_pw<float>(what, x, y, hd, P, update, beta, nullptr, p_in, p_out, r, w, kx, ky,
+                            &nb, false));
+    hipLaunchKernelGGL(k_mix_alpha, dim3(1), dim3(256), 0, tl_stream(), nb,
+                       (const double*)tl_reduce_slots().partials, rro, alpha_dev, tl_gate());
+    return tl_check_launch(what);
 }
 
 }  // namespace mix
@@ -1194,6 +1231,37 @@ extern "C" int tlhip2d_ppcg_inner_iterations_f32(int x, int y, int hd, int pitch
                                    alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
                                    r_in, r_out, u_in, u_out, kx, ky, alpha, p, w);
 }
+
+extern "C" int tlhip2d_ppcg_inner_iterations_f32_norm(
+    int x, int y, int hd, int pitch, int count, const double* alphas, const double* betas,
+    double theta, int init_from_r, const int* neighbours, const float* sd_in, float* sd_out,
+    const float* r_in, float* r_out, const float* u_in, float* u_out, const float* kx,
+    const float* ky, double alpha, const float* p, const float* w, double* rrn)
+{
+    if (!rrn) return tl_fail(TLHIP_ERR_ARG, "tlhip2d_ppcg_inner_iterations_f32_norm: null rrn");
+    return inner_iterations<float>("tlhip2d_ppcg_inner_iterations_f32", x, y, hd, pitch, count,
+                                   alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
+                                   r_in, r_out, u_in, u_out, kx, ky, alpha, p, w, nullptr, rrn);
+}
+
+namespace tl {
+namespace mix {
+
+int ppcg_inner32_norm(int x, int y, int hd, int pitch, int count, const double* alphas,
+                      const double* betas, double theta, int init_from_r, const int* neighbours,
+                      const float* sd_in, float* sd_out, const float* r_in, float* r_out,
+                      const float* u_in, float* u_out, const float* kx, const float* ky,
+                      double alpha, const float* p, const float* w, const double* alpha_dev,
+                      double* rrn)
+{
+    return inner_iterations<float>("tlhip2d_ppcg_inner_iterations_f32", x, y, hd, pitch, count,
+                                   alphas, betas, theta, init_from_r, neighbours, sd_in, sd_out,
+                                   r_in, r_out, u_in, u_out, kx, ky, alpha, p, w, alpha_dev, rrn,
+                                   false);
+}
+
+}  // namespace mix
+}  // namespace tl
 
 extern "C" int tlhip2d_cheby_iterations(int x, int y, int hd, int count, const double* alphas,
                                         const double* betas, const int* neighbours,

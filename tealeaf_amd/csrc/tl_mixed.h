@@ -26,6 +26,22 @@ int calc_p32(int x, int y, int hd, int P, double beta, bool init, float* p, cons
 int norm32(int x, int y, int hd, int P, const float* a, double* out);
 // u += d (FP64); d = 0
 int promote(int x, int y, int hd, int P, double* u, float* d);
+// calc_w32 (update false) or calc_pw32 (true) with *alpha_dev = rro / pw
+// written on the device and no wait for the result; p_out's halo is not
+// written (no kernel here reads an FP32 halo)
+int calc_pw32_alpha(int x, int y, int hd, int P, bool update, double beta, double rro,
+                    double* alpha_dev, const float* p_in, float* p_out, const float* r, float* w,
+                    const float* kx, const float* ky);
+// tlhip2d_ppcg_inner_iterations_f32, with cg_calc_ur's alpha read from the
+// device (alpha_dev, when not null), and, when rrn is not null, also forming
+// *rrn = sum r*r (FP64) over the r it writes in the same launch; the
+// outputs' halos are not written
+int ppcg_inner32_norm(int x, int y, int hd, int pitch, int count, const double* alphas,
+                      const double* betas, double theta, int init_from_r, const int* neighbours,
+                      const float* sd_in, float* sd_out, const float* r_in, float* r_out,
+                      const float* u_in, float* u_out, const float* kx, const float* ky,
+                      double alpha, const float* p, const float* w, const double* alpha_dev,
+                      double* rrn);
 // r32 = (float)(u0 - A u) computed in FP64; *rr = sum (u0 - A u)^2 (FP64)
 int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
                const double* ky, float* r32, double* rr);

@@ -109,6 +109,13 @@ __device__ __forceinline__ float shl1(float v)
     return __builtin_bit_cast(float, shl1_i(__builtin_bit_cast(int, v)));
 }
 
+// a * b + c.  FP64: two roundings, the reference's arithmetic.  FP32 (the
+// inner sweeps of the FP32-refined solve, which the FP64 residual
+// replacement corrects): one fused multiply-add, fewer instructions in an
+// issue-bound loop.
+__device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // A * a for a lane's N cells (2d/shared.h:50-54, same association) with
 // the diagonal and the next lane's kx formed beforehand, and x neighbours
 // outside the interior mirroring the cell (XM 0: the wave's window holds
@@ -142,7 +149,8 @@ __device__ __forceinline__ Cells<T> apply_a_ring(const Cells<T>& cen, const Cell
             if (c + i == xin1 - 1) rgt = cen.v[i];
         }
         const T kxr = i == N - 1 ? kxe : kxc.v[i + 1];
-        a.v[i] = d.v[i] * cen.v[i] - (kxr * rgt + kxc.v[i] * lft) - (kyu.v[i] * up.v[i] + kyc.v[i] * dn.v[i]);
+        const T tx = mad(kxr, rgt, kxc.v[i] * lft), ty = mad(kyu.v[i], up.v[i], kyc.v[i] * dn.v[i]);
+        a.v[i] = mad(d.v[i], cen.v[i], -tx) - ty;
     }
     return a;
 }
@@ -155,7 +163,9 @@ using Phase = std::integral_constant<int, P>;
 // waves, and the tile count can be sized to the SIMDs the device has.
 constexpr int kRingBlock = TL_WAVE;
 
-template <class T, int S, int MODE>
+// NORM: also the squared 2-norm of the r written (the kept interior cells),
+// as one FP64 partial per block into rd (summed by the finisher).
+template <class T, int S, int MODE, bool NORM>
 __global__ void __launch_bounds__(kRingBlock)
 k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg,
               const double* __restrict__ alpha_dev,
@@ -163,7 +173,7 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
               const T* __restrict__ r_in, T* __restrict__ r_out,
               const T* __restrict__ u_in, T* __restrict__ u_out,
               const T* __restrict__ kx, const T* __restrict__ ky,
-              const T* __restrict__ p_in, const T* __restrict__ w_in, TlGate gate)
+              const T* __restrict__ p_in, const T* __restrict__ w_in, TlGate gate, TlReduce rd)
 {
     using RG = RingGeom<S>;
     constexpr int R = RG::R;
@@ -218,20 +228,25 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
         return __builtin_bit_cast(
             C, __builtin_amdgcn_raw_buffer_load_b128(f, lane_off, row_of(row), 0));
     };
+    // sd = r / theta (ppcg.c:19-26); FP32 multiplies by 1/theta
+    const T inv_theta = (T)1 / theta;
     auto over_theta = [&](C a) -> C {
 #pragma unroll
-        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] / theta;
+        for (int i = 0; i < N; ++i) {
+            if constexpr (std::is_same_v<T, double>) a.v[i] = a.v[i] / theta;
+            else a.v[i] = a.v[i] * inv_theta;
+        }
         return a;
     };
     // r - alpha w and u + alpha p (cg.c:128-129, same association)
     auto minus_aw = [&](C a, const C& b) -> C {
 #pragma unroll
-        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] - alpha_ur * b.v[i];
+        for (int i = 0; i < N; ++i) a.v[i] = mad(-alpha_ur, b.v[i], a.v[i]);
         return a;
     };
     auto plus_ap = [&](C a, const C& b) -> C {
 #pragma unroll
-        for (int i = 0; i < N; ++i) a.v[i] = a.v[i] + alpha_ur * b.v[i];
+        for (int i = 0; i < N; ++i) a.v[i] = mad(alpha_ur, b.v[i], a.v[i]);
         return a;
     };
     auto r_level0 = [&](int row) -> C {
@@ -311,6 +326,7 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
         return v;
     };
     In pre = load_in(rs);
+    double racc = 0.0;    // NORM: this lane's sum of r^2 over the cells it stores
 
     // gen = 4 * XM + YM: YM the y-mirror tests, XM the x mirror (as
     // apply_a_ring)
@@ -397,7 +413,7 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
             for (int i = 0; i < N; ++i) {
                 rn.v[i] = rin.v[i] - av.v[i];
                 un.v[i] = uin.v[i] + cen.v[i];
-                sn.v[i] = co.al * cen.v[i] + co.be * rn.v[i];
+                sn.v[i] = mad(co.al, cen.v[i], co.be * rn.v[i]);
             }
             if (k < S) {
                 sdw[k][P2] = sn;
@@ -405,6 +421,12 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
                 uc[P1][k] = un;
             } else if (any_out && (unsigned)dst < (unsigned)(j1 - j0)) {
                 const int ro = rho * row_bytes;          // rho is an interior row here
+                if constexpr (NORM) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (all_out || (c + i >= o0 && c + i < o1))
+                            racc += (double)rn.v[i] * (double)rn.v[i];
+                }
                 if (all_out) {
                     using V = typename Lane16<T>::V;
                     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, rn), o_r,
@@ -485,17 +507,18 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
     } else {
         trips(0, niter3, GenXY{});
     }
+    if constexpr (NORM) tl_reduce_partial<1, kRingBlock>(&racc, rd);
 }
 
 // Default tile height: as many tiles (one wave each) as fill a fraction of
 // the waves the device holds at once -- one round; a second, mostly idle
 // round is the cliff.
-template <class T, int S, int MODE>
+template <class T, int S, int MODE, bool NORM>
 int ring_tile_rows(const PitchGrid& g, int ns)
 {
     static int occ = 0;
     if (occ <= 0 &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ppcg_ring2d<T, S, MODE>,
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ppcg_ring2d<T, S, MODE, NORM>,
                                                       kRingBlock, 0) != hipSuccess ||
          occ <= 0))
         occ = 1;
@@ -505,14 +528,14 @@ int ring_tile_rows(const PitchGrid& g, int ns)
     return (int)std::max(1L, ((long)g.ny + nby - 1) / nby);
 }
 
-template <class T, int S, int MODE>
+template <class T, int S, int MODE, bool NORM>
 int launch_ring(const FusedArgs<T>& a)
 {
     // Windows reach S columns beyond their output columns on both sides.
     StripTile t = strip_tile<T>(a.g, S, 1, 1);
-    const int ty = g_fused_ty > 0 ? g_fused_ty : ring_tile_rows<T, S, MODE>(a.g, t.ns);
+    const int ty = g_fused_ty > 0 ? g_fused_ty : ring_tile_rows<T, S, MODE, NORM>(a.g, t.ns);
     t = strip_tile<T>(a.g, S, ty, 1);
-    if ((long)t.nbx * t.nby > (1L << 30))
+    if ((long)t.nbx * t.nby > (NORM ? (long)TL_MAX_RED_BLOCKS : (1L << 30)))
         return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", a.what);
     PpcgCoefs<T, S> cf;
     for (int k = 0; k < S; ++k) {
@@ -521,29 +544,44 @@ int launch_ring(const FusedArgs<T>& a)
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
     const int slot = tl_timer_begin(a.what);
-    hipLaunchKernelGGL((k_ppcg_ring2d<T, S, MODE>), dim3(nb), dim3(kRingBlock), 0, tl_stream(),
-                       a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev, a.sd_in, a.sd_out,
-                       a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w, tl_gate());
+    hipLaunchKernelGGL((k_ppcg_ring2d<T, S, MODE, NORM>), dim3(nb), dim3(kRingBlock), 0,
+                       tl_stream(), a.g, t, cf, (T)a.theta, (T)a.alpha_ur, a.alpha_dev, a.sd_in,
+                       a.sd_out, a.r_in, a.r_out, a.u_in, a.u_out, a.kx, a.ky, a.p, a.w,
+                       tl_gate(), tl_reduce_slots());
     tl_timer_end(slot);
-    return tl_check_launch(a.what);
+    TL_TRY(tl_check_launch(a.what));
+    if constexpr (NORM) {
+        TL_TRY(tl_finish_reduce(1, nb, a.what));
+        return tl_fetch(1, a.rrn, a.what);
+    }
+    return 0;
 }
 
+template <class T, int MODE, bool NORM>
+int dispatch_ring_n(int count, const FusedArgs<T>& a)
+{
+    switch (count) {
+    case 1: return launch_ring<T, 1, MODE, NORM>(a);
+    case 2: return launch_ring<T, 2, MODE, NORM>(a);
+    case 3: return launch_ring<T, 3, MODE, NORM>(a);
+    case 4: return launch_ring<T, 4, MODE, NORM>(a);
+    case 5: return launch_ring<T, 5, MODE, NORM>(a);
+    case 6: return launch_ring<T, 6, MODE, NORM>(a);
+    case 7: return launch_ring<T, 7, MODE, NORM>(a);
+    case 8: return launch_ring<T, 8, MODE, NORM>(a);
+    case 9: return launch_ring<T, 9, MODE, NORM>(a);
+    case 10: return launch_ring<T, 10, MODE, NORM>(a);
+    default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", a.what, kMaxFused);
+    }
+}
+
+// The norm-forming kernels are built for FP32, the mixed-precision loop's.
 template <class T, int MODE>
 int dispatch_ring(int count, const FusedArgs<T>& a)
 {
-    switch (count) {
-    case 1: return launch_ring<T, 1, MODE>(a);
-    case 2: return launch_ring<T, 2, MODE>(a);
-    case 3: return launch_ring<T, 3, MODE>(a);
-    case 4: return launch_ring<T, 4, MODE>(a);
-    case 5: return launch_ring<T, 5, MODE>(a);
-    case 6: return launch_ring<T, 6, MODE>(a);
-    case 7: return launch_ring<T, 7, MODE>(a);
-    case 8: return launch_ring<T, 8, MODE>(a);
-    case 9: return launch_ring<T, 9, MODE>(a);
-    case 10: return launch_ring<T, 10, MODE>(a);
-    default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", a.what, kMaxFused);
-    }
+    if (!a.rrn) return dispatch_ring_n<T, MODE, false>(count, a);
+    if constexpr (std::is_same_v<T, float>) return dispatch_ring_n<T, MODE, true>(count, a);
+    return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm is FP32 only", a.what);
 }
 
 }  // namespace

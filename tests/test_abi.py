@@ -42,5 +42,15 @@ def test_header_parses_to_ctypes():
 
 
 def test_library_loads_without_gpu():
-    L = abi.lib()
-    assert L.last_error() == ""
+    """Loading the library in a fresh process needs no GPU and records no
+    error (a subprocess, so that calls made by earlier tests in this one do
+    not count)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, sys.argv[1]); from tealeaf_amd import abi; "
+            "print(repr(abi.lib().last_error()))")
+    out = subprocess.run([sys.executable, "-c", code, str(abi.ROOT)], capture_output=True,
+                         text=True, timeout=120, env={**__import__("os").environ,
+                                                      "HIP_VISIBLE_DEVICES": ""})
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "''", out.stdout

@@ -11,13 +11,17 @@ shorter than the step count, grids narrower than one window, odd halo
 depths (lanes that straddle the interior edge) and both loop schedules.
 
 The FP32 variant (the mixed-precision solve's inner sweeps) is held to the
-same per-step arithmetic done in numpy float32: IEEE single precision, no
-fused multiply-add, same association -- so it too must match bit for bit.
+same per-step arithmetic done in numpy float32 with the kernel's fused
+multiply-adds emulated exactly (tests/fp32_emulation.py) -- so it too must
+match bit for bit; its fused 2-norm variant to the FP64 sum of squares.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 from oracle.pyoracle import ptr
+from tests.fp32_emulation import fma32
 from tealeaf_amd import deck as tdeck
 
 pytestmark = pytest.mark.gpu
@@ -164,13 +168,22 @@ def test_set_ppcg_fusion_range(hip):
     assert hip.dll.tlhip_set_ppcg_fusion(10) == 0
 
 
-def _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
-    """The reference's per-step expressions in numpy float32 on (rows, pitch) arrays."""
+def _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky, ur=None):
+    """The reference's per-step expressions on (rows, pitch) float32 arrays,
+    with the FP32 kernel's fused multiply-adds: tx = kxr*rgt + kxc*lft and
+    ty = kyu*up + kyc*dn each as one fma after a product, A sd = fma(d, cen,
+    -tx) - ty, sd' = fma(alpha, cen, beta*r'), and sd = r * (1/theta).
+    ur = (alpha, p, w): cg_calc_ur first, r = fma(-alpha, w, r) and
+    u = fma(alpha, p, u)."""
     f = np.float32
     sd, r, u = sd.copy(), r.copy(), u.copy()
     J, K = slice(hd, y - hd), slice(hd, x - hd)
+    if ur is not None:
+        al, p, w = ur
+        r[J, K] = fma32(-f(al), w[J, K], r[J, K])
+        u[J, K] = fma32(f(al), p[J, K], u[J, K])
     if from_r:
-        sd[J, K] = r[J, K] / f(theta)
+        sd[J, K] = r[J, K] * (f(1) / f(theta))
     for a, b in zip(alphas, betas):
         a, b = f(a), f(b)
         s = sd.copy()                                     # depth-1 reflective halo
@@ -184,10 +197,12 @@ def _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r, u, kx, ky):
         kxc, kxr = kx[J, K], kx[J, hd + 1:x - hd + 1]
         kyc, kyu = ky[J, K], ky[hd + 1:y - hd + 1, K]
         d = (f(1) + (kxr + kxc)) + (kyu + kyc)
-        av = d * cen - (kxr * rgt + kxc * lft) - (kyu * up + kyc * dn)
+        tx = fma32(kxr, rgt, kxc * lft)
+        ty = fma32(kyu, up, kyc * dn)
+        av = fma32(d, cen, -tx) - ty
         r[J, K] = r[J, K] - av
         u[J, K] = u[J, K] + cen
-        sd[J, K] = a * cen + b * r[J, K]
+        sd[J, K] = fma32(a, cen, b * r[J, K])
     return sd, r, u
 
 
@@ -197,9 +212,9 @@ F32_GRIDS = [(32, 30, 2, 0), (250, 70, 2, 0), (510, 37, 2, 4), (6, 5, 2, 0), (61
 
 @pytest.mark.parametrize("grid", F32_GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in F32_GRIDS])
 @pytest.mark.parametrize("count", [1, 3, 5, 10])
-@pytest.mark.parametrize("from_r", [True, False], ids=["init", "cont"])
+@pytest.mark.parametrize("mode", [1, 0, 2], ids=["init", "cont", "init_ur"])
 @pytest.mark.parametrize("pad", [0, 4])
-def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
+def test_fused_f32_matches_float32_steps(hip, grid, count, mode, pad):
     from tealeaf_amd.device import DeviceArray
     nx, ny, hd, ty = grid
     x, y = nx + 2 * hd, ny + 2 * hd
@@ -212,10 +227,15 @@ def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
     alphas = rng.uniform(0.1, 0.9, count)
     betas = rng.uniform(0.1, 0.9, count)
     theta = 1.7
+    from_r = mode >= 1
+    p, w = f32(-1, 1), f32(-1, 1)
+    al_ur = 0.37 if mode == 2 else 0.0
     want = dict(zip(("sd", "r", "u"), _f32_steps(x, y, hd, alphas, betas, theta, from_r, sd, r,
-                                                  u, kx, ky)))
+                                                  u, kx, ky,
+                                                  ur=(al_ur, p, w) if mode == 2 else None)))
     src = dict(sd=sd, r=r, u=u)
-    d = {k: DeviceArray.from_host(v) for k, v in dict(sd=sd, r=r, u=u, kx=kx, ky=ky).items()}
+    d = {k: DeviceArray.from_host(v) for k, v in dict(sd=sd, r=r, u=u, kx=kx, ky=ky, p=p,
+                                                      w=w).items()}
     out = {k: DeviceArray.from_host(np.full(shape, np.nan, dtype=np.float32)) for k in ("sd", "r", "u")}
     nb = np.full(4, -1, dtype=np.int32)
     hip.tlhip_set_fused_tuning(ty if ty else -1, -1)
@@ -223,19 +243,35 @@ def test_fused_f32_matches_float32_steps(hip, grid, count, from_r, pad):
         hip.tlhip2d_ppcg_inner_iterations_f32(
             x, y, hd, pitch, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
             nb.ctypes.data, None if from_r else d["sd"].ptr, out["sd"].ptr, d["r"].ptr,
-            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr, 0.0, None, None)
+            out["r"].ptr, d["u"].ptr, out["u"].ptr, d["kx"].ptr, d["ky"].ptr, al_ur,
+            d["p"].ptr if mode == 2 else None, d["w"].ptr if mode == 2 else None)
     finally:
         hip.tlhip_set_fused_tuning(-1, -1)
     hip.tlhip_synchronize()
     J, K = slice(hd, y - hd), slice(hd, x - hd)
     ring = np.ones(shape, dtype=bool)
     ring[J, K] = False
+    got = {}
     for name in ("r", "u", "sd"):
-        g = out[name].download().reshape(shape)
+        g = got[name] = out[name].download().reshape(shape)
         np.testing.assert_array_equal(g[J, K], want[name][J, K], err_msg=f"{name} interior")
         if name != "sd" or not from_r:
             np.testing.assert_array_equal(g[ring], src[name][ring], err_msg=f"{name} halo ring")
-    for v in list(d.values()) + list(out.values()):
+    # the same launch forming r.r: identical fields, and the FP64 sum of the
+    # interior's squares
+    out2 = {k: DeviceArray.from_host(np.full(shape, np.nan, dtype=np.float32)) for k in ("sd", "r", "u")}
+    rrn = ctypes.c_double(-1.0)
+    hip.tlhip2d_ppcg_inner_iterations_f32_norm(
+        x, y, hd, pitch, count, alphas.ctypes.data, betas.ctypes.data, theta, int(from_r),
+        nb.ctypes.data, None if from_r else d["sd"].ptr, out2["sd"].ptr, d["r"].ptr,
+        out2["r"].ptr, d["u"].ptr, out2["u"].ptr, d["kx"].ptr, d["ky"].ptr, al_ur,
+        d["p"].ptr if mode == 2 else None, d["w"].ptr if mode == 2 else None,
+        ctypes.addressof(rrn))
+    for name in ("r", "u", "sd"):
+        np.testing.assert_array_equal(out2[name].download().reshape(shape)[J, K], got[name][J, K])
+    want_rr = float(np.sum(got["r"][J, K].astype(np.float64) ** 2))
+    assert abs(rrn.value - want_rr) <= 1e-13 * want_rr, (rrn.value, want_rr)
+    for v in list(d.values()) + list(out.values()) + list(out2.values()):
         v.free()
 
 
