@@ -70,6 +70,9 @@ def parse_args():
     ap.add_argument("--refine-cells", type=int, default=8192,
                     help="grid of the FP32-refined vs FP64 time-to-1e-10 comparison (0 = skip)")
     ap.add_argument("--refine-steps", type=int, default=5)
+    ap.add_argument("--refine-cheby-steps", type=int, default=2,
+                    help="steps of the FP32-refined vs FP64 Chebyshev comparison on the "
+                         "--cells deck (0 = skip)")
     ap.add_argument("--refine-only", action="store_true",
                     help="run only the FP32-refined vs FP64 comparison and print its JSON")
     ap.add_argument("--blocks3d", action="store_true",
@@ -232,10 +235,60 @@ def refine_run(args):
         res["rel_error_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         res["time_ratio"] = res["fp32_refined"]["mean_s"] / res["fp64"]["mean_s"]
         out[f"contrast_{contrast:g}"] = res
+    out["cheby"] = refine_cheby(args) if args.refine_cheby_steps > 0 else None
     return {"workload": f"2D {n}x{n}, 64x64-cell blocks, log-uniform density [1, C], energy "
                         f"U(0.1,25), dt 0.004, PPCG 10 inner via the error switch, "
                         f"{args.refine_steps} steps each solved to relative residual 1e-10",
             "target_ratio": 0.6, **out}
+
+
+def refine_cheby(args):
+    """FP32-refined vs FP64 Chebyshev on the benchmark's own deck (2D
+    4096^2 layered, presteps 30), each timestep solved to the deck's eps
+    (1e-15 on the squared residual norm; the FP32-refined solve tests it on
+    FP64 true residuals)."""
+    import numpy as np
+    import torch
+    from tealeaf_amd import deck as tdeck
+    from tealeaf_amd.abi import lib
+    from tealeaf_amd.problem import Problem
+
+    L = lib()
+    n, steps = args.cells, args.refine_cheby_steps
+    text = tdeck.layered_2d(n, steps, solver=tdeck.CHEBY, eps=1e-15, max_iters=10000,
+                            presteps=30)
+    s, states = tdeck.parse_deck(text, 2)
+    res, u_final = {}, {}
+    for prec in (0, 1):
+        prob = Problem(s, states)
+        prob.set_solve(prec, 0.0)
+        times, reps = [], []
+        for tt in range(steps):
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps.append(prob.step(tt))
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] refine cheby {'fp32' if prec else 'fp64'} step {tt}: {times[-1]:.3f} s, "
+                  f"{reps[-1]['printed_cg_iters']} CG + {reps[-1]['printed_inner_iters']} Chebyshev",
+                  file=sys.stderr, flush=True)
+        u_final[prec] = prob.get("u")
+        prob.close()
+        res["fp32_refined" if prec else "fp64"] = {
+            "seconds_per_step": times, "mean_s": sum(times) / len(times),
+            "printed_cg": [r["printed_cg_iters"] for r in reps],
+            "printed_cheby": [r["printed_inner_iters"] for r in reps],
+            "true_residual_evals": [r["n_true_residuals"] for r in reps]}
+    hd = s.halo_depth
+    a = u_final[1].reshape(s.shape())[hd:-hd, hd:-hd]
+    b = u_final[0].reshape(s.shape())[hd:-hd, hd:-hd]
+    res["rel_error_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    res["time_ratio"] = res["fp32_refined"]["mean_s"] / res["fp64"]["mean_s"]
+    res["workload"] = (f"2D {n}x{n} layered deck, Chebyshev (presteps 30), eps 1e-15, "
+                       f"{steps} steps")
+    return res
 
 
 def profile_run(args):

@@ -246,6 +246,19 @@ int tlhip2d_cheby_iterations(int x, int y, int halo_depth, int count,
         const double* alphas, const double* betas, const int* neighbours,
         const double* u_in, double* u_out, const double* u0, const double* p_in,
         double* p_out, double* r, const double* kx, const double* ky);
+/* The same loop on FP32 fields (the Chebyshev sweeps of the mixed-precision
+ * solve, where u is the correction and u0 the residual it corrects): rows
+ * of `pitch` floats (a multiple of 4, >= x).  Coefficients are rounded to
+ * FP32; A u as in tlhip2d_ppcg_inner_iterations_f32 (fused multiply-adds),
+ * p = fma(alpha, p, beta*r).  rrn: null, or (host) the squared 2-norm,
+ * accumulated in FP64, of the interior of the r written, formed in the same
+ * launch (no reference counterpart: the driver's calculate_2norm,
+ * 2d/drivers/cheby_driver.c:128-139). */
+int tlhip2d_cheby_iterations_f32(int x, int y, int halo_depth, int pitch,
+        int count, const double* alphas, const double* betas,
+        const int* neighbours, const float* u_in, float* u_out, const float* u0,
+        const float* p_in, float* p_out, float* r, const float* kx,
+        const float* ky, double* rrn);
 /* Tuning of the fused PPCG kernels: output rows per tile (<= 0: default,
  * the height whose block count fills ~0.88 of what the device holds at
  * once, from the kernel's occupancy and the CU count) and prefetch distance

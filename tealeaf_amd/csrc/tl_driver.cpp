@@ -1082,8 +1082,9 @@ struct tlhip_problem {
 
     int mixed_driver(double rx, double ry, double* error)
     {
-        if (s.solver != 1 && s.solver != 3)
-            return tl_fail(TLHIP_ERR_ARG, "mixed precision: CG or PPCG only (solver %d)", s.solver);
+        if (s.solver != 1 && s.solver != 2 && s.solver != 3)
+            return tl_fail(TLHIP_ERR_ARG, "mixed precision: CG, Chebyshev or PPCG only (solver %d)",
+                           s.solver);
         TRY(mixed_alloc());
         double rro;
         TRY(cg_init_driver(rx, ry, 0.0, &rro));   // FP64 kx, ky, u, r = u0 - A u, p = r
@@ -1096,9 +1097,42 @@ struct tlhip_problem {
         const double target = rel_tol > 0.0 ? rel_tol * rel_tol * bb : s.eps;
         double rr_true = rro, last_true = rro;
         *error = rro;
-        int nppcg = 0, tt;
+        int nppcg = 0, est = 0, tt;      // nppcg: PPCG outer or Chebyshev iterations
         for (tt = 0; tt < s.max_iters; ++tt) {
             double rrn = 0.0;
+            if (s.solver == 2 && switch_now(nppcg, tt, *error)) {
+                // Chebyshev on the correction equation A d = b, b = r32 (the
+                // last true residual), r of each turn in r32b
+                int cnt = 1;
+                if (++nppcg == 1) {
+                    TRY(flush_p32());
+                    TRY(eigenvalues(tt));
+                    cheby_coefs(s.max_iters - tt);
+                    TRY(replace_residual(&rr_true));          // d = 0, b = u0 - A u
+                    last_true = rr_true;
+                    TRY(tl::mix::cheby_init32(x, y, hd, P, theta, p32, r32, d32));
+                } else {
+                    cnt = std::min({g_cheby_fusion > 1 ? g_cheby_fusion : 1, s.max_iters - tt,
+                                    (int)cheby_alphas.size() - nppcg});
+                }
+                TRY(tl::mix::cheby_iterations32(x, y, hd, P, cnt, &cheby_alphas[nppcg],
+                                                &cheby_betas[nppcg], neighbours, d32, d32b, r32,
+                                                p32, p32b, r32b, kx32, ky32, &rrn));
+                std::swap(d32, d32b);
+                std::swap(p32, p32b);
+                rep->n_cheby_steps += cnt;
+                if (nppcg == 1) est = est_iterations(rrn, bb);
+                nppcg += cnt - 1;
+                tt += cnt - 1;
+                *error = rrn;
+                if (rrn <= target || rrn < kDrop * kDrop * last_true) {
+                    TRY(replace_residual(&rr_true));
+                    last_true = rr_true;
+                    *error = rr_true;
+                    if (rr_true <= target) break;
+                }
+                continue;
+            }
             if (s.solver == 3 && switch_now(nppcg, tt, *error)) {
                 if (++nppcg == 1) {
                     TRY(eigenvalues(tt));
@@ -1125,8 +1159,9 @@ struct tlhip_problem {
         if (s.solver == 1) {
             rep->printed_cg_iters = tt;                  // as cg_driver.c prints
         } else {
-            rep->printed_cg_iters = tt - nppcg + 1;      // as ppcg_driver.c prints
+            rep->printed_cg_iters = tt - nppcg + 1;      // as ppcg_driver.c / cheby_driver.c print
             rep->printed_inner_iters = nppcg;
+            if (s.solver == 2) rep->est_iters = est;
         }
         return 0;
     }
@@ -1420,8 +1455,9 @@ extern "C" int tlhip_problem_set_solve(tlhip_problem* P, int precision, double r
     if (precision == 1) {
         if (P->dims != 2)
             return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: mixed precision is 2D only");
-        if (P->s.solver != 1 && P->s.solver != 3)
-            return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: mixed precision needs CG or PPCG");
+        if (P->s.solver != 1 && P->s.solver != 2 && P->s.solver != 3)
+            return tl_fail(TLHIP_ERR_ARG,
+                           "tlhip_problem_set_solve: mixed precision needs CG, Chebyshev or PPCG");
         TRY(P->mixed_alloc());
     }
     P->precision = precision;

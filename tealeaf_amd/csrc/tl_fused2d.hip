@@ -363,13 +363,17 @@ k_ppcg_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_ar
 // level it keeps three rows of u and the row of p the next level needs;
 // u0, kx and ky rows are shared by all levels.  r and p of the last level
 // and u are stored; w is not (nothing reads it before cg_init or cg_calc_w
-// rewrite it).  Bit-exact with S single iterations.
-template <class T, int S, int PF>
+// rewrite it).  Bit-exact with S single iterations.  FP32 (the sweeps of
+// the FP32-refined solve) forms p with one fused multiply-add, as A u (mad).
+// NORM: also the squared 2-norm of the r written, one FP64 partial per wave
+// (slot blockIdx.x * kWaves + wave of rd.partials).
+template <class T, int S, int PF, bool NORM>
 __global__ void __launch_bounds__(kBlock)
 k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
                 const T* __restrict__ u_in, T* __restrict__ u_out,
                 const T* __restrict__ u0f, const T* __restrict__ p_in, T* __restrict__ p_out,
-                T* __restrict__ r_out, const T* __restrict__ kx, const T* __restrict__ ky)
+                T* __restrict__ r_out, const T* __restrict__ kx, const T* __restrict__ ky,
+                TlReduce rd)
 {
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
@@ -377,7 +381,14 @@ k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
     const int tile = xcd_tile_of(blockIdx.x, t.nbx * t.nby);
     const int bx = tile % t.nbx, by = tile / t.nbx;
     const int sx = bx * kWaves + wave;
-    if (sx >= t.ns) return;                      // uniform per wave; no block sync below
+    double* const part = NORM ? rd.partials + (size_t)blockIdx.x * kWaves + wave : nullptr;
+    if (sx >= t.ns) {                            // uniform per wave; no block sync below
+        if constexpr (NORM) {
+            if (lane == 0) *part = 0.0;
+        }
+        return;
+    }
+    double racc = 0.0;
 
     const int xin0 = g.hd, xin1 = g.hd + g.nx;
     const int yin0 = g.hd, yin1 = g.hd + g.ny;
@@ -540,7 +551,7 @@ k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 rn.v[i] = u0w[k - 1].v[i] - wv.v[i];
-                pn.v[i] = cf.alpha[k - 1] * pin.v[i] + cf.beta[k - 1] * rn.v[i];
+                pn.v[i] = mad(cf.alpha[k - 1], pin.v[i], cf.beta[k - 1] * rn.v[i]);
                 un.v[i] = cen.v[i] + pn.v[i];
             }
             pcur[k - 1] = pn;
@@ -550,6 +561,12 @@ k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
                 uw[k][2] = un;
             } else if (any_out && rho >= j0 && rho < j1) {
                 const long i0 = (long)rho * X + c;
+                if constexpr (NORM) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i)
+                        if (all_out || (c + i >= o0 && c + i < o1))
+                            racc += (double)rn.v[i] * (double)rn.v[i];
+                }
                 if (all_out) {
                     stc(u_out, i0, un);
                     stc(p_out, i0, pn);
@@ -581,6 +598,10 @@ k_cheby_fused2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf,
             if (yedge) body(it, No{}, Yes{});
             else body(it, No{}, No{});
         }
+    }
+    if constexpr (NORM) {
+        const double tot = tl_wave_sum(racc);
+        if (lane == 0) *part = tot;
     }
 }
 
@@ -933,17 +954,17 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
     return dispatch_fused<T, 2>(count, a);
 }
 
-template <int S>
-int launch_cheby(const PitchGrid& g, const double* al, const double* be, const double* u_in,
-                 double* u_out, const double* u0, const double* p_in, double* p_out, double* r,
-                 const double* kx, const double* ky, const char* what)
+template <class T, int S, bool NORM>
+int launch_cheby(const PitchGrid& g, const double* al, const double* be, const T* u_in,
+                 T* u_out, const T* u0, const T* p_in, T* p_out, T* r, const T* kx, const T* ky,
+                 double* rrn, const char* what)
 {
     const int pf = g_fused_pf < 0 ? 1 : std::min(g_fused_pf, 1);
-    StripTile t = strip_tile<double>(g, S, 1, kWaves);
+    StripTile t = strip_tile<T>(g, S, 1, kWaves);
     static int occ = 0;
     if (occ <= 0 &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cheby_fused2d<double, S, 1>, kBlock,
-                                                      0) != hipSuccess ||
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cheby_fused2d<T, S, 1, NORM>,
+                                                      kBlock, 0) != hipSuccess ||
          occ <= 0))
         occ = 1;
     int ty = g_fused_ty;
@@ -952,40 +973,56 @@ int launch_cheby(const PitchGrid& g, const double* al, const double* be, const d
         const long nby = std::max(1L, cap / std::max(1, t.nbx));
         ty = (int)std::max(1L, ((long)g.ny + nby - 1) / nby);
     }
-    t = strip_tile<double>(g, S, ty, kWaves);
-    if ((long)t.nbx * t.nby > (1L << 30))
+    t = strip_tile<T>(g, S, ty, kWaves);
+    if ((long)t.nbx * t.nby > (NORM ? (long)TL_MAX_RED_BLOCKS / kWaves : (1L << 30)))
         return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
-    if ((long)(t.ty + 3 * S + pf + 2) * g.X * (long)sizeof(double) > (long)kBufferRange)
+    if ((long)(t.ty + 3 * S + pf + 2) * g.X * (long)sizeof(T) > (long)kBufferRange)
         return tl_fail(TLHIP_ERR_ARG, "%s: %d-row tiles of %d-element rows exceed a buffer range",
                        what, t.ty, g.X);
-    PpcgCoefs<double, S> cf;
+    PpcgCoefs<T, S> cf;
     for (int k = 0; k < S; ++k) {
-        cf.alpha[k] = al[k];
-        cf.beta[k] = be[k];
+        cf.alpha[k] = (T)al[k];
+        cf.beta[k] = (T)be[k];
     }
     const unsigned nb = (unsigned)(t.nbx * t.nby);
     const int slot = tl_timer_begin(what);
     if (pf)
-        hipLaunchKernelGGL((k_cheby_fused2d<double, S, 1>), dim3(nb), dim3(kBlock), 0, tl_stream(),
-                           g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky);
+        hipLaunchKernelGGL((k_cheby_fused2d<T, S, 1, NORM>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky,
+                           tl_reduce_slots());
     else
-        hipLaunchKernelGGL((k_cheby_fused2d<double, S, 0>), dim3(nb), dim3(kBlock), 0, tl_stream(),
-                           g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky);
+        hipLaunchKernelGGL((k_cheby_fused2d<T, S, 0, NORM>), dim3(nb), dim3(kBlock), 0,
+                           tl_stream(), g, t, cf, u_in, u_out, u0, p_in, p_out, r, kx, ky,
+                           tl_reduce_slots());
     tl_timer_end(slot);
-    return tl_check_launch(what);
+    TL_TRY(tl_check_launch(what));
+    if constexpr (NORM) {
+        TL_TRY(tl_finish_reduce(1, nb * kWaves, what));
+        return tl_fetch(1, rrn, what);
+    }
+    return 0;
 }
 
-int cheby_iterations(int x, int y, int hd, int count, const double* alphas, const double* betas,
-                     const int* neighbours, const double* u_in, double* u_out, const double* u0,
-                     const double* p_in, double* p_out, double* r, const double* kx,
-                     const double* ky)
+// The fused Chebyshev loop: FP64 on x-pitched fields, or FP32 on fields of
+// row pitch `pitch`; rrn (FP32 only) also forms r.r; copy_ring: carry the
+// halo rings of u and p over to the outputs.
+template <class T>
+int cheby_iterations(const char* what, int x, int y, int hd, int pitch, int count,
+                     const double* alphas, const double* betas, const int* neighbours,
+                     const T* u_in, T* u_out, const T* u0, const T* p_in, T* p_out, T* r,
+                     const T* kx, const T* ky, double* rrn = nullptr, bool copy_ring = true)
 {
-    const char* what = "tlhip2d_cheby_iterations";
+    constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
-    if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || x % 2 || (long)x * y > (1L << 31) - 1)
-        return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x (x=%d y=%d hd=%d)", what, x, y, hd);
+    if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || pitch < x || pitch % N ||
+        (long)pitch * y > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG,
+                       "%s: needs a row pitch that is a multiple of %d and >= x (x=%d pitch=%d "
+                       "y=%d hd=%d)", what, N, x, pitch, y, hd);
     if (count < 1 || count > kMaxFused)
         return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d (got %d)", what, kMaxFused, count);
+    if (rrn && !std::is_same_v<T, float>)
+        return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm is FP32 only", what);
     TL_REQUIRE(alphas && betas && neighbours, what, "null coefficients/neighbours");
     TL_REQUIRE(u_in && u_out && u0 && p_in && p_out && r && kx && ky, what, "null field");
     TL_REQUIRE(u_in != u_out && p_in != p_out, what, "input and output buffers must be distinct");
@@ -995,25 +1032,35 @@ int cheby_iterations(int x, int y, int hd, int count, const double* alphas, cons
         TL_REQUIRE(neighbours[f] == -1, what, "every face must be external (single chunk)");
     TL_REQUIRE(all_aligned(u_in, u_out, u0, p_in, p_out, r, kx, ky), what,
                "fields must be 16-byte aligned");
-    const PitchGrid g{x, hd, x - 2 * hd, y - 2 * hd};
-    Ring3<double> rg{};
-    rg.src[0] = u_in;  rg.dst[0] = u_out;
-    rg.src[1] = p_in;  rg.dst[1] = p_out;
-    rg.n = 2;
-    const long ring = ((long)(y - g.ny) * x + (long)(x - g.nx) * g.ny) * rg.n;
-    hipLaunchKernelGGL(k_copy_rings2d<double>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
-                       tl_stream(), g, y, rg, tl_gate());
-    TL_TRY(tl_check_launch(what));
+    const PitchGrid g{pitch, hd, x - 2 * hd, y - 2 * hd};
+    if (copy_ring) {
+        Ring3<T> rg{};
+        rg.src[0] = u_in;  rg.dst[0] = u_out;
+        rg.src[1] = p_in;  rg.dst[1] = p_out;
+        rg.n = 2;
+        const long ring = ((long)(y - g.ny) * pitch + (long)(pitch - g.nx) * g.ny) * rg.n;
+        hipLaunchKernelGGL(k_copy_rings2d<T>, dim3(flat_blocks(ring)), dim3(kBlock), 0,
+                           tl_stream(), g, y, rg, tl_gate());
+        TL_TRY(tl_check_launch(what));
+    }
+    const bool norm = rrn != nullptr;
     switch (count) {
 #define TL_CHEBY_CASE(S)                                                                      \
     case S:                                                                                   \
-        return launch_cheby<S>(g, alphas, betas, u_in, u_out, u0, p_in, p_out, r, kx, ky, what);
+        if constexpr (std::is_same_v<T, float>) {                                             \
+            if (norm)                                                                         \
+                return launch_cheby<T, S, true>(g, alphas, betas, u_in, u_out, u0, p_in,      \
+                                                p_out, r, kx, ky, rrn, what);                 \
+        }                                                                                     \
+        return launch_cheby<T, S, false>(g, alphas, betas, u_in, u_out, u0, p_in, p_out, r,   \
+                                         kx, ky, nullptr, what);
     TL_CHEBY_CASE(1) TL_CHEBY_CASE(2) TL_CHEBY_CASE(3) TL_CHEBY_CASE(4) TL_CHEBY_CASE(5)
     TL_CHEBY_CASE(6) TL_CHEBY_CASE(7) TL_CHEBY_CASE(8) TL_CHEBY_CASE(9) TL_CHEBY_CASE(10)
 #undef TL_CHEBY_CASE
     default: return tl_fail(TLHIP_ERR_ARG, "%s: count must be 1..%d", what, kMaxFused);
     }
 }
+
 
 // Checks, the ring copy p_in -> p_out (the halo the reference's p keeps;
 // skipped with ring = false) and the k_calc_pw launch under the current
@@ -1269,6 +1316,36 @@ extern "C" int tlhip2d_cheby_iterations(int x, int y, int hd, int count, const d
                                         const double* p_in, double* p_out, double* r,
                                         const double* kx, const double* ky)
 {
-    return cheby_iterations(x, y, hd, count, alphas, betas, neighbours, u_in, u_out, u0, p_in,
-                            p_out, r, kx, ky);
+    if (x % 2)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip2d_cheby_iterations: needs an even x (got %d)", x);
+    return cheby_iterations<double>("tlhip2d_cheby_iterations", x, y, hd, x, count, alphas, betas,
+                                    neighbours, u_in, u_out, u0, p_in, p_out, r, kx, ky);
 }
+
+extern "C" int tlhip2d_cheby_iterations_f32(int x, int y, int hd, int pitch, int count,
+                                            const double* alphas, const double* betas,
+                                            const int* neighbours, const float* u_in,
+                                            float* u_out, const float* u0, const float* p_in,
+                                            float* p_out, float* r, const float* kx,
+                                            const float* ky, double* rrn)
+{
+    return cheby_iterations<float>("tlhip2d_cheby_iterations_f32", x, y, hd, pitch, count, alphas,
+                                   betas, neighbours, u_in, u_out, u0, p_in, p_out, r, kx, ky,
+                                   rrn);
+}
+
+namespace tl {
+namespace mix {
+
+int cheby_iterations32(int x, int y, int hd, int P, int count, const double* alphas,
+                       const double* betas, const int* neighbours, const float* d_in,
+                       float* d_out, const float* b, const float* p_in, float* p_out, float* r,
+                       const float* kx, const float* ky, double* rrn)
+{
+    return cheby_iterations<float>("tlhip2d_cheby_iterations_f32", x, y, hd, P, count, alphas,
+                                   betas, neighbours, d_in, d_out, b, p_in, p_out, r, kx, ky, rrn,
+                                   false);
+}
+
+}  // namespace mix
+}  // namespace tl

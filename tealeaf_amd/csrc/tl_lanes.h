@@ -152,8 +152,16 @@ __device__ __forceinline__ Cells<T> diag_lane(const Cells<T>& kxc, T kxe, const 
     return d;
 }
 
+// a * b + c.  FP64: two roundings, the reference's arithmetic.  FP32 (the
+// sweeps of the FP32-refined solve, which FP64 residual replacement
+// corrects): one fused multiply-add, fewer instructions in the issue-bound
+// fused loops.
+__device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // apply_a_lane (don't-care wave ends) with the diagonal and next-lane kx
-// formed beforehand.  XMIRROR: the lane's cells may include the first or
+// formed beforehand.  FP32 uses fused multiply-adds (mad): the x and y
+// terms as mad(a, b, c*d), then mad(diag, sd, -x terms) - y terms.  XMIRROR: the lane's cells may include the first or
 // last interior column (a wave whose window holds neither skips the tests).
 template <bool XMIRROR, class T>
 __device__ __forceinline__ Cells<T> apply_a_pre(const Cells<T>& cen, const Cells<T>& up,
@@ -174,7 +182,8 @@ __device__ __forceinline__ Cells<T> apply_a_pre(const Cells<T>& cen, const Cells
             if (c + i == xin1 - 1) rgt = cen.v[i];
         }
         const T kxr = i == N - 1 ? kxe : kxc.v[i + 1];
-        a.v[i] = d.v[i] * cen.v[i] - (kxr * rgt + kxc.v[i] * lft) - (kyu.v[i] * up.v[i] + kyc.v[i] * dn.v[i]);
+        const T tx = mad(kxr, rgt, kxc.v[i] * lft), ty = mad(kyu.v[i], up.v[i], kyc.v[i] * dn.v[i]);
+        a.v[i] = mad(d.v[i], cen.v[i], -tx) - ty;
     }
     return a;
 }

@@ -42,6 +42,16 @@ int ppcg_inner32_norm(int x, int y, int hd, int pitch, int count, const double* 
                       const float* u_in, float* u_out, const float* kx, const float* ky,
                       double alpha, const float* p, const float* w, const double* alpha_dev,
                       double* rrn);
+// Chebyshev iterations on the correction equation A d = b (FP32): count
+// turns of r = b - A d (d mirrored at the interior's edges), p = alpha_k p +
+// beta_k r, d += p in one fused launch; when rrn is not null, *rrn = sum r*r
+// (FP64) of the last turn's r.  Outputs' halos are not written.
+int cheby_iterations32(int x, int y, int hd, int P, int count, const double* alphas,
+                       const double* betas, const int* neighbours, const float* d_in,
+                       float* d_out, const float* b, const float* p_in, float* p_out, float* r,
+                       const float* kx, const float* ky, double* rrn);
+// Chebyshev start on the correction equation (FP32): p = r / theta, d += p
+int cheby_init32(int x, int y, int hd, int P, double theta, float* p, const float* r, float* d);
 // r32 = (float)(u0 - A u) computed in FP64; *rr = sum (u0 - A u)^2 (FP64)
 int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
                const double* ky, float* r32, double* rr);

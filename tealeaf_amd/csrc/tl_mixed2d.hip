@@ -138,6 +138,39 @@ struct OpNorm32 {
     }
 };
 
+// p = r / theta; d += p   (cheby.c:27-48 and cheby_calc_u, cheby.c:8-24, on
+// the correction equation, whose residual r the caller has set)
+struct OpChebyInit32 {
+    static constexpr int NRED = 0;
+    int P;
+    float theta;
+    float* p;
+    const float* r;
+    float* d;
+    __device__ void apply(int row, int c, unsigned mask, double*) const
+    {
+        const long i0 = (long)row * P + c;
+        const C4 rv = ldc(r, i0);
+        C4 pv, dv = ldc(d, i0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            pv.v[i] = rv.v[i] / theta;
+            dv.v[i] = dv.v[i] + pv.v[i];
+        }
+        if (mask == 15u) {
+            stc(p, i0, pv);
+            stc(d, i0, dv);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (mask >> i & 1) {
+                    p[i0 + i] = pv.v[i];
+                    d[i0 + i] = dv.v[i];
+                }
+        }
+    }
+};
+
 // u += d (FP64 update of the solution); d = 0
 struct OpPromote {
     static constexpr int NRED = 0;
@@ -173,11 +206,52 @@ struct OpResidual64 {
     __device__ void apply(int row, int c, unsigned mask, double* acc) const
     {
         const int xin0 = hd, xin1 = x - hd, yin0 = hd, yin1 = y - hd;
+        const long j0 = (long)row * x + c;
+        if (mask == 15u && (x & 1) == 0) {
+            // the group's four cells from 16-byte loads (j0 is even)
+            using D2 = Cells<double>;
+            auto ld4 = [&](const double* f, long j, double* v) {
+                const D2 a = ldc(f, j), b = ldc(f, j + 2);
+                v[0] = a.v[0]; v[1] = a.v[1]; v[2] = b.v[0]; v[3] = b.v[1];
+            };
+            double cu[6], dn[4], up[4], kxv[5], kyc[4], kyu[4], b0[4];
+            ld4(u, j0, cu + 1);
+            cu[0] = u[j0 - 1];
+            cu[5] = u[j0 + 4];
+            if (row != yin0) ld4(u, j0 - x, dn);
+            if (row != yin1 - 1) ld4(u, j0 + x, up);
+            ld4(kx, j0, kxv);
+            kxv[4] = kx[j0 + 4];
+            ld4(ky, j0, kyc);
+            ld4(ky, j0 + x, kyu);
+            ld4(u0, j0, b0);
+            float out[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = c + i;
+                const double cen = cu[i + 1];
+                const double lft = k == xin0 ? cen : cu[i];
+                const double rgt = k == xin1 - 1 ? cen : cu[i + 2];
+                const double dv = row == yin0 ? cen : dn[i];
+                const double uv = row == yin1 - 1 ? cen : up[i];
+                const double d = (1.0 + (kxv[i + 1] + kxv[i])) + (kyu[i] + kyc[i]);
+                const double a = d * cen - (kxv[i + 1] * rgt + kxv[i] * lft) -
+                                 (kyu[i] * uv + kyc[i] * dv);
+                const double rv = b0[i] - a;
+                out[i] = (float)rv;
+                acc[0] += rv * rv;
+            }
+            C4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o.v[i] = out[i];
+            stc(r32, (long)row * P + c, o);
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (!(mask >> i & 1)) continue;
             const int k = c + i;
-            const long j = (long)row * x + k;
+            const long j = j0 + i;
             const double cen = u[j];
             const double lft = k == xin0 ? cen : u[j - 1];
             const double rgt = k == xin1 - 1 ? cen : u[j + 1];
@@ -269,6 +343,12 @@ int norm32(int x, int y, int hd, int P, const float* a, double* out)
 {
     const OpNorm32 op{P, a};
     return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_norm32", out);
+}
+
+int cheby_init32(int x, int y, int hd, int P, double theta, float* p, const float* r, float* d)
+{
+    const OpChebyInit32 op{P, (float)theta, p, r, d};
+    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_cheby_init32", nullptr);
 }
 
 int promote(int x, int y, int hd, int P, double* u, float* d)

@@ -109,13 +109,6 @@ __device__ __forceinline__ float shl1(float v)
     return __builtin_bit_cast(float, shl1_i(__builtin_bit_cast(int, v)));
 }
 
-// a * b + c.  FP64: two roundings, the reference's arithmetic.  FP32 (the
-// inner sweeps of the FP32-refined solve, which the FP64 residual
-// replacement corrects): one fused multiply-add, fewer instructions in an
-// issue-bound loop.
-__device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
-__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
 // A * a for a lane's N cells (2d/shared.h:50-54, same association) with
 // the diagonal and the next lane's kx formed beforehand, and x neighbours
 // outside the interior mirroring the cell (XM 0: the wave's window holds

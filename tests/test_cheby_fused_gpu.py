@@ -9,10 +9,13 @@ halo rings of u and p pass through unchanged.  The driver with the fused
 loop must give the same bits as with one launch per iteration, on every
 way the Chebyshev phase can end.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 from oracle.pyoracle import ptr
+from tests.fp32_emulation import fma32
 from tealeaf_amd import blocks
 from tealeaf_amd import deck as tdeck
 
@@ -87,6 +90,81 @@ def test_fused_cheby_matches_separate_steps(hip, oracle, grid, count, pf):
     np.testing.assert_array_equal(_ring(got_p, x, y, hd), _ring(p, x, y, hd))
     assert np.all(_ring(got_r, x, y, hd) == -7.0)          # r's halo is not written
     for v in list(d.values()) + list(out.values()) + [r_dev]:
+        v.free()
+
+
+def _f32_turns(x, y, hd, alphas, betas, u, u0, p, kx, ky):
+    """Chebyshev turns on (rows, pitch) float32 arrays, the FP32 kernel's
+    arithmetic: the reference's expressions with the products that feed an
+    addition fused (A u as in the FP32 PPCG loop, p = fma(alpha, p, beta*r))."""
+    f = np.float32
+    u, p = u.copy(), p.copy()
+    r = np.zeros_like(u)
+    J, K = slice(hd, y - hd), slice(hd, x - hd)
+    for a, b in zip(alphas, betas):
+        a, b = f(a), f(b)
+        s = u.copy()                                      # depth-1 reflective halo
+        s[J, hd - 1] = s[J, hd]
+        s[J, x - hd] = s[J, x - hd - 1]
+        s[hd - 1, K] = s[hd, K]
+        s[y - hd, K] = s[y - hd - 1, K]
+        cen = s[J, K]
+        lft, rgt = s[J, hd - 1:x - hd - 1], s[J, hd + 1:x - hd + 1]
+        dn, up = s[hd - 1:y - hd - 1, K], s[hd + 1:y - hd + 1, K]
+        kxc, kxr = kx[J, K], kx[J, hd + 1:x - hd + 1]
+        kyc, kyu = ky[J, K], ky[hd + 1:y - hd + 1, K]
+        d = (f(1) + (kxr + kxc)) + (kyu + kyc)
+        wv = fma32(d, cen, -fma32(kxr, rgt, kxc * lft)) - fma32(kyu, up, kyc * dn)
+        r[J, K] = u0[J, K] - wv
+        p[J, K] = fma32(a, p[J, K], b * r[J, K])
+        u[J, K] = u[J, K] + p[J, K]
+    return u, p, r
+
+
+F32_GRIDS = [(32, 30, 2, 0), (250, 70, 2, 0), (510, 37, 2, 4), (6, 5, 2, 0), (61, 9, 1, 0),
+             (1000, 21, 3, 8)]
+
+
+@pytest.mark.parametrize("grid", F32_GRIDS, ids=[f"{g[0]}x{g[1]}h{g[2]}t{g[3]}" for g in F32_GRIDS])
+@pytest.mark.parametrize("count", [1, 3, 10])
+@pytest.mark.parametrize("norm", [False, True], ids=["plain", "norm"])
+def test_fused_cheby_f32_matches_float32_turns(hip, grid, count, norm):
+    from tealeaf_amd.device import DeviceArray
+    nx, ny, hd, ty = grid
+    x, y = nx + 2 * hd, ny + 2 * hd
+    pitch = (x + 3) // 4 * 4
+    shape = (y, pitch)
+    rng = np.random.default_rng(9000 + nx + 10 * count)
+    f32 = lambda lo, hi: rng.uniform(lo, hi, shape).astype(np.float32)  # noqa: E731
+    u, u0, p = f32(-1, 1), f32(-1, 1), f32(-1, 1)
+    kx, ky = f32(0.05, 0.3), f32(0.05, 0.3)
+    alphas = rng.uniform(0.1, 0.9, count)
+    betas = rng.uniform(0.1, 0.9, count)
+    want_u, want_p, want_r = _f32_turns(x, y, hd, alphas, betas, u, u0, p, kx, ky)
+    d = {k: DeviceArray.from_host(v) for k, v in dict(u=u, u0=u0, p=p, kx=kx, ky=ky).items()}
+    out = {k: DeviceArray.from_host(np.full(shape, np.nan, dtype=np.float32))
+           for k in ("u", "p", "r")}
+    nb = np.full(4, -1, dtype=np.int32)
+    rrn = ctypes.c_double(-1.0)
+    hip.tlhip_set_fused_tuning(ty if ty else -1, -1)
+    try:
+        hip.tlhip2d_cheby_iterations_f32(x, y, hd, pitch, count, alphas.ctypes.data,
+                                         betas.ctypes.data, nb.ctypes.data, d["u"].ptr,
+                                         out["u"].ptr, d["u0"].ptr, d["p"].ptr, out["p"].ptr,
+                                         out["r"].ptr, d["kx"].ptr, d["ky"].ptr,
+                                         ctypes.addressof(rrn) if norm else None)
+    finally:
+        hip.tlhip_set_fused_tuning(-1, -1)
+    hip.tlhip_synchronize()
+    J, K = slice(hd, y - hd), slice(hd, x - hd)
+    got = {k: v.download().reshape(shape) for k, v in out.items()}
+    np.testing.assert_array_equal(got["u"][J, K], want_u[J, K])
+    np.testing.assert_array_equal(got["p"][J, K], want_p[J, K])
+    np.testing.assert_array_equal(got["r"][J, K], want_r[J, K])
+    if norm:
+        want_rr = float(np.sum(want_r[J, K].astype(np.float64) ** 2))
+        assert abs(rrn.value - want_rr) <= 1e-13 * want_rr, (rrn.value, want_rr)
+    for v in list(d.values()) + list(out.values()):
         v.free()
 
 

@@ -4,7 +4,8 @@
   oracle driver with the same stop -- printed iterations within +-1, field
   summary within 1e-12 -- and the true residual, recomputed here in numpy,
   at or below the target.
-* Mixed precision (FP32 CG / PPCG sweeps, FP64 residual replacement): the
+* Mixed precision (FP32 CG / Chebyshev / PPCG sweeps, FP64 residual
+  replacement): the
   true residual, recomputed here in numpy from the returned u, at or below
   1e-10 ||u0||; the solution against the oracle's FP64 solve to the same
   target; and the work against the FP64 GPU solve (iteration counts within
@@ -96,7 +97,17 @@ MIXED_CASES = [
                                           max_iters=8000, dt=1.0, epslim=1e30), (16, 1e6, 22)),
     ("cg-contrast1e6", tdeck.layered_2d(128, 1, solver=tdeck.CG, max_iters=8000, dt=1.0),
      (16, 1e6, 23)),
+    ("cheby-layered", tdeck.layered_2d(64, 2, solver=tdeck.CHEBY, presteps=20, max_iters=5000,
+                                       dt=0.05, epslim=1e30), None),
+    ("cheby-odd", tdeck.layered_2d(77, 1, solver=tdeck.CHEBY, presteps=20, max_iters=5000,
+                                   dt=0.05, epslim=1e30), None),
+    ("cheby-contrast1e3", tdeck.layered_2d(128, 1, solver=tdeck.CHEBY, presteps=30,
+                                           max_iters=10000, dt=1.0, epslim=1e30), (16, 1e3, 24)),
 ]
+
+
+def _work(reps):
+    return sum(r["n_cg_steps"] + r["n_ppcg_outer"] + r["n_cheby_steps"] for r in reps)
 
 
 @pytest.mark.parametrize("name,text,blk", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
@@ -120,9 +131,41 @@ def test_mixed_precision_reaches_fp64_residual(hip, name, text, blk):
     # the same algorithm as the FP64 solve, and about as much of it
     if s.solver == tdeck.PPCG:
         assert all(r["n_ppcg_outer"] > 0 for r in reps_m + reps_d)
-    it_m = sum(r["n_cg_steps"] + r["n_ppcg_outer"] for r in reps_m)
-    it_d = sum(r["n_cg_steps"] + r["n_ppcg_outer"] for r in reps_d)
-    assert it_m <= 1.25 * it_d + 5, (it_m, it_d)
+    if s.solver == tdeck.CHEBY:
+        assert all(r["n_cheby_steps"] > 0 for r in reps_m + reps_d)
+    it_m, it_d = _work(reps_m), _work(reps_d)
+    assert it_m <= 1.25 * it_d + 10, (it_m, it_d)
+
+
+# 1024^2: CG and PPCG on seeded block fields (64^2-cell blocks, log-uniform
+# density over a 1e3 contrast, dt 1, PPCG after 20 CG iterations);
+# Chebyshev on the reference's layered deck (presteps 30) -- Chebyshev needs
+# the eigenvalue bounds that the reference's switch rule gives it.  The
+# returned u meets 1e-10 (numpy residual) and lies within 1e-6 (relative) of
+# the FP64 GPU solve to the same target.
+LARGE_CASES = [("cg-blocks", tdeck.CG, True), ("cheby-layered", tdeck.CHEBY, False),
+               ("ppcg-blocks", tdeck.PPCG, True)]
+
+
+@pytest.mark.parametrize("name,solver,blk", LARGE_CASES, ids=[c[0] for c in LARGE_CASES])
+def test_mixed_precision_1024(hip, name, solver, blk):
+    if blk:
+        text = tdeck.layered_2d(1024, 1, solver=solver, inner=10, max_iters=20000, dt=1.0,
+                                epslim=1e30)
+    else:
+        text = tdeck.layered_2d(1024, 1, solver=solver, presteps=30, max_iters=20000)
+    s, states = tdeck.parse_deck(text, 2)
+    inject = (blocks.block_fields((1024, 1024), 64, seed=31, rho=(1.0, 1e3), energy=(0.1, 25.0),
+                                  halo_depth=s.halo_depth) if blk else None)
+    reps_m, f_m, _ = _gpu_solve(s, states, inject, 1, TOL)
+    reps_d, f_d, _ = _gpu_solve(s, states, inject, 0, TOL)
+    assert reps_m[0]["rel_residual"] <= TOL and reps_d[0]["rel_residual"] <= TOL
+    res_m = _residual(s, f_m)
+    assert res_m <= TOL * (1 + 1e-6), res_m
+    inner = (slice(s.halo_depth, -s.halo_depth),) * 2
+    err = np.linalg.norm((f_m["u"] - f_d["u"])[inner]) / np.linalg.norm(f_d["u"][inner])
+    assert err < 1e-6, err
+    assert _work(reps_m) <= 1.25 * _work(reps_d) + 10, (_work(reps_m), _work(reps_d))
 
 
 def test_mixed_precision_rejects_unsupported(hip):
@@ -131,7 +174,7 @@ def test_mixed_precision_rejects_unsupported(hip):
     with Problem.from_deck(tdeck.cuboid_3d(8, 1), 3) as prob:
         with pytest.raises(TealeafError):
             prob.set_solve(1, TOL)
-    with Problem.from_deck(tdeck.layered_2d(16, 1, solver=tdeck.CHEBY), 2) as prob:
+    with Problem.from_deck(tdeck.layered_2d(16, 1, solver=tdeck.JACOBI), 2) as prob:
         with pytest.raises(TealeafError):
             prob.set_solve(1, TOL)
         with pytest.raises(TealeafError):
