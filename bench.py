@@ -757,6 +757,7 @@ def blocks3d_run(args):
     solved to ||u0 - A u|| <= 1e-10 ||u0||.  Reports the wall time per step,
     the iterations the reference would print, and the per-launch time of
     the PPCG inner sweep (72 B/cell)."""
+    import numpy as np
     import torch
     from tealeaf_amd import blocks
     from tealeaf_amd import deck as tdeck
@@ -793,16 +794,45 @@ def blocks3d_run(args):
                   file=sys.stderr, flush=True)
         ms, nl = ctypes.c_double(), ctypes.c_long()
         L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
+        u64 = prob.get("u")
     total = sum(st["seconds"] for st in steps)
     iters = sum(st["printed_cg"] + st["printed_ppcg"] for st in steps)
     avg_ms = ms.value / max(nl.value, 1)
     gbs = BYTES_PER_CELL[timed] * n ** 3 / (avg_ms * 1e-3) / 1e9 if nl.value else None
+    # the same steps FP32-refined (FP32 sweeps, FP64 residual replacement)
+    steps32 = []
+    with Problem(s, states) as prob:
+        prob.inject(dens, en)
+        prob.set_solve(1, 1e-10)
+        for tt in range(args.blocks3d_steps):
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rep = prob.step(tt)
+            L.tlhip_synchronize()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            steps32.append({"seconds": dt, "printed_cg": rep["printed_cg_iters"],
+                            "printed_ppcg": rep["printed_inner_iters"],
+                            "true_residual_evals": rep["n_true_residuals"],
+                            "rel_residual": rep["rel_residual"]})
+            print(f"[bench] 3d blocks fp32 step {tt}: {dt:.3f} s, {rep['printed_cg_iters']} CG + "
+                  f"{rep['printed_inner_iters']} PPCG, rel residual {rep['rel_residual']:.2e}",
+                  file=sys.stderr, flush=True)
+        u32 = prob.get("u")
+    total32 = sum(st["seconds"] for st in steps32)
+    hd = s.halo_depth
+    a = u32.reshape(s.shape())[hd:-hd, hd:-hd, hd:-hd]
+    b = u64.reshape(s.shape())[hd:-hd, hd:-hd, hd:-hd]
     return {"workload": "3D 256^3 [0,10]^3, recip conductivity, 16^3-cell blocks (density "
                         "log-uniform 0.1..100, energy U(0.1,25), seed 256), dt 0.004, PPCG 10 inner "
                         "(presteps 20), relative residual 1e-10, FP64",
             "steps": steps, "seconds_per_step": total / len(steps), "iters_per_s": iters / total,
             "kernel": {timed: {"launches": nl.value, "avg_ms": avg_ms, "gbs": gbs,
-                               "frac": gbs / HBM_PEAK_GBS if gbs else None}}}
+                               "frac": gbs / HBM_PEAK_GBS if gbs else None}},
+            "fp32_refined": {"steps": steps32, "seconds_per_step": total32 / len(steps32),
+                             "time_ratio": total32 / total,
+                             "rel_error_vs_fp64": float(np.linalg.norm(a - b) / np.linalg.norm(b))}}
 
 
 def adjoint_run(args):

@@ -149,6 +149,7 @@ struct tlhip_problem {
     float *kx32 = nullptr, *ky32 = nullptr, *p32 = nullptr, *w32 = nullptr, *r32 = nullptr,
           *d32 = nullptr, *sd32 = nullptr, *sd32b = nullptr, *r32b = nullptr, *d32b = nullptr,
           *p32b = nullptr;
+    float* kz32 = nullptr;       // 3D
     double* alpha32 = nullptr;   // device: a PPCG step's cg_calc_ur alpha (rro / pw)
     // A CG/PPCG step's calc_p waits to be fused into the next step's calc_w
     // (tlhip2d_cg_calc_pw); flush_p() applies it where p is read otherwise.
@@ -968,21 +969,30 @@ struct tlhip_problem {
     // only ever declared on a true FP64 residual.
     static constexpr double kDrop = 0.1;
 
+    // FP32 rows of the chunk (3D: planes of rows)
+    long rows32() const { return dims == 3 ? (long)y * z : (long)y; }
+    int zz() const { return dims == 3 ? z : 1; }
+
     int mixed_alloc()
     {
         if (kx32) return 0;
-        if (dims != 2) return tl_fail(TLHIP_ERR_ARG, "mixed precision: 2D only");
         P = (x + 3) / 4 * 4;
-        const size_t bytes = sizeof(float) * (size_t)P * (size_t)y;
+        const size_t bytes = sizeof(float) * (size_t)P * (size_t)rows32();
         float** all[] = {&kx32, &ky32, &p32, &w32, &r32, &d32, &sd32, &sd32b, &r32b, &d32b, &p32b};
         for (float** f : all) TRY(tlhip_alloc(bytes, (void**)f));
+        if (dims == 3) TRY(tlhip_alloc(bytes, (void**)&kz32));
         TRY(tlhip_alloc(sizeof(double), (void**)&alpha32));
         return 0;
     }
 
-    // FP32 calc_w, with a deferred calc_p fused in (as calc_w_pending)
+    // FP32 calc_w, with a deferred calc_p fused in (as calc_w_pending; 3D
+    // applies it first)
     int calc_w32_pending(double* pw)
     {
+        if (dims == 3) {
+            TRY(flush_p32());
+            return tl::mix::calc_w3_32(x, y, z, hd, P, pw, p32, w32, kx32, ky32, kz32);
+        }
         if (!pending_p) return tl::mix::calc_w32(x, y, hd, P, pw, p32, w32, kx32, ky32);
         pending_p = false;
         TRY(tl::mix::calc_pw32(x, y, hd, P, pending_beta, pw, p32, p32b, r32, w32, kx32, ky32));
@@ -1009,7 +1019,7 @@ struct tlhip_problem {
     {
         if (!pending_p) return 0;
         pending_p = false;
-        return tl::mix::calc_p32(x, y, hd, P, pending_beta, false, p32, r32);
+        return tl::mix::calc_p32(x, y, zz(), hd, P, pending_beta, false, p32, r32);
     }
 
     // u += d (FP64); r32 = u0 - A u computed in FP64; *rr = its squared norm.
@@ -1018,9 +1028,12 @@ struct tlhip_problem {
     int replace_residual(double* rr)
     {
         TRY(flush_p32());
-        TRY(tl::mix::promote(x, y, hd, P, u, d32));
+        TRY(tl::mix::promote(x, y, zz(), hd, P, u, d32));
         *rr = 0.0;
-        TRY(tl::mix::residual64(x, y, hd, P, u, u0, kx, ky, r32, rr));
+        if (dims == 3)
+            TRY(tl::mix::residual3_64(x, y, z, hd, P, u, u0, kx, ky, kz, r32, rr));
+        else
+            TRY(tl::mix::residual64(x, y, hd, P, u, u0, kx, ky, r32, rr));
         rep->n_true_residuals++;
         rep->rel_residual = bb > 0.0 ? std::sqrt(*rr / bb) : 0.0;
         return 0;
@@ -1032,7 +1045,7 @@ struct tlhip_problem {
         TRY(calc_w32_pending(&pw));
         const double alpha = *rro / pw;
         cg_alphas[tt] = alpha;
-        TRY(tl::mix::calc_ur32(x, y, hd, P, alpha, rrn, d32, p32, r32, w32));
+        TRY(tl::mix::calc_ur32(x, y, zz(), hd, P, alpha, rrn, d32, p32, r32, w32));
         const double beta = *rrn / *rro;
         cg_betas[tt] = beta;
         pending_p = true;
@@ -1042,12 +1055,36 @@ struct tlhip_problem {
         return 0;
     }
 
+    // 3D: calc_w, calc_ur, ppcg_init and one launch per inner step
+    // (ppcg_solver_driver.c), then the 2-norm of r.
+    int mixed_ppcg_step3(double* rro, double* rrn)
+    {
+        double pw = 0.0, unused = 0.0;
+        TRY(calc_w32_pending(&pw));
+        TRY(tl::mix::calc_ur32(x, y, z, hd, P, *rro / pw, &unused, d32, p32, r32, w32));
+        if (s.ppcg_inner_steps > 0) TRY(tl::mix::ppcg_init3_32(x, y, z, hd, P, theta, r32, sd32));
+        for (int k = 0; k < s.ppcg_inner_steps; ++k) {
+            TRY(tl::mix::ppcg_inner3_32(x, y, z, hd, P, cheby_alphas[k], cheby_betas[k], sd32,
+                                        sd32b, r32, d32, kx32, ky32, kz32));
+            std::swap(sd32, sd32b);
+            rep->n_ppcg_inner++;
+        }
+        *rrn = 0.0;
+        TRY(tl::mix::norm32(x, y, z, hd, P, r32, rrn));
+        pending_p = true;
+        pending_beta = *rrn / *rro;
+        *rro = *rrn;
+        rep->n_ppcg_outer++;
+        return 0;
+    }
+
     int mixed_ppcg_step(double* rro, double* rrn)
     {
+        if (dims == 3) return mixed_ppcg_step3(rro, rrn);
         if (s.ppcg_inner_steps == 0) {
             double pw = 0.0, unused = 0.0;
             TRY(calc_w32_pending(&pw));
-            TRY(tl::mix::calc_ur32(x, y, hd, P, *rro / pw, &unused, d32, p32, r32, w32));
+            TRY(tl::mix::calc_ur32(x, y, 1, hd, P, *rro / pw, &unused, d32, p32, r32, w32));
         } else {
             TRY(calc_w32_pending_alpha(*rro));    // alpha stays on the device
         }
@@ -1071,7 +1108,7 @@ struct tlhip_problem {
             done += cnt;
             rep->n_ppcg_inner += cnt;
         }
-        if (done == 0) TRY(tl::mix::norm32(x, y, hd, P, r32, rrn));
+        if (done == 0) TRY(tl::mix::norm32(x, y, 1, hd, P, r32, rrn));
         const double beta = *rrn / *rro;
         pending_p = true;
         pending_beta = beta;
@@ -1080,20 +1117,43 @@ struct tlhip_problem {
         return 0;
     }
 
-    int mixed_driver(double rx, double ry, double* error)
+    // One Chebyshev launch of cnt turns on A d = b (b = r32), r in r32b;
+    // *rrn = the last turn's r.r.  2D: one fused launch; 3D: one per turn.
+    int mixed_cheby_turns(int first, int cnt, double* rrn)
+    {
+        if (dims == 2) {
+            TRY(tl::mix::cheby_iterations32(x, y, hd, P, cnt, &cheby_alphas[first],
+                                            &cheby_betas[first], neighbours, d32, d32b, r32, p32,
+                                            p32b, r32b, kx32, ky32, rrn));
+            std::swap(d32, d32b);
+            std::swap(p32, p32b);
+            return 0;
+        }
+        for (int k = 0; k < cnt; ++k) {
+            TRY(tl::mix::cheby_iter3_32(x, y, z, hd, P, cheby_alphas[first + k],
+                                        cheby_betas[first + k], d32, d32b, r32, p32, r32b, kx32,
+                                        ky32, kz32, k == cnt - 1 ? rrn : nullptr));
+            std::swap(d32, d32b);
+        }
+        return 0;
+    }
+
+    int mixed_driver(double rx, double ry, double rz, double* error)
     {
         if (s.solver != 1 && s.solver != 2 && s.solver != 3)
             return tl_fail(TLHIP_ERR_ARG, "mixed precision: CG, Chebyshev or PPCG only (solver %d)",
                            s.solver);
         TRY(mixed_alloc());
         double rro;
-        TRY(cg_init_driver(rx, ry, 0.0, &rro));   // FP64 kx, ky, u, r = u0 - A u, p = r
-        TRY(tl::mix::demote(kx, x, y, kx32, P));
-        TRY(tl::mix::demote(ky, x, y, ky32, P));
-        TRY(tl::mix::demote(r, x, y, r32, P));
+        TRY(cg_init_driver(rx, ry, rz, &rro));    // FP64 kx, ky(, kz), u, r = u0 - A u, p = r
+        const int ny = (int)rows32();             // demote treats planes as rows
+        TRY(tl::mix::demote(kx, x, ny, kx32, P));
+        TRY(tl::mix::demote(ky, x, ny, ky32, P));
+        if (dims == 3) TRY(tl::mix::demote(kz, x, ny, kz32, P));
+        TRY(tl::mix::demote(r, x, ny, r32, P));
         pending_p = false;
-        TRY(tl::mix::calc_p32(x, y, hd, P, 0.0, true, p32, r32));
-        TRY(tl::mix::zero32(d32, (long)P * y));
+        TRY(tl::mix::calc_p32(x, y, zz(), hd, P, 0.0, true, p32, r32));
+        TRY(tl::mix::zero32(d32, (long)P * rows32()));
         const double target = rel_tol > 0.0 ? rel_tol * rel_tol * bb : s.eps;
         double rr_true = rro, last_true = rro;
         *error = rro;
@@ -1110,16 +1170,12 @@ struct tlhip_problem {
                     cheby_coefs(s.max_iters - tt);
                     TRY(replace_residual(&rr_true));          // d = 0, b = u0 - A u
                     last_true = rr_true;
-                    TRY(tl::mix::cheby_init32(x, y, hd, P, theta, p32, r32, d32));
+                    TRY(tl::mix::cheby_init32(x, y, zz(), hd, P, theta, p32, r32, d32));
                 } else {
                     cnt = std::min({g_cheby_fusion > 1 ? g_cheby_fusion : 1, s.max_iters - tt,
                                     (int)cheby_alphas.size() - nppcg});
                 }
-                TRY(tl::mix::cheby_iterations32(x, y, hd, P, cnt, &cheby_alphas[nppcg],
-                                                &cheby_betas[nppcg], neighbours, d32, d32b, r32,
-                                                p32, p32b, r32b, kx32, ky32, &rrn));
-                std::swap(d32, d32b);
-                std::swap(p32, p32b);
+                TRY(mixed_cheby_turns(nppcg, cnt, &rrn));
                 rep->n_cheby_steps += cnt;
                 if (nppcg == 1) est = est_iterations(rrn, bb);
                 nppcg += cnt - 1;
@@ -1338,7 +1394,7 @@ struct tlhip_problem {
         TRY(halo_update(2));
         double error = 1e+10;
         if (precision == 1) {
-            TRY(mixed_driver(rx, ry, &error));
+            TRY(mixed_driver(rx, ry, rz, &error));
         } else switch (s.solver) {
         case 0: TRY(jacobi_driver(rx, ry, rz, &error)); break;
         case 1: TRY(cg_driver(rx, ry, rz, &error)); break;
@@ -1399,6 +1455,7 @@ struct tlhip_problem {
         float* all32[] = {kx32, ky32, p32, w32, r32, d32, sd32, sd32b, r32b, d32b, p32b};
         for (float* a : all32)
             if (a) (void)tlhip_free(a);
+        if (kz32) (void)tlhip_free(kz32);
         if (alpha32) (void)tlhip_free(alpha32);
         if (adopted) {      // the caller's fields stay; only the partners of r and p are ours
             for (double* a : {r == home[2] ? r_alt : r, p == home[3] ? p_alt : p})
@@ -1453,8 +1510,6 @@ extern "C" int tlhip_problem_set_solve(tlhip_problem* P, int precision, double r
     if (!(rel_residual >= 0.0))
         return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: rel_residual must be >= 0");
     if (precision == 1) {
-        if (P->dims != 2)
-            return tl_fail(TLHIP_ERR_ARG, "tlhip_problem_set_solve: mixed precision is 2D only");
         if (P->s.solver != 1 && P->s.solver != 2 && P->s.solver != 3)
             return tl_fail(TLHIP_ERR_ARG,
                            "tlhip_problem_set_solve: mixed precision needs CG, Chebyshev or PPCG");

@@ -1,7 +1,8 @@
-// Internal interface of the mixed-precision 2D kernels (FP32 sweeps, FP64
+// Internal interface of the mixed-precision kernels (FP32 sweeps, FP64
 // dots and updates) used by the native driver.  FP32 arrays have row pitch
-// P (a multiple of 4, >= x); FP64 arrays have row pitch x.  Scalars come
-// back synchronously, like the reference's reductions.
+// P (a multiple of 4, >= x); FP64 arrays have row pitch x.  A 3D field is z
+// planes of y such rows (z <= 1: 2D); the pointwise kernels take z.
+// Scalars come back synchronously, like the reference's reductions.
 #pragma once
 
 namespace tl {
@@ -18,14 +19,15 @@ int calc_w32(int x, int y, int hd, int P, double* pw, const float* p, float* w, 
 int calc_pw32(int x, int y, int hd, int P, double beta, double* pw, const float* p_in,
               float* p_out, const float* r, float* w, const float* kx, const float* ky);
 // d += alpha p; r -= alpha w; *rrn = sum r*r (FP64)
-int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,
-              float* r, const float* w);
+int calc_ur32(int x, int y, int z, int hd, int P, double alpha, double* rrn, float* d,
+              const float* p, float* r, const float* w);
 // p = beta p + r, or p = r when init
-int calc_p32(int x, int y, int hd, int P, double beta, bool init, float* p, const float* r);
+int calc_p32(int x, int y, int z, int hd, int P, double beta, bool init, float* p,
+             const float* r);
 // *out = sum a*a (FP64)
-int norm32(int x, int y, int hd, int P, const float* a, double* out);
+int norm32(int x, int y, int z, int hd, int P, const float* a, double* out);
 // u += d (FP64); d = 0
-int promote(int x, int y, int hd, int P, double* u, float* d);
+int promote(int x, int y, int z, int hd, int P, double* u, float* d);
 // calc_w32 (update false) or calc_pw32 (true) with *alpha_dev = rro / pw
 // written on the device and no wait for the result; p_out's halo is not
 // written (no kernel here reads an FP32 halo)
@@ -51,10 +53,30 @@ int cheby_iterations32(int x, int y, int hd, int P, int count, const double* alp
                        float* d_out, const float* b, const float* p_in, float* p_out, float* r,
                        const float* kx, const float* ky, double* rrn);
 // Chebyshev start on the correction equation (FP32): p = r / theta, d += p
-int cheby_init32(int x, int y, int hd, int P, double theta, float* p, const float* r, float* d);
+int cheby_init32(int x, int y, int z, int hd, int P, double theta, float* p, const float* r,
+                 float* d);
 // r32 = (float)(u0 - A u) computed in FP64; *rr = sum (u0 - A u)^2 (FP64)
 int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
                const double* ky, float* r32, double* rr);
+
+// 3D (7-point operator, neighbours outside the interior mirrored):
+// w = A p, *pw = sum w*p (FP64)
+int calc_w3_32(int x, int y, int z, int hd, int P, double* pw, const float* p, float* w,
+               const float* kx, const float* ky, const float* kz);
+// sd = r / theta
+int ppcg_init3_32(int x, int y, int z, int hd, int P, double theta, const float* r, float* sd);
+// one PPCG inner step: w = A sd_in, r -= w, d += sd_in, sd_out = alpha sd_in + beta r
+int ppcg_inner3_32(int x, int y, int z, int hd, int P, double alpha, double beta,
+                   const float* sd_in, float* sd_out, float* r, float* d, const float* kx,
+                   const float* ky, const float* kz);
+// one Chebyshev turn on A d = b: r = b - A d_in, p = alpha p + beta r,
+// d_out = d_in + p; *rr = sum r*r (FP64) when rr is not null
+int cheby_iter3_32(int x, int y, int z, int hd, int P, double alpha, double beta,
+                   const float* d_in, float* d_out, const float* b, float* p, float* r,
+                   const float* kx, const float* ky, const float* kz, double* rr);
+// r32 = (float)(u0 - A u) computed in FP64; *rr = sum (u0 - A u)^2 (FP64)
+int residual3_64(int x, int y, int z, int hd, int P, const double* u, const double* u0,
+                 const double* kx, const double* ky, const double* kz, float* r32, double* rr);
 
 }  // namespace mix
 }  // namespace tl

@@ -1,5 +1,5 @@
-// FP32 kernels of the mixed-precision 2D solve (FP32 Krylov sweeps inside an
-// FP64 residual-replacement loop).
+// FP32 kernels of the mixed-precision solve (FP32 Krylov sweeps inside an
+// FP64 residual-replacement loop), 2D and 3D.
 //
 // The reference solves in FP64 throughout (2d/drivers/cg_driver.c,
 // ppcg_driver.c).  Here the CG and PPCG vector sweeps run on FP32 copies of
@@ -28,14 +28,18 @@ using C4 = Cells<float>;
 // ---------------------------------------------------------------- pointwise
 // Thread q of a row covers the 4-cell group [4q, 4q+4) of the pitched row;
 // groups that meet the interior are visited, cells outside it are masked.
+// 3D fields are rows of x cells too: row (k*y + j) holds plane k's row j,
+// and the interior rows are those with j and k inside (z <= 1: 2D).
 struct Groups {
     int hd, x, P;      // FP32 pitch P
     int q0, nq;        // first group, groups per row
-    int y0, ny;        // interior rows
-    __host__ __device__ long total() const { return (long)nq * ny; }
+    int y0, ny;        // interior rows of a plane
+    int Y, z, z0, nz;  // rows per plane, planes, interior planes
+    __host__ __device__ long total() const { return (long)nq * ny * nz; }
+    __device__ int row_of(long r) const { return (z0 + (int)(r / ny)) * Y + y0 + (int)(r % ny); }
 };
 
-Groups groups_of(int x, int y, int hd, int P)
+Groups groups_of(int x, int y, int z, int hd, int P)
 {
     Groups G;
     G.hd = hd;
@@ -45,6 +49,10 @@ Groups groups_of(int x, int y, int hd, int P)
     G.nq = (x - hd + 3) / 4 - G.q0;
     G.y0 = hd;
     G.ny = y - 2 * hd;
+    G.Y = y;
+    G.z = z > 1 ? z : 1;
+    G.z0 = z > 1 ? hd : 0;
+    G.nz = z > 1 ? z - 2 * hd : 1;
     return G;
 }
 
@@ -54,7 +62,7 @@ __global__ void __launch_bounds__(kBlock) k_groups(Groups G, Op op, TlReduce rd)
     double acc[Op::NRED > 0 ? Op::NRED : 1] = {};
     for (long t = blockIdx.x * (long)kBlock + threadIdx.x; t < G.total();
          t += (long)gridDim.x * kBlock) {
-        const int row = G.y0 + (int)(t / G.nq);
+        const int row = G.row_of(t / G.nq);
         const int c = 4 * (G.q0 + (int)(t % G.nq));
         unsigned mask = 0;
 #pragma unroll
@@ -270,7 +278,7 @@ struct OpResidual64 {
 template <class Op>
 int launch_groups(const Groups& G, const Op& op, const char* what, double* out)
 {
-    if (G.nq <= 0 || G.ny <= 0) {
+    if (G.nq <= 0 || G.ny <= 0 || G.nz <= 0) {
         if (out) *out = 0.0;
         return 0;
     }
@@ -304,6 +312,197 @@ __global__ void __launch_bounds__(kBlock) k_zero32(float* __restrict__ a, long n
         a[t] = 0.0f;
 }
 
+// ---------------------------------------------------------------- 3D
+// A a (FP32, 7-point, 3d/shared.h:52-58 association) at a group's four
+// cells of row `row` (plane row / Y, row row % Y); a neighbour outside the
+// interior takes the cell's own value (the depth-1 reflective halo).
+// Cells outside the interior (mask) get don't-care values; only in-bounds
+// elements are read.
+__device__ __forceinline__ void apply3_32(const Groups& G, const float* __restrict__ a,
+                                          const float* __restrict__ kx,
+                                          const float* __restrict__ ky,
+                                          const float* __restrict__ kz, int row, int c, C4& cen,
+                                          C4& av)
+{
+    const int j = row % G.Y, k = row / G.Y;
+    const long P = G.P, PY = (long)G.P * G.Y;
+    const long i0 = (long)row * P + c;
+    cen = ldc(a, i0);
+    const C4 so = j == G.hd ? cen : ldc(a, i0 - P);
+    const C4 no = j == G.Y - G.hd - 1 ? cen : ldc(a, i0 + P);
+    const C4 dn = k == G.hd ? cen : ldc(a, i0 - PY);
+    const C4 up = k == G.z - G.hd - 1 ? cen : ldc(a, i0 + PY);
+    const float el = c >= 1 ? a[i0 - 1] : 0.0f;
+    const float er = c + 4 < G.x ? a[i0 + 4] : 0.0f;
+    const C4 kxc = ldc(kx, i0);
+    const float kxe = c + 4 < G.x ? kx[i0 + 4] : 0.0f;
+    const C4 kyc = ldc(ky, i0), kyn = ldc(ky, i0 + P);
+    const C4 kzc = ldc(kz, i0), kzu = ldc(kz, i0 + PY);
+    const int xin0 = G.hd, xin1 = G.x - G.hd;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float cv = cen.v[i];
+        const float lft = c + i == xin0 ? cv : (i == 0 ? el : cen.v[i - 1]);
+        const float rgt = c + i == xin1 - 1 ? cv : (i == 3 ? er : cen.v[i + 1]);
+        const float kxr = i == 3 ? kxe : kxc.v[i + 1];
+        const float diag = ((1.0f + (kxr + kxc.v[i])) + (kyn.v[i] + kyc.v[i])) +
+                           (kzu.v[i] + kzc.v[i]);
+        av.v[i] = diag * cv - (kxr * rgt + kxc.v[i] * lft) - (kyn.v[i] * no.v[i] + kyc.v[i] * so.v[i]) -
+                  (kzu.v[i] * up.v[i] + kzc.v[i] * dn.v[i]);
+    }
+}
+
+__device__ __forceinline__ void store_masked(float* __restrict__ f, long i0, unsigned mask,
+                                             const C4& v)
+{
+    if (mask == 15u) {
+        stc(f, i0, v);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (mask >> i & 1) f[i0 + i] = v.v[i];
+    }
+}
+
+// w = A p; pw += w*p   (3d/c_kernels/omp3/cg_solver.c calc_w, FP32, FP64 sum)
+struct OpCalcW3_32 {
+    static constexpr int NRED = 1;
+    Groups G;
+    const float* p;
+    float* w;
+    const float *kx, *ky, *kz;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        C4 cen, av;
+        apply3_32(G, p, kx, ky, kz, row, c, cen, av);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (mask >> i & 1) acc[0] += (double)av.v[i] * (double)cen.v[i];
+        store_masked(w, (long)row * G.P + c, mask, av);
+    }
+};
+
+// w = A sd; r -= w; d += sd; sd_out = alpha sd + beta r
+// (3d/c_kernels/omp3/ppcg_solver.c inner iteration, FP32; d is the
+// correction to u)
+struct OpPpcgInner3_32 {
+    static constexpr int NRED = 0;
+    Groups G;
+    float alpha, beta;
+    const float* sd;
+    float* sd_out;
+    float* r;
+    float* d;
+    const float *kx, *ky, *kz;
+    __device__ void apply(int row, int c, unsigned mask, double*) const
+    {
+        C4 cen, av;
+        apply3_32(G, sd, kx, ky, kz, row, c, cen, av);
+        const long i0 = (long)row * G.P + c;
+        C4 rv = ldc(r, i0), dv = ldc(d, i0), sv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rv.v[i] = rv.v[i] - av.v[i];
+            dv.v[i] = dv.v[i] + cen.v[i];
+            sv.v[i] = alpha * cen.v[i] + beta * rv.v[i];
+        }
+        store_masked(r, i0, mask, rv);
+        store_masked(d, i0, mask, dv);
+        store_masked(sd_out, i0, mask, sv);
+    }
+};
+
+// r = b - A d; p = alpha p + beta r; d_out = d + p; rr += r*r
+// (3d/c_kernels/omp3/cheby_solver.c iterate and calc_u, FP32, on the
+// correction equation A d = b)
+template <bool NORM>
+struct OpChebyIter3_32 {
+    static constexpr int NRED = NORM ? 1 : 0;
+    Groups G;
+    float alpha, beta;
+    const float* d;
+    float* d_out;
+    const float* b;
+    float* p;
+    float* r;
+    const float *kx, *ky, *kz;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        C4 cen, av;
+        apply3_32(G, d, kx, ky, kz, row, c, cen, av);
+        const long i0 = (long)row * G.P + c;
+        const C4 bv = ldc(b, i0);
+        C4 pv = ldc(p, i0), rv, dv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rv.v[i] = bv.v[i] - av.v[i];
+            pv.v[i] = alpha * pv.v[i] + beta * rv.v[i];
+            dv.v[i] = cen.v[i] + pv.v[i];
+            if constexpr (NORM) {
+                if (mask >> i & 1) acc[0] += (double)rv.v[i] * (double)rv.v[i];
+            }
+        }
+        store_masked(r, i0, mask, rv);
+        store_masked(p, i0, mask, pv);
+        store_masked(d_out, i0, mask, dv);
+    }
+};
+
+// sd = r / theta   (3d ppcg_init)
+struct OpScale32 {
+    static constexpr int NRED = 0;
+    int P;
+    float theta;
+    const float* r;
+    float* sd;
+    __device__ void apply(int row, int c, unsigned mask, double*) const
+    {
+        const long i0 = (long)row * P + c;
+        C4 v = ldc(r, i0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v.v[i] = v.v[i] / theta;
+        store_masked(sd, i0, mask, v);
+    }
+};
+
+// r = u0 - A u in FP64 (3d/c_kernels/omp3/solver_methods.c, neighbours
+// outside the interior mirrored), stored as FP32; rr += r*r in FP64.
+struct OpResidual3_64 {
+    static constexpr int NRED = 1;
+    Groups G;
+    const double* u;
+    const double* u0;
+    const double *kx, *ky, *kz;
+    float* r32;
+    __device__ void apply(int row, int c, unsigned mask, double* acc) const
+    {
+        const int j = row % G.Y, k = row / G.Y;
+        const long X = G.x, XY = (long)G.x * G.Y;
+        const int xin0 = G.hd, xin1 = G.x - G.hd;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!(mask >> i & 1)) continue;
+            const int q = c + i;
+            const long n = (long)row * X + q;
+            const double cen = u[n];
+            const double lft = q == xin0 ? cen : u[n - 1];
+            const double rgt = q == xin1 - 1 ? cen : u[n + 1];
+            const double so = j == G.hd ? cen : u[n - X];
+            const double no = j == G.Y - G.hd - 1 ? cen : u[n + X];
+            const double dn = k == G.hd ? cen : u[n - XY];
+            const double up = k == G.z - G.hd - 1 ? cen : u[n + XY];
+            const double kxc = kx[n], kxr = kx[n + 1], kyc = ky[n], kyn = ky[n + X];
+            const double kzc = kz[n], kzu = kz[n + XY];
+            const double diag = ((1.0 + (kxr + kxc)) + (kyn + kyc)) + (kzu + kzc);
+            const double a = diag * cen - (kxr * rgt + kxc * lft) - (kyn * no + kyc * so) -
+                             (kzu * up + kzc * dn);
+            const double rv = u0[n] - a;
+            r32[(long)row * G.P + q] = (float)rv;
+            acc[0] += rv * rv;
+        }
+    }
+};
+
 
 }  // namespace
 
@@ -326,42 +525,91 @@ int demote(const double* src, int x, int y, float* dst, int P)
     return tl_check_launch(what);
 }
 
-int calc_ur32(int x, int y, int hd, int P, double alpha, double* rrn, float* d, const float* p,
-              float* r, const float* w)
+int calc_ur32(int x, int y, int z, int hd, int P, double alpha, double* rrn, float* d,
+              const float* p, float* r, const float* w)
 {
     const OpUr32 op{P, (float)alpha, d, p, r, w};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_calc_ur32", rrn);
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip2d_mix_calc_ur32", rrn);
 }
 
-int calc_p32(int x, int y, int hd, int P, double beta, bool init, float* p, const float* r)
+int calc_p32(int x, int y, int z, int hd, int P, double beta, bool init, float* p,
+             const float* r)
 {
     const OpP32 op{P, (float)beta, init ? 1 : 0, p, r};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_calc_p32", nullptr);
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip2d_mix_calc_p32", nullptr);
 }
 
-int norm32(int x, int y, int hd, int P, const float* a, double* out)
+int norm32(int x, int y, int z, int hd, int P, const float* a, double* out)
 {
     const OpNorm32 op{P, a};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_norm32", out);
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip2d_mix_norm32", out);
 }
 
-int cheby_init32(int x, int y, int hd, int P, double theta, float* p, const float* r, float* d)
+int cheby_init32(int x, int y, int z, int hd, int P, double theta, float* p, const float* r,
+                 float* d)
 {
     const OpChebyInit32 op{P, (float)theta, p, r, d};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_cheby_init32", nullptr);
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip2d_mix_cheby_init32", nullptr);
 }
 
-int promote(int x, int y, int hd, int P, double* u, float* d)
+int promote(int x, int y, int z, int hd, int P, double* u, float* d)
 {
     const OpPromote op{P, x, u, d};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_promote", nullptr);
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip2d_mix_promote", nullptr);
 }
 
 int residual64(int x, int y, int hd, int P, const double* u, const double* u0, const double* kx,
                const double* ky, float* r32, double* rr)
 {
     const OpResidual64 op{P, x, y, hd, u, u0, kx, ky, r32};
-    return launch_groups(groups_of(x, y, hd, P), op, "tlhip2d_mix_residual64", rr);
+    return launch_groups(groups_of(x, y, 1, hd, P), op, "tlhip2d_mix_residual64", rr);
+}
+
+int calc_w3_32(int x, int y, int z, int hd, int P, double* pw, const float* p, float* w,
+               const float* kx, const float* ky, const float* kz)
+{
+    const Groups G = groups_of(x, y, z, hd, P);
+    const OpCalcW3_32 op{G, p, w, kx, ky, kz};
+    *pw = 0.0;
+    return launch_groups(G, op, "tlhip3d_mix_calc_w32", pw);
+}
+
+int ppcg_init3_32(int x, int y, int z, int hd, int P, double theta, const float* r, float* sd)
+{
+    const OpScale32 op{P, (float)theta, r, sd};
+    return launch_groups(groups_of(x, y, z, hd, P), op, "tlhip3d_mix_ppcg_init32", nullptr);
+}
+
+int ppcg_inner3_32(int x, int y, int z, int hd, int P, double alpha, double beta,
+                   const float* sd_in, float* sd_out, float* r, float* d, const float* kx,
+                   const float* ky, const float* kz)
+{
+    const Groups G = groups_of(x, y, z, hd, P);
+    const OpPpcgInner3_32 op{G, (float)alpha, (float)beta, sd_in, sd_out, r, d, kx, ky, kz};
+    return launch_groups(G, op, "tlhip3d_mix_ppcg_inner32", nullptr);
+}
+
+int cheby_iter3_32(int x, int y, int z, int hd, int P, double alpha, double beta,
+                   const float* d_in, float* d_out, const float* b, float* p, float* r,
+                   const float* kx, const float* ky, const float* kz, double* rr)
+{
+    const Groups G = groups_of(x, y, z, hd, P);
+    if (rr) {
+        const OpChebyIter3_32<true> op{G, (float)alpha, (float)beta, d_in, d_out, b, p, r, kx,
+                                       ky, kz};
+        return launch_groups(G, op, "tlhip3d_mix_cheby_iter32", rr);
+    }
+    const OpChebyIter3_32<false> op{G, (float)alpha, (float)beta, d_in, d_out, b, p, r, kx, ky,
+                                    kz};
+    return launch_groups(G, op, "tlhip3d_mix_cheby_iter32", nullptr);
+}
+
+int residual3_64(int x, int y, int z, int hd, int P, const double* u, const double* u0,
+                 const double* kx, const double* ky, const double* kz, float* r32, double* rr)
+{
+    const Groups G = groups_of(x, y, z, hd, P);
+    const OpResidual3_64 op{G, u, u0, kx, ky, kz, r32};
+    return launch_groups(G, op, "tlhip3d_mix_residual64", rr);
 }
 
 }  // namespace mix

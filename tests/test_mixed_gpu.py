@@ -103,6 +103,11 @@ MIXED_CASES = [
                                    dt=0.05, epslim=1e30), None),
     ("cheby-contrast1e3", tdeck.layered_2d(128, 1, solver=tdeck.CHEBY, presteps=30,
                                            max_iters=10000, dt=1.0, epslim=1e30), (16, 1e3, 24)),
+    ("3d-cg", tdeck.cuboid_3d(24, 2, max_iters=5000), None),
+    ("3d-ppcg", tdeck.cuboid_3d(24, 1, solver=tdeck.PPCG, inner=8, coefficient=2,
+                                max_iters=5000), None),
+    ("3d-cheby", tdeck.cuboid_3d(22, 1, solver=tdeck.CHEBY, max_iters=5000), None),
+    ("3d-cg-blocks", tdeck.cuboid_3d(32, 1, max_iters=8000, dt=0.5), (8, 1e3, 26)),
 ]
 
 
@@ -113,7 +118,7 @@ def _work(reps):
 @pytest.mark.parametrize("name,text,blk", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
 def test_mixed_precision_reaches_fp64_residual(hip, name, text, blk):
     from oracle.pyoracle import run_problem
-    s, states = tdeck.parse_deck(text, 2)
+    s, states = tdeck.parse_deck(text, 3 if name.startswith("3d") else 2)
     inject = _inject_for(s, blk)
     reps_m, f_m, _ = _gpu_solve(s, states, inject, 1, TOL, steps=s.end_step)
     reps_d, f_d, _ = _gpu_solve(s, states, inject, 0, TOL, steps=s.end_step)
@@ -125,7 +130,7 @@ def test_mixed_precision_reaches_fp64_residual(hip, name, text, blk):
     assert res_m <= TOL * (1 + 1e-6), res_m
     # the same state as the FP64 solves (u0 of the last step derives from it)
     u_ref = want["fields"]["u"].reshape(s.shape())
-    inner = (slice(s.halo_depth, -s.halo_depth),) * 2
+    inner = (slice(s.halo_depth, -s.halo_depth),) * s.dims
     err = np.linalg.norm((f_m["u"] - u_ref)[inner]) / np.linalg.norm(u_ref[inner])
     assert err < 1e-6, err
     # the same algorithm as the FP64 solve, and about as much of it
@@ -171,7 +176,7 @@ def test_mixed_precision_1024(hip, name, solver, blk):
 def test_mixed_precision_rejects_unsupported(hip):
     from tealeaf_amd.abi import TealeafError
     from tealeaf_amd.problem import Problem
-    with Problem.from_deck(tdeck.cuboid_3d(8, 1), 3) as prob:
+    with Problem.from_deck(tdeck.cuboid_3d(8, 1, solver=tdeck.JACOBI), 3) as prob:
         with pytest.raises(TealeafError):
             prob.set_solve(1, TOL)
     with Problem.from_deck(tdeck.layered_2d(16, 1, solver=tdeck.JACOBI), 2) as prob:
@@ -179,3 +184,31 @@ def test_mixed_precision_rejects_unsupported(hip):
             prob.set_solve(1, TOL)
         with pytest.raises(TealeafError):
             prob.set_solve(0, -1.0)
+
+
+# 3D at 128^3: CG on seeded 16^3-cell blocks (density log-uniform over a 1e3
+# contrast), PPCG on the reference's cuboid deck (recip conductivity); the
+# returned u meets 1e-10 (numpy residual) and lies within 1e-6 of the FP64
+# GPU solve.
+LARGE3_CASES = [
+    ("cg-blocks", tdeck.cuboid_3d(128, 1, max_iters=20000, dt=0.5), True),
+    ("ppcg", tdeck.cuboid_3d(128, 1, solver=tdeck.PPCG, inner=10, coefficient=2,
+                             max_iters=20000), False),
+]
+
+
+@pytest.mark.parametrize("name,text,blk", LARGE3_CASES, ids=[c[0] for c in LARGE3_CASES])
+def test_mixed_precision_3d_128(hip, name, text, blk):
+    s, states = tdeck.parse_deck(text, 3)
+    inject = (blocks.block_fields((128, 128, 128), 16, seed=41, rho=(1.0, 1e3),
+                                  energy=(0.1, 25.0), halo_depth=s.halo_depth) if blk else None)
+    reps_m, f_m, _ = _gpu_solve(s, states, inject, 1, TOL)
+    reps_d, f_d, _ = _gpu_solve(s, states, inject, 0, TOL)
+    assert reps_m[0]["rel_residual"] <= TOL and reps_d[0]["rel_residual"] <= TOL
+    assert reps_m[0]["n_true_residuals"] >= 1
+    res_m = _residual(s, f_m)
+    assert res_m <= TOL * (1 + 1e-6), res_m
+    inner = (slice(s.halo_depth, -s.halo_depth),) * 3
+    err = np.linalg.norm((f_m["u"] - f_d["u"])[inner]) / np.linalg.norm(f_d["u"][inner])
+    assert err < 1e-6, err
+    assert _work(reps_m) <= 1.25 * _work(reps_d) + 10, (_work(reps_m), _work(reps_d))
