@@ -7,10 +7,11 @@ log-density is checked
 * against autodiff through an FP64 unrolled CG written independently in
   torch (CPU), and against the exact gradient from sparse LU solves, at
   128^2;
-* against central finite differences (h = 1e-6 and 1e-4) of the suite's
-  own forward at 32 seeded cells, at the uniform density, on the 2048^2
-  layered deck and the 128^3 cuboid deck (fixed-length solves, see the
-  test);
+* against central finite differences (second order at h = 1e-4, fourth
+  order at h = 3e-4) of the suite's own forward at 32 seeded cells, at the
+  uniform density, on the 2048^2 layered deck and the 128^3 cuboid deck,
+  and at the layered deck's own density at 512^2 (fixed-length solves, see
+  the tests);
 * through a torch.autograd.Function wrapper (test harness only), against a
   finite difference along a random direction.
 The adjoint's formula itself is pinned on the CPU (test_adjoint_cpu.py).
@@ -107,36 +108,77 @@ FD_CASES = [
 ]
 
 
+def _fixed_k(text_of, dims, steps, base):
+    """Iterations for the fixed-length solves: twice the count that reaches a
+    relative residual of 1e-10 at the base density, plus 200."""
+    from tealeaf_amd.problem import Problem
+    s0, states0 = tdeck.parse_deck(text_of(1e-12, 20000), dims)
+    with Problem(s0, states0) as p:
+        p.set_solve(0, 1e-10)
+        p.inject(base if base is not None else np.ones(p.n), p.get("energy0"))
+        return 2 * max(r["printed_cg_iters"] for r in p.run(steps)) + 200
+
+
+class _Diff:
+    """Loss differences along one cell's log-density, summed per cell,
+    (u+ - u-)(u+ + u- - 2 target), so that they do not drown in the rounding
+    of the totals."""
+
+    def __init__(self, make, base, e0, target, steps, shape, inner):
+        self.make, self.base, self.e0, self.t = make, base, e0, target[inner]
+        self.steps, self.shape, self.inner = steps, shape, inner
+
+    def _u(self, c, x):
+        d = self.base.copy()
+        d[c] = d[c] * np.exp(x)
+        with self.make() as p:
+            p.inject(d, self.e0)
+            p.run(self.steps)
+            return p.get("u").reshape(self.shape)[self.inner]
+
+    def delta(self, c, h):           # L(+h) - L(-h)
+        up, um = self._u(c, h), self._u(c, -h)
+        return float(np.sum((up - um) * (up + um - 2.0 * self.t)))
+
+    def central(self, c, h):         # second order
+        return self.delta(c, h) / (2 * h)
+
+    def central4(self, c, h):        # fourth order: (8 D(h) - D(2h)) / 12h
+        return (8.0 * self.delta(c, h) - self.delta(c, 2 * h)) / (12 * h)
+
+
+def _cells(g, inner, hd, n_rand, n_top, seed):
+    gi = np.abs(g[inner])
+    rng = np.random.default_rng(seed)
+    rand = [tuple(int(rng.integers(hd, m + hd)) for m in gi.shape) for _ in range(n_rand)]
+    top_pool = np.argsort(gi.ravel())[-max(n_top, gi.size // 100):]
+    top = [tuple(int(i) + hd for i in np.unravel_index(f, gi.shape))
+           for f in rng.choice(top_pool, n_top, replace=False)]
+    return rand, top
+
+
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("name,deck,dims", FD_CASES, ids=[c[0] for c in FD_CASES])
 def test_adjoint_matches_central_differences(hip, name, deck, dims):
     """The gradient at the uniform density (1.0) the target was made from,
-    the first step of recovering the density from the noisy target; 32
-    seeded cells: 16 uniformly drawn, 16 drawn from the 1% of cells with the
-    largest |g|.
+    the first step of recovering the density from the noisy target; 16
+    seeded cells drawn uniformly and 16 from the 1% with the largest |g|.
 
     Central differences resolve small errors only if the forward map is
     smooth and its derivative has converged.  A solve that stops on a
     residual test is neither (the iteration count can change between s+h
     and s-h), and the derivative of CG iterates settles later than the
-    iterates.  So every solve here runs a fixed K iterations (eps 0): twice
-    the count that reaches a relative residual of 1e-10, plus 200, for the
-    adjoint's solves as for the forward ones.  The loss difference is summed
-    per cell, (u+ - u-)(u+ + u- - 2 target), so that it does not drown in
-    the rounding of the totals.  What is left is FP64 rounding in u+ - u-,
-    which grows as 1/h and limits a difference quotient at h = 1e-6 to a few
-    1e-6 max|g| even with exact solves (measured on the CPU with sparse LU
-    at 512^2 and the same conditioning; 1.4e-5 (2D) and 3.9e-5 (3D) max|g|
-    here, 100x the h = 1e-4 errors, as rounding predicts).  So h = 1e-6 is
-    held to 1e-4 max|g|, and h = 1e-4 (truncation about 1e-8) to 1e-6
-    max|g| at every cell and 1e-5 relative at the large-gradient cells."""
-    from tealeaf_amd.problem import Problem
+    iterates.  So every solve here runs a fixed K iterations (eps 0).  What
+    limits a difference quotient then is truncation, O(h^2) (5e-6 relative
+    at h = 1e-3, 4e-7 at 3e-4, measured at 2048^2), and FP64 rounding in
+    u+ - u-, which grows as 1/h (a floor of 2-5e-7 relative at h = 1e-4,
+    1e-6 at 3e-5).  The second-order quotient at h = 1e-4 sits on that
+    floor: held to 1e-6 max|g| everywhere and 2e-6 relative at the
+    large-gradient cells.  The fourth-order quotient at h = 3e-4 (truncation
+    O(h^4), rounding 1.5x the floor at that h) is held to 1e-6 relative at
+    the large-gradient cells."""
     steps = 5
-    s0, states0 = tdeck.parse_deck(deck(1e-12, 20000), dims)
-    with Problem(s0, states0) as p:
-        p.set_solve(0, 1e-10)
-        p.inject(np.ones(p.n), p.get("energy0"))
-        k = 2 * max(r["printed_cg_iters"] for r in p.run(steps)) + 200
+    k = _fixed_k(deck, dims, steps, None)
     make, s, _, e0, target = _setup(deck(0.0, k), dims, steps, 0.0, seed=77 + dims)
     shape = s.shape()
     hd = s.halo_depth
@@ -147,38 +189,75 @@ def test_adjoint_matches_central_differences(hip, name, deck, dims):
         loss, grad, reps = p.adjoint(steps, target)
     assert all(r["printed_cg_iters"] == k for r in reps)
     g = grad.reshape(shape)
-    gi = np.abs(g[inner])
-    gmax = gi.max()
-    rng = np.random.default_rng(123 + dims)
-    cells = [tuple(int(rng.integers(hd, n - hd)) for n in shape) for _ in range(16)]
-    top = np.argsort(gi.ravel())[-max(16, gi.size // 100):]
-    for f in rng.choice(top, 16, replace=False):
-        cells.append(tuple(int(i) + hd for i in np.unravel_index(f, gi.shape)))
-    t = target[inner]
-    for h, tol in ((1e-4, 1e-6), (1e-6, 1e-4)):
-        worst, worst_top = 0.0, 0.0
-        for i, c in enumerate(cells):
-            us = []
-            for sign in (1, -1):
-                d = ones.copy()
-                d[c] = np.exp(sign * h)
-                with make() as p:
-                    p.inject(d, e0)
-                    p.run(steps)
-                    us.append(p.get("u").reshape(shape)[inner])
-            fd = float(np.sum((us[0] - us[1]) * (us[0] + us[1] - 2.0 * t))) / (2 * h)
-            err = abs(fd - g[c]) / gmax
-            rel = abs(fd - g[c]) / abs(g[c])
-            worst = max(worst, err)
-            if i >= 16:
-                worst_top = max(worst_top, rel)
-            print(f"{name} K={k} h={h:g} cell {c}: adjoint {g[c]:.10e} fd {fd:.10e} "
-                  f"|fd-g|/max|g| {err:.2e} rel {rel:.2e}", flush=True)
-        print(f"{name} h={h:g}: worst |fd-g|/max|g| {worst:.2e}, worst relative at the "
-              f"large-gradient cells {worst_top:.2e}", flush=True)
-        assert worst < tol, (h, worst)
-        if h == 1e-4:
-            assert worst_top < 1e-5, worst_top
+    gmax = np.abs(g[inner]).max()
+    rand, top = _cells(g, inner, hd, 16, 16, 123 + dims)
+    D = _Diff(make, ones, e0, target, steps, shape, inner)
+    worst_abs = worst_rel2 = worst_rel4 = 0.0
+    for c in rand + top:
+        fd = D.central(c, 1e-4)
+        worst_abs = max(worst_abs, abs(fd - g[c]) / gmax)
+        line = f"{name} K={k} cell {c}: adjoint {g[c]:.10e} fd2 {fd:.10e}"
+        if c in top:
+            worst_rel2 = max(worst_rel2, abs(fd - g[c]) / abs(g[c]))
+            fd4 = D.central4(c, 3e-4)
+            worst_rel4 = max(worst_rel4, abs(fd4 - g[c]) / abs(g[c]))
+            line += f" fd4 {fd4:.10e} rel2 {abs(fd - g[c]) / abs(g[c]):.2e} " \
+                    f"rel4 {abs(fd4 - g[c]) / abs(g[c]):.2e}"
+        print(line, flush=True)
+    print(f"{name}: worst |fd2-g|/max|g| {worst_abs:.2e}, large-gradient cells: worst relative "
+          f"fd2 {worst_rel2:.2e}, fd4 {worst_rel4:.2e}", flush=True)
+    assert worst_abs < 1e-6, worst_abs
+    assert worst_rel2 < 2e-6, worst_rel2
+    assert worst_rel4 < 1e-6, worst_rel4
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("coefficient", [1, 2])
+def test_adjoint_central_differences_layered_density(hip, coefficient):
+    """The gradient at the layered deck's own (non-uniform) density, 512^2,
+    dt 0.064 (the conditioning of 2048^2 at dt 0.004), fixed-length solves:
+    fourth-order central differences at h = 3e-4 at 8 uniformly drawn cells
+    (within 1e-6 max|g|) and 8 of the largest-|g| cells (within 1e-6
+    relative)."""
+    steps = 5
+
+    def deck(eps, it):
+        text = tdeck.layered_2d(512, steps, solver=tdeck.CG, eps=eps, max_iters=it, dt=0.064)
+        if coefficient == 2:
+            text = text.replace("*endtea", "coefficient_inverse_density\n*endtea")
+        return text
+
+    from tealeaf_amd.problem import Problem
+    s0, states0 = tdeck.parse_deck(deck(1e-12, 20000), 2)
+    with Problem(s0, states0) as p:
+        dens0 = p.get("density")
+    k = _fixed_k(deck, 2, steps, dens0)
+    make, s, dens, e0, target = _setup(deck(0.0, k), 2, steps, 0.0, seed=90 + coefficient)
+    shape = s.shape()
+    hd = s.halo_depth
+    inner = tuple(slice(hd, n - hd) for n in shape)
+    base = dens.reshape(shape)
+    assert np.ptp(base[inner]) > 0                       # non-uniform
+    with make() as p:
+        p.inject(base, e0)
+        loss, grad, reps = p.adjoint(steps, target)
+    assert all(r["printed_cg_iters"] == k for r in reps)
+    g = grad.reshape(shape)
+    gmax = np.abs(g[inner]).max()
+    rand, top = _cells(g, inner, hd, 8, 8, 300 + coefficient)
+    D = _Diff(make, base, e0, target, steps, shape, inner)
+    worst_abs = worst_rel = 0.0
+    for c in rand + top:
+        fd4 = D.central4(c, 3e-4)
+        worst_abs = max(worst_abs, abs(fd4 - g[c]) / gmax)
+        if c in top:
+            worst_rel = max(worst_rel, abs(fd4 - g[c]) / abs(g[c]))
+        print(f"layered coefficient {coefficient} K={k} cell {c}: adjoint {g[c]:.10e} "
+              f"fd4 {fd4:.10e}", flush=True)
+    print(f"layered coefficient {coefficient}: worst |fd4-g|/max|g| {worst_abs:.2e}, worst "
+          f"relative at the large-gradient cells {worst_rel:.2e}", flush=True)
+    assert worst_abs < 1e-6, worst_abs
+    assert worst_rel < 1e-6, worst_rel
 
 
 @pytest.mark.parametrize("coefficient", [1, 2])
