@@ -10,11 +10,11 @@ all three; the three chunks stay resident in HBM throughout.
 value = solver iterations per second over the timed steps, counting
 iterations as the reference host prints them (CG + Chebyshev + PPCG lines).
 The JSON line also carries the per-solver rates, the roofline of the
-dominant kernel (of calc_pw, calc_ur, the temporally blocked PPCG loop and
-the temporally blocked Chebyshev loop, timed live with HIP events on the
-suite's stream in every leg, the one with the most total time; the fused
-PPCG loop's own roofline beside it), and the reference omp3 CPU path timed
-on this host on a bounded sample of the same deck.
+dominant kernel (of calc_pw, timed in every leg, and the temporally
+blocked PPCG and Chebyshev loops, timed in theirs -- live, with HIP events
+on the suite's stream -- the one with the most total time; the fused PPCG
+loop's own roofline beside it), and the reference omp3 CPU path timed on
+this host on a bounded sample of the same deck.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 4096]
 """
@@ -119,9 +119,12 @@ def gpu_run(args):
     total_time = 0.0
     total_iters = 0
     kernel = {}
-    # the kernels that hold most of the time, timed in every leg
-    timed = ["tlhip2d_cg_calc_pw", "tlhip2d_cg_calc_ur", dominant(args), "tlhip2d_cheby_iterations"]
+    # calc_pw (every leg) and the leg's own sweep; each timed launch adds two
+    # event records to the stream, so the rest (calc_ur: 19% of the kernel
+    # time, never the largest) is left untimed
+    own = {"cg": [], "cheby": ["tlhip2d_cheby_iterations"], "ppcg": [dominant(args)]}
     for s in solvers:
+        timed = ["tlhip2d_cg_calc_pw"] + own[s]
         L.tlhip_synchronize()
         torch.cuda.synchronize()
         L.tlhip_kernel_timer_start(",".join(timed).encode())
@@ -296,7 +299,7 @@ def profile_run(args):
     hot kernel on synthetic fields -- 2D at --cells^2 (the fused PPCG and
     Chebyshev loops, calc_w, calc_pw, calc_ur, calc_p, the 2-norm,
     field_summary, cg_init and the single-step PPCG sweep; the FP32 fused
-    loop), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
+    PPCG and Chebyshev loops), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
     2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
     2D PPCG solve at 2048^2 for the FP32 CG kernels, and a 1 GiB device copy
     (8 B read + 8 B written per element) whose FETCH_SIZE / WRITE_SIZE
@@ -356,6 +359,11 @@ def profile_run(args):
                                             f32["r"].ptr, f32["r2"].ptr, f32["u"].ptr,
                                             f32["u2"].ptr, f32["kx"].ptr, f32["ky"].ptr, 0.3,
                                             f32["p"].ptr, f32["w"].ptr)
+        # 10 FP32 Chebyshev iterations (u -> u2, p -> sd2; u0 = sd)
+        L.tlhip2d_cheby_iterations_f32(x, y, hd, P, 10, al.ctypes.data, be.ctypes.data,
+                                       nb.ctypes.data, f32["u"].ptr, f32["u2"].ptr, f32["sd"].ptr,
+                                       f32["p"].ptr, f32["sd2"].ptr, f32["r2"].ptr, f32["kx"].ptr,
+                                       f32["ky"].ptr, None)
         L.tlhip2d_ppcg_inner_iteration(x, y, hd, 0.7, 0.2, f64["u"].ptr, f64["r"].ptr, f64["kx"].ptr,
                                        f64["ky"].ptr, f64["sd"].ptr, f64["sd2"].ptr)
         L.tlhip2d_cg_calc_w(x, y, hd, C.addressof(pw), f64["p"].ptr, f64["w"].ptr, f64["kx"].ptr,
@@ -569,16 +577,18 @@ TRAFFIC_JSON = "profiles/hbm_traffic_2d4096_3d256.json"
 # order: (entry point, HIP kernel name fragments, compulsory bytes per
 # interior cell = each array the call reads once + each it writes once).
 _TRAFFIC_2D = [
-    ("tlhip2d_ppcg_inner_iterations (from r)", ["k_ppcg_ring2d<double, 10, 2>"], 72,
+    ("tlhip2d_ppcg_inner_iterations (from r)", ["k_ppcg_ring2d<double, 10, 2,"], 72,
      "cg_calc_ur + ppcg_init + 10 inner steps: reads r, w, u, p, kx, ky; writes sd, r, u"),
-    ("tlhip2d_ppcg_inner_iterations", ["k_ppcg_ring2d<double, 10, 0>"], 64,
+    ("tlhip2d_ppcg_inner_iterations", ["k_ppcg_ring2d<double, 10, 0,"], 64,
      "10 inner steps from sd: reads sd, kx, ky, r, u; writes sd, r, u"),
     ("tlhip2d_cg_calc_pw", ["k_calc_pw<double, true>"], 48,
      "calc_p fused into calc_w: reads p, r, kx, ky; writes p', w"),
     ("tlhip2d_cheby_iterations", ["k_cheby_fused2d<double, 10"], 64,
      "10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
-    ("tlhip2d_ppcg_inner_iterations_f32 (from r)", ["k_ppcg_ring2d<float, 10, 2>"], 36,
+    ("tlhip2d_ppcg_inner_iterations_f32 (from r)", ["k_ppcg_ring2d<float, 10, 2, false>"], 36,
      "FP32 fields, cg_calc_ur + ppcg_init + 10 inner steps"),
+    ("tlhip2d_cheby_iterations_f32", ["k_cheby_fused2d<float, 10"], 32,
+     "FP32 fields, 10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
     ("tlhip2d_ppcg_inner_iteration", ["k_stencil2d<2, tl::OpPpcgInner>"], 64,
      "one unfused inner step: reads sd, kx, ky, r, u; writes sd', r, u"),
     ("tlhip2d_cg_calc_w", ["k_stencil2d<2, tl::OpCalcW>"], 32, "reads p, kx, ky; writes w"),
@@ -608,8 +618,8 @@ _TRAFFIC_3D = [
 # FP32 kernels of the FP32-refined PPCG timestep at 2048^2 (many launches
 # each): name fragment -> (call, bytes per cell)
 _TRAFFIC_MIXED = [
-    ("tlhip2d_ppcg_inner_iterations_f32 (from r), 2048^2", "k_ppcg_ring2d<float, 10, 2>", 36,
-     "FP32 cg_calc_ur + ppcg_init + 10 inner steps"),
+    ("tlhip2d_ppcg_inner_iterations_f32_norm (from r), 2048^2", "k_ppcg_ring2d<float, 10, 2, true>",
+     36, "FP32 cg_calc_ur + ppcg_init + 10 inner steps + the 2-norm of r"),
     ("FP32 calc_pw", "k_calc_pw_rows<float, true>", 24, "reads p, r, kx, ky; writes p', w (FP32)"),
     ("FP32 calc_w", "k_calc_pw_rows<float, false>", 16, "reads p, kx, ky; writes w (FP32)"),
     ("FP32 calc_ur", "k_groups<(anonymous namespace)::OpUr32>", 24,
