@@ -264,10 +264,11 @@ int tlhip2d_cheby_iterations_f32(int x, int y, int halo_depth, int pitch,
  * once, from the kernel's occupancy and the CU count) and prefetch distance
  * in rows (0-3; < 0: default 1; the Chebyshev loop uses 0 or 1). */
 int tlhip_set_fused_tuning(int tile_rows, int prefetch);
-/* Which kernel the FP64 fused PPCG loop launches: 0 (default) keeps each
- * row's diagonal, kx and ky in a per-wave LDS ring (prefetch fixed at one
- * row); 1 keeps them in register windows (the prefetch setting applies).
- * Results are bit-identical.  The FP32 loop always runs the ring kernel. */
+/* Which kernel the FP64 fused PPCG and Chebyshev loops launch: 0 (default)
+ * keeps each row's diagonal, kx and ky in a per-wave LDS ring (prefetch
+ * fixed at one row); 1 keeps them in register windows (the prefetch setting
+ * applies).  Results are bit-identical.  The FP32 loops always run the ring
+ * kernel. */
 int tlhip_set_fused_variant(int variant);
 /* Tuning of the LDS-ring PPCG kernel: its tiles (one wave each) number
  * about fill x the waves the device holds at once (<= 0: default 0.95,

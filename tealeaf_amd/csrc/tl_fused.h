@@ -61,7 +61,11 @@ extern int g_fused_variant;
 extern double g_ring_fill;
 
 // The PPCG loop with the coefficient rows in per-wave LDS rings
-// (tl_ppcg_ring.hip); count 1..kMaxFused, mode 0..2 as FusedArgs.
+// (tl_ppcg_ring.hip); count 1..kMaxFused, mode 0..2 as FusedArgs.  Mode 3
+// runs the Chebyshev loop the same way (2d/drivers/cheby_driver.c turns:
+// mirror of u, r = u0 - A u, p = alpha_k p + beta_k r, u += p) with the
+// fields in the PPCG roles: sd_in/sd_out = u, r_in = u0, u_in/u_out = p,
+// r_out = r.
 template <class T>
 int launch_ppcg_ring(int count, int mode, const FusedArgs<T>& a);
 

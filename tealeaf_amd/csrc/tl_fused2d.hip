@@ -1043,6 +1043,12 @@ int cheby_iterations(const char* what, int x, int y, int hd, int pitch, int coun
                            tl_stream(), g, y, rg, tl_gate());
         TL_TRY(tl_check_launch(what));
     }
+    if (g_fused_variant == 0 || std::is_same_v<T, float>) {
+        // the LDS-ring kernel, Chebyshev mode: u in sd's role, u0 in r's, p in u's
+        FusedArgs<T> a{g, alphas, betas, 1.0, 0.0, nullptr, u_in, u_out, u0, r, p_in, p_out,
+                       kx, ky, nullptr, nullptr, what, rrn};
+        return launch_ppcg_ring<T>(count, 3, a);
+    }
     const bool norm = rrn != nullptr;
     switch (count) {
 #define TL_CHEBY_CASE(S)                                                                      \

@@ -25,6 +25,12 @@
 //    whose window holds the first or last interior column, take the body
 //    with the mirror tests; the others the body without.
 //
+// Mode 3 runs S turns of the Chebyshev loop (2d/drivers/cheby_driver.c: a
+// depth-1 mirror of u, then cheby_iterate and cheby_calc_u,
+// 2d/c_kernels/omp3/cheby.c:8-24,57-85) the same way: u takes sd's place
+// (three-row windows per level), and the per-level hand-off carries u0 and p
+// of the row instead of r and u.
+//
 // The per-cell expressions are those of the reference, in its order and
 // association (the diagonal included), so the FP64 result is bit-for-bit
 // that of S separate steps; HBM traffic for S steps is that of one (read
@@ -171,8 +177,9 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
     using RG = RingGeom<S>;
     constexpr int R = RG::R;
     __shared__ __attribute__((aligned(16))) unsigned char s_ring[RG::kWave];
-    constexpr bool FROM_R = MODE >= 1;
+    constexpr bool FROM_R = MODE == 1 || MODE == 2;
     constexpr bool UR = MODE == 2;
+    constexpr bool CHEBY = MODE == 3;
     constexpr int N = Lane16<T>::N;
     using C = Cells<T>;
     if (gate.closed()) return;
@@ -396,21 +403,29 @@ k_ppcg_ring2d(PitchGrid g, StripTile t, PpcgCoefs<T, S> cf, T theta, T alpha_arg
                 if (dy1 == k - 1) upv = cen;
                 if (dy0 == k - 1) dnv = cen;
             }
+            // PPCG: r and u of the row from level k-1; Chebyshev: u0 and p
             const C rin = k == 1 ? r0 : rc[P][k - 1];
             const C uin = k == 1 ? u0 : uc[P][k - 1];
-            // SMVP (2d/shared.h:50-54) and the ppcg.c:46-59 updates, same order
+            // SMVP (2d/shared.h:50-54) and the ppcg.c:46-59 (Chebyshev:
+            // cheby.c:76-84 and cheby_calc_u) updates, same order
             const C av = apply_a_ring<XG>(cen, upv, dnv, co.kx, co.kxe, kyu, co.ky, co.d, c, xin0,
                                           xin1, lm, rm);
+            // stored as r, u, sd (Chebyshev: r, p, u)
             C rn, un, sn;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 rn.v[i] = rin.v[i] - av.v[i];
-                un.v[i] = uin.v[i] + cen.v[i];
-                sn.v[i] = mad(co.al, cen.v[i], co.be * rn.v[i]);
+                if constexpr (CHEBY) {
+                    un.v[i] = mad(co.al, uin.v[i], co.be * rn.v[i]);    // p
+                    sn.v[i] = cen.v[i] + un.v[i];                       // u
+                } else {
+                    un.v[i] = uin.v[i] + cen.v[i];
+                    sn.v[i] = mad(co.al, cen.v[i], co.be * rn.v[i]);
+                }
             }
             if (k < S) {
                 sdw[k][P2] = sn;
-                rc[P1][k] = rn;
+                rc[P1][k] = CHEBY ? rin : rn;
                 uc[P1][k] = un;
             } else if (any_out && (unsigned)dst < (unsigned)(j1 - j0)) {
                 const int ro = rho * row_bytes;          // rho is an interior row here
@@ -588,7 +603,8 @@ int launch_ppcg_ring(int count, int mode, const FusedArgs<T>& a)
     switch (mode) {
     case 0: return dispatch_ring<T, 0>(count, a);
     case 1: return dispatch_ring<T, 1>(count, a);
-    default: return dispatch_ring<T, 2>(count, a);
+    case 2: return dispatch_ring<T, 2>(count, a);
+    default: return dispatch_ring<T, 3>(count, a);
     }
 }
 
