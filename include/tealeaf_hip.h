@@ -462,6 +462,14 @@ int tlhip_set_graphs(int on);
  * reference deck's 10 inner steps in one launch (FP64, and FP32 in the
  * mixed-precision solve). */
 int tlhip_set_ppcg_fusion(int steps);
+/* Whether the 2D driver's fused PPCG launches form the 2-norm of r that
+ * ends each outer iteration (2d/drivers/ppcg_driver.c:121-131) in the last
+ * launch instead of a separate 2-norm pass (default 1; FP64 with the
+ * LDS-ring kernel, tlhip_set_fused_variant(0)).  The sum then runs in
+ * another order, so rrn, and through beta the fields, differ in the last
+ * bits from the unfused path's; 0 restores bit-identity with it.  No
+ * counterpart in the reference. */
+int tlhip_set_ppcg_fused_norm(int on);
 /* Chebyshev iterations per fused launch used by the 2D driver with the
  * reference stop test (0 or 1 = one tlhip2d_cheby_iterate per iteration).
  * Default 10; results are identical either way. */

@@ -45,6 +45,7 @@ enum { F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD };
 // pass over memory either way; the deeper kernel holds 1 wave per SIMD
 // instead of 2); in FP32 at 8192^2, 0.94 ms against 0.50 + 0.56 ms.
 int g_ppcg_fusion = 10;
+int g_ppcg_fused_norm = 1;   // tlhip_set_ppcg_fused_norm
 // Chebyshev iterations per fused launch (2D, FP64, reference stop test).
 int g_cheby_fusion = 10;
 int g_cg_fusion = 1;     // fuse calc_p into the next calc_w (2D, tlhip2d_cg_calc_pw)
@@ -427,13 +428,15 @@ struct tlhip_problem {
         if (can_defer_p()) TRY(calc_w_resident());
         else TRY(tl::loop2d::calc_w(x, y, hd, p, w, kx, ky));
         int done = 0;
+        unsigned norm_nb = 0;
         while (done < s.ppcg_inner_steps) {
             const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
-            const bool first = done == 0;
+            const bool first = done == 0, last = done + cnt == s.ppcg_inner_steps;
             TRY(tl::loop2d::ppcg_inner(x, y, hd, cnt, &cheby_alphas[done], &cheby_betas[done],
                                        theta, first, neighbours, first ? nullptr : sd, sd_alt, r,
                                        r_alt, u, u_alt, kx, ky, first ? p : nullptr,
-                                       first ? w : nullptr));
+                                       first ? w : nullptr,
+                                       last && fused_norm() ? &norm_nb : nullptr));
             std::swap(sd, sd_alt);
             std::swap(r, r_alt);
             std::swap(u, u_alt);
@@ -441,7 +444,10 @@ struct tlhip_problem {
         }
         reset_fields();
         fields[F_P] = true;
-        TRY(tl::loop2d::norm(x, y, hd, r, rule));
+        if (norm_nb > 0)      // the last launch formed r.r (ppcg_driver.c:121-131)
+            TRY(tl::loop::finish_beta(norm_nb, rule, "tlhip2d_calculate_2norm"));
+        else
+            TRY(tl::loop2d::norm(x, y, hd, r, rule));
         if (can_defer_p()) {
             pending_p = true;
             return 0;            // p is read pointwise or mirrored: halo at the end
@@ -833,6 +839,10 @@ struct tlhip_problem {
     // -------------------------------------------------------- PPCG
     // The temporally blocked inner loop needs the 16-byte row layout and
     // all faces external (always true for this single-chunk driver).
+    // The 2-norm of r formed in the last fused launch of a PPCG step (2D
+    // FP64 ring kernel; tlhip_set_ppcg_fused_norm)
+    bool fused_norm() const { return g_ppcg_fused_norm != 0 && tl::loop2d::ring_kernel(); }
+
     bool fused_ppcg() const
     {
         return dims == 2 && g_ppcg_fusion > 1 && hd >= 2 && hd % 2 == 0 && x % 2 == 0 && r_alt;
@@ -844,6 +854,7 @@ struct tlhip_problem {
         TRY(calc_w_pending(&pw));
         const double alpha = *rro / pw;
         double rrn = 0.0;
+        bool norm_done = false;    // the fused launch formed rrn
 
         if (fused_ppcg() && s.ppcg_inner_steps > 0) {
             // calc_ur, ppcg_init and the inner loop, several steps per pass;
@@ -852,18 +863,20 @@ struct tlhip_problem {
             int done = 0;
             while (done < s.ppcg_inner_steps) {
                 const int cnt = std::min(g_ppcg_fusion, s.ppcg_inner_steps - done);
-                const bool first = done == 0;
-                TRY(tlhip2d_ppcg_inner_iterations(x, y, hd, cnt, &cheby_alphas[done],
-                                                  &cheby_betas[done], theta, first, neighbours,
-                                                  first ? nullptr : sd, sd_alt, r, r_alt, u,
-                                                  u_alt, kx, ky, alpha, first ? p : nullptr,
-                                                  first ? w : nullptr));
+                const bool first = done == 0, last = done + cnt == s.ppcg_inner_steps;
+                TRY(tl::loop2d::ppcg_inner_norm(x, y, hd, cnt, &cheby_alphas[done],
+                                                &cheby_betas[done], theta, first, neighbours,
+                                                first ? nullptr : sd, sd_alt, r, r_alt, u, u_alt,
+                                                kx, ky, alpha, first ? p : nullptr,
+                                                first ? w : nullptr,
+                                                last && fused_norm() ? &rrn : nullptr));
                 std::swap(sd, sd_alt);
                 std::swap(r, r_alt);
                 std::swap(u, u_alt);
                 done += cnt;
                 rep->n_ppcg_inner += cnt;
             }
+            norm_done = fused_norm();
         } else if (fused_ppcg()) {
             TRY(calc_ur(alpha, &rrn));
             TRY(tlhip2d_ppcg_init(x, y, hd, theta, r, sd));
@@ -890,8 +903,10 @@ struct tlhip_problem {
         reset_fields();
         fields[F_P] = true;
 
-        rrn = 0.0;
-        TRY(norm2(r, &rrn));
+        if (!norm_done) {
+            rrn = 0.0;
+            TRY(norm2(r, &rrn));
+        }
         const double beta = rrn / *rro;
         TRY(calc_p_deferred(beta));
         *error = rrn;
@@ -1499,6 +1514,12 @@ extern "C" int tlhip_set_ppcg_fusion(int steps)
 {
     if (steps < 0 || steps > 10) return tl_fail(TLHIP_ERR_ARG, "tlhip_set_ppcg_fusion: 0..10");
     g_ppcg_fusion = steps;
+    return 0;
+}
+
+extern "C" int tlhip_set_ppcg_fused_norm(int on)
+{
+    g_ppcg_fused_norm = on ? 1 : 0;
     return 0;
 }
 

@@ -44,8 +44,11 @@ struct FusedArgs {
     const T* w;
     const char* what;
     // Non-null: the launch also forms the squared 2-norm of the r it writes
-    // (the interior), fetched to *rrn (FP32 ring kernel only).
+    // (the interior): fetched to *rrn, or (norm_nb) left as per-block
+    // partials in the reduction slots for the caller's finisher, their count
+    // in *norm_nb.  Ring kernel only.
     double* rrn = nullptr;
+    unsigned* norm_nb = nullptr;
 };
 
 // Compute units of the current device (cached; the runtime uses one device).

@@ -906,12 +906,12 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
                      const int* neighbours, const T* sd_in, T* sd_out, const T* r_in, T* r_out,
                      const T* u_in, T* u_out, const T* kx, const T* ky, double alpha_ur,
                      const T* p, const T* w, const double* alpha_dev = nullptr,
-                     double* rrn = nullptr, bool copy_ring = true)
+                     double* rrn = nullptr, bool copy_ring = true, unsigned* norm_nb = nullptr)
 {
     constexpr int N = Lane16<T>::N;
     TL_TRY(tl_ensure_init());
-    if (rrn && !std::is_same_v<T, float>)
-        return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm is FP32 only", what);
+    if ((rrn || norm_nb) && std::is_same_v<T, double> && g_fused_variant != 0)
+        return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm needs the ring kernel", what);
     if (hd < 1 || x < 2 * hd + 1 || y < 2 * hd + 1 || pitch < x || pitch % N ||
         (long)pitch * y > (1L << 31) - 1)
         return tl_fail(TLHIP_ERR_ARG,
@@ -948,7 +948,7 @@ int inner_iterations(const char* what, int x, int y, int hd, int pitch, int coun
         TL_TRY(tl_check_launch(what));
     }
     const FusedArgs<T> a{g, alphas, betas, theta, alpha_ur, alpha_dev, sd_in, sd_out, r_in, r_out,
-                         u_in, u_out, kx, ky, p, w, what, rrn};
+                         u_in, u_out, kx, ky, p, w, what, rrn, norm_nb};
     if (!init_from_r) return dispatch_fused<T, 0>(count, a);
     if (!p) return dispatch_fused<T, 1>(count, a);
     return dispatch_fused<T, 2>(count, a);
@@ -1172,12 +1172,27 @@ int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const doub
                double theta, int init_from_r, const int* neighbours, const double* sd_in,
                double* sd_out, const double* r_in, double* r_out, const double* u_in,
                double* u_out, const double* kx, const double* ky, const double* p,
-               const double* w)
+               const double* w, unsigned* norm_nb)
 {
     return inner_iterations<double>("tlhip2d_ppcg_inner_iterations", x, y, hd, x, count, alphas,
                                     betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
                                     r_out, u_in, u_out, kx, ky, 0.0, p, w,
-                                    p ? loop::alpha() : nullptr);
+                                    p ? loop::alpha() : nullptr, nullptr, true, norm_nb);
+}
+
+bool ring_kernel() { return g_fused_variant == 0; }
+
+// The same launch on the host-driven path, with the 2-norm of r fetched to
+// *rrn when rrn is not null.
+int ppcg_inner_norm(int x, int y, int hd, int count, const double* alphas, const double* betas,
+                    double theta, int init_from_r, const int* neighbours, const double* sd_in,
+                    double* sd_out, const double* r_in, double* r_out, const double* u_in,
+                    double* u_out, const double* kx, const double* ky, double alpha,
+                    const double* p, const double* w, double* rrn)
+{
+    return inner_iterations<double>("tlhip2d_ppcg_inner_iterations", x, y, hd, x, count, alphas,
+                                    betas, theta, init_from_r, neighbours, sd_in, sd_out, r_in,
+                                    r_out, u_in, u_out, kx, ky, alpha, p, w, nullptr, rrn);
 }
 
 // calc_p with the loop's beta fused into calc_w, pw finished into alpha.

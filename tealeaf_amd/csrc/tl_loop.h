@@ -104,11 +104,24 @@ int jacobi(int x, int y, int hd, const double* kx, const double* ky, const doubl
            const double* u_in, double* u_out, loop::ErrMode mode, const loop::StopRule& rule);
 // The 2-norm of buf added to the Jacobi error.
 int norm_err(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
+// The fused PPCG launch (calc_ur's alpha from the loop); with norm_nb, the
+// launch also leaves the 2-norm of the r it writes as per-block partials,
+// their count in *norm_nb, for finish_beta.
 int ppcg_inner(int x, int y, int hd, int count, const double* alphas, const double* betas,
                double theta, int init_from_r, const int* neighbours, const double* sd_in,
                double* sd_out, const double* r_in, double* r_out, const double* u_in,
                double* u_out, const double* kx, const double* ky, const double* p,
-               const double* w);
+               const double* w, unsigned* norm_nb = nullptr);
+// Whether the FP64 fused PPCG loop runs the LDS-ring kernel (the one that
+// can form the 2-norm in-launch; tlhip_set_fused_variant).
+bool ring_kernel();
+// The same on the host-driven path (alpha from the host), with the 2-norm
+// of r fetched to *rrn when rrn is not null.
+int ppcg_inner_norm(int x, int y, int hd, int count, const double* alphas, const double* betas,
+                    double theta, int init_from_r, const int* neighbours, const double* sd_in,
+                    double* sd_out, const double* r_in, double* r_out, const double* u_in,
+                    double* u_out, const double* kx, const double* ky, double alpha,
+                    const double* p, const double* w, double* rrn);
 }  // namespace loop2d
 
 namespace loop3d {

@@ -559,6 +559,10 @@ int launch_ring(const FusedArgs<T>& a)
     tl_timer_end(slot);
     TL_TRY(tl_check_launch(a.what));
     if constexpr (NORM) {
+        if (a.norm_nb) {
+            *a.norm_nb = nb;
+            return 0;
+        }
         TL_TRY(tl_finish_reduce(1, nb, a.what));
         return tl_fetch(1, a.rrn, a.what);
     }
@@ -583,13 +587,13 @@ int dispatch_ring_n(int count, const FusedArgs<T>& a)
     }
 }
 
-// The norm-forming kernels are built for FP32, the mixed-precision loop's.
 template <class T, int MODE>
 int dispatch_ring(int count, const FusedArgs<T>& a)
 {
-    if (!a.rrn) return dispatch_ring_n<T, MODE, false>(count, a);
-    if constexpr (std::is_same_v<T, float>) return dispatch_ring_n<T, MODE, true>(count, a);
-    return tl_fail(TLHIP_ERR_ARG, "%s: the fused 2-norm is FP32 only", a.what);
+    if (!a.rrn && !a.norm_nb) return dispatch_ring_n<T, MODE, false>(count, a);
+    if constexpr (MODE == 0 || MODE == 2 || std::is_same_v<T, float>)
+        return dispatch_ring_n<T, MODE, true>(count, a);
+    return tl_fail(TLHIP_ERR_ARG, "%s: no fused 2-norm for this launch", a.what);
 }
 
 }  // namespace

@@ -151,15 +151,19 @@ def test_ppcg_4096_fused_and_unfused(hip):
     L = lib()
     out = {}
     try:
-        for fusion in (10, 1):
+        for fusion in (10, 1, "10-norm"):
             print(f"[fullsize] ppcg 4096 fusion {fusion}", flush=True)
-            L.tlhip_set_ppcg_fusion(fusion)
+            # the fused launches with the 2-norm in its own pass, as the
+            # unfused path; then (the default) formed in the last launch
+            L.tlhip_set_ppcg_fused_norm(1 if fusion == "10-norm" else 0)
+            L.tlhip_set_ppcg_fusion(1 if fusion == 1 else 10)
             with Problem.from_deck(fma["deck"], 2) as prob:
                 steps = prob.run()
                 out[fusion] = (steps, prob.field_summary(), prob.get("u"),
                                [prob.get(k) for k in ("density", "energy0", "u")])
     finally:
         L.tlhip_set_ppcg_fusion(10)
+        L.tlhip_set_ppcg_fused_norm(1)
     steps, summary, u, fields = out[10]
     for want in ([rec, fma] if _fma_agrees(rec, fma) else [rec]):
         _check_iters(steps, want["steps"], s.solver)
@@ -175,3 +179,9 @@ def test_ppcg_4096_fused_and_unfused(hip):
         [printed_iters(b, s.solver) for b in steps1]
     assert summary == summary1
     assert np.array_equal(u, u1)
+    # the 2-norm formed in the last fused launch: counts +-1 of omp3 and the
+    # known answer, the summary within omp3's own PPCG spread
+    steps_n, summary_n, _, fields_n = out["10-norm"]
+    _check_iters(steps_n, rec["steps"], s.solver)
+    assert reference_check_passes(summary_n[3], want), (summary_n[3], want)
+    assert rel(reference_order_summary(s, *fields_n)[3], rec["summary"][3]) < 1e-10

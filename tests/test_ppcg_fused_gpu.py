@@ -137,21 +137,32 @@ def test_fused_rejects_bad_layout(hip):
         b.free()
 
 
-@pytest.mark.parametrize("inner", [1, 7, 10, 12])
-def test_driver_fusion_is_bit_identical(hip, inner):
-    """The native driver with the fused inner loop gives the same bits as
-    one launch per inner step, for inner-step counts that split unevenly."""
+def _run_fusions(hip, text, fusions, fused_norm):
     from tealeaf_amd.problem import Problem
-    text = tdeck.layered_2d(90, 2, solver=tdeck.PPCG, presteps=15, inner=inner, max_iters=2000)
     res = {}
-    for fusion in (1, 10, 5, 3):
-        hip.tlhip_set_ppcg_fusion(fusion)
-        try:
+    hip.tlhip_set_ppcg_fused_norm(int(fused_norm))
+    try:
+        for fusion in fusions:
+            hip.tlhip_set_ppcg_fusion(fusion)
             with Problem.from_deck(text, 2) as prob:
                 steps = prob.run()
                 res[fusion] = (steps, prob.get("u"), prob.get("energy"), prob.field_summary())
-        finally:
-            hip.tlhip_set_ppcg_fusion(10)
+    finally:
+        hip.tlhip_set_ppcg_fusion(10)
+        hip.tlhip_set_ppcg_fused_norm(1)
+    return res
+
+
+@pytest.mark.parametrize("inner", [1, 7, 10, 12])
+def test_driver_fusion_is_bit_identical(hip, inner):
+    """The native driver with the fused inner loop gives the same bits as
+    one launch per inner step, for inner-step counts that split unevenly
+    (the 2-norm of r in its own pass, as the unfused path does); with the
+    2-norm formed in the last fused launch (the default), rrn is summed in
+    another order: the same iteration counts, fields within 1e-13 relative,
+    and the summary within 1e-13."""
+    text = tdeck.layered_2d(90, 2, solver=tdeck.PPCG, presteps=15, inner=inner, max_iters=2000)
+    res = _run_fusions(hip, text, (1, 10, 5, 3), False)
     base = res[1]
     for f in (3, 5, 10):
         steps, u, e, summ = res[f]
@@ -160,6 +171,14 @@ def test_driver_fusion_is_bit_identical(hip, inner):
         np.testing.assert_array_equal(u, base[1])
         np.testing.assert_array_equal(e, base[2])
         assert summ == base[3]
+    normed = _run_fusions(hip, text, (10, 3), True)
+    for f in (10, 3):
+        steps, u, e, summ = normed[f]
+        assert [s["n_ppcg_outer"] for s in steps] == [s["n_ppcg_outer"] for s in base[0]]
+        assert np.max(np.abs(u - base[1])) <= 1e-13 * np.max(np.abs(base[1]))
+        assert np.max(np.abs(e - base[2])) <= 1e-13 * np.max(np.abs(base[2]))
+        for a, b in zip(summ, base[3]):
+            assert abs(a - b) <= 1e-13 * abs(b), (summ, base[3])
 
 
 def test_set_ppcg_fusion_range(hip):
