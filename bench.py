@@ -47,6 +47,7 @@ BYTES_PER_CELL = {
     "tlhip2d_cheby_iterate": 64,          # read u, u0, p, kx, ky; write u', p, r
     "tlhip2d_cheby_iterations": 64,       # the same, once per launch of up to 10 iterations
     "tlhip3d_ppcg_inner_iteration": 72,   # read sd, kx, ky, kz, r, u; write r, u, sd'
+    "tlhip3d_ppcg_inner_iterations": 72,  # the same arrays once for two inner steps
 }
 
 
@@ -300,7 +301,7 @@ def profile_run(args):
     Chebyshev loops, calc_w, calc_pw, calc_ur, calc_p, the 2-norm,
     field_summary, cg_init and the single-step PPCG sweep; the FP32 fused
     PPCG and Chebyshev loops), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
-    2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
+    two-step PPCG sweep, the 2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
     2D PPCG solve at 2048^2 for the FP32 CG kernels, and a 1 GiB device copy
     (8 B read + 8 B written per element) whose FETCH_SIZE / WRITE_SIZE
     calibrate the counters."""
@@ -384,12 +385,14 @@ def profile_run(args):
     m = 256
     X3 = m + 2 * hd
     N3 = X3 ** 3
-    g = {k: DeviceArray.from_host(rng.uniform(-1, 1, N3)) for k in ("u", "r", "sd", "sd2", "p", "w")}
+    g = {k: DeviceArray.from_host(rng.uniform(-1, 1, N3))
+         for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
     for k in ("kx", "ky", "kz"):
         g[k] = DeviceArray.from_host(rng.uniform(0.02, 0.1, N3))
     g["rho"] = DeviceArray.from_host(rng.uniform(0.5, 2.0, N3))
     g["e"] = DeviceArray.from_host(rng.uniform(0.1, 1.0, N3))
     g["vol"] = DeviceArray.from_host(np.full(N3, 1e-6))
+    nb6 = np.full(6, -1, dtype=np.int32)
     for _ in range(3):
         L.tlhip3d_cg_solver_calc_w(X3, X3, X3, hd, C.addressof(pw), g["p"].ptr, g["w"].ptr,
                                    g["kx"].ptr, g["ky"].ptr, g["kz"].ptr)
@@ -399,6 +402,10 @@ def profile_run(args):
         L.tlhip3d_ppcg_inner_iteration(X3, X3, X3, hd, 0.7, 0.2, g["u"].ptr, g["r"].ptr,
                                        g["kx"].ptr, g["ky"].ptr, g["kz"].ptr, g["sd"].ptr,
                                        g["sd2"].ptr)
+        L.tlhip3d_ppcg_inner_iterations(X3, X3, X3, hd, 2, al.ctypes.data, be.ctypes.data,
+                                        nb6.ctypes.data, g["sd"].ptr, g["sd2"].ptr, g["r"].ptr,
+                                        g["r2"].ptr, g["u"].ptr, g["u2"].ptr, g["kx"].ptr,
+                                        g["ky"].ptr, g["kz"].ptr)
         L.tlhip3d_calculate_2norm(X3, X3, X3, hd, g["r"].ptr, C.addressof(pw))
         L.tlhip3d_field_summary(X3, X3, X3, hd, g["vol"].ptr, g["rho"].ptr, g["e"].ptr,
                                 g["u"].ptr, *a4)
@@ -609,6 +616,10 @@ _TRAFFIC_3D = [
     ("tlhip3d_cg_solver_calc_p", ["k_interior<2, tl::OpCalcP>"], 24, "reads p, r; writes p"),
     ("tlhip3d_ppcg_inner_iteration", ["k_stencil3d<2, tl::OpPpcgInner>"], 72,
      "reads sd, kx, ky, kz, r, u; writes sd', r, u"),
+    ("tlhip3d_ppcg_inner_iterations", ["k_copy_ring3d", "k_copy_ring3d", "k_copy_ring3d",
+                                       "k_ppcg3d_pair_lds<8>"], 72,
+     "two inner steps (halo rings of sd, r, u copied, then one sweep): reads sd, kx, ky, kz, r, "
+     "u; writes sd'', r, u"),
     ("tlhip3d_calculate_2norm", ["k_interior<2, tl::OpNorm>"], 8, "reads r"),
     ("tlhip3d_field_summary", ["k_interior<2, tl::OpFieldSummary>"], 32,
      "reads volume, density, energy, u"),
@@ -766,7 +777,9 @@ def blocks3d_run(args):
     generator, dt 0.004, PPCG with 10 inner steps (presteps 20), each step
     solved to ||u0 - A u|| <= 1e-10 ||u0||.  Reports the wall time per step,
     the iterations the reference would print, and the per-launch time of
-    the PPCG inner sweep (72 B/cell)."""
+    the PPCG inner sweeps: the two-step launch (72 B/cell algorithmic, 144
+    B/cell for the two single steps it replaces) and the single step (72
+    B/cell)."""
     import numpy as np
     import torch
     from tealeaf_amd import blocks
@@ -782,12 +795,12 @@ def blocks3d_run(args):
     s, states = tdeck.parse_deck(text, 3)
     dens, en = blocks.block_fields((n, n, n), 16, seed=256, rho=(0.1, 100.0), energy=(0.1, 25.0),
                                    halo_depth=s.halo_depth)
-    timed = "tlhip3d_ppcg_inner_iteration"
+    timed = ("tlhip3d_ppcg_inner_iterations", "tlhip3d_ppcg_inner_iteration")
     steps = []
     with Problem(s, states) as prob:
         prob.inject(dens, en)
         prob.set_solve(0, 1e-10)
-        L.tlhip_kernel_timer_start(timed.encode())
+        L.tlhip_kernel_timer_start(",".join(timed).encode())
         for tt in range(args.blocks3d_steps):
             L.tlhip_synchronize()
             torch.cuda.synchronize()
@@ -802,13 +815,22 @@ def blocks3d_run(args):
             print(f"[bench] 3d blocks step {tt}: {dt:.3f} s, {rep['printed_cg_iters']} CG + "
                   f"{rep['printed_inner_iters']} PPCG, rel residual {rep['rel_residual']:.2e}",
                   file=sys.stderr, flush=True)
-        ms, nl = ctypes.c_double(), ctypes.c_long()
-        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))
+        kernel = {}
+        for name in timed:
+            ms, nl = ctypes.c_double(), ctypes.c_long()
+            L.tlhip_kernel_timer_read_entry(name.encode(), ctypes.addressof(ms),
+                                            ctypes.addressof(nl))
+            avg_ms = ms.value / max(nl.value, 1)
+            gbs = BYTES_PER_CELL[name] * n ** 3 / (avg_ms * 1e-3) / 1e9 if nl.value else None
+            kernel[name] = {"launches": nl.value, "avg_ms": avg_ms, "gbs": gbs,
+                            "frac": gbs / HBM_PEAK_GBS if gbs else None}
+            if name == timed[0] and gbs:
+                kernel[name]["unfused_equiv_gbs"] = 2 * gbs
+                kernel[name]["unfused_equiv_frac"] = 2 * gbs / HBM_PEAK_GBS
+        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))   # stop
         u64 = prob.get("u")
     total = sum(st["seconds"] for st in steps)
     iters = sum(st["printed_cg"] + st["printed_ppcg"] for st in steps)
-    avg_ms = ms.value / max(nl.value, 1)
-    gbs = BYTES_PER_CELL[timed] * n ** 3 / (avg_ms * 1e-3) / 1e9 if nl.value else None
     # the same steps FP32-refined (FP32 sweeps, FP64 residual replacement)
     steps32 = []
     with Problem(s, states) as prob:
@@ -838,8 +860,7 @@ def blocks3d_run(args):
                         "log-uniform 0.1..100, energy U(0.1,25), seed 256), dt 0.004, PPCG 10 inner "
                         "(presteps 20), relative residual 1e-10, FP64",
             "steps": steps, "seconds_per_step": total / len(steps), "iters_per_s": iters / total,
-            "kernel": {timed: {"launches": nl.value, "avg_ms": avg_ms, "gbs": gbs,
-                               "frac": gbs / HBM_PEAK_GBS if gbs else None}},
+            "kernel": kernel,
             "fp32_refined": {"steps": steps32, "seconds_per_step": total32 / len(steps32),
                              "time_ratio": total32 / total,
                              "rel_error_vs_fp64": float(np.linalg.norm(a - b) / np.linalg.norm(b))}}

@@ -364,6 +364,23 @@ int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int halo_depth,
         double alpha, double beta, double* u, double* r, const double* kx,
         const double* ky, const double* kz, const double* sd_in,
         double* sd_out);
+/* Two PPCG inner steps in one pass (count must be 2).  No single
+ * counterpart in the kernel interface: it replaces two turns of the inner
+ * loop of 3d/drivers/ppcg_solver_driver.c:170-183 (a depth-1 halo update of
+ * sd, then run_ppcg_inner_iteration, 3d/kernel_interface.h:54, with
+ * alphas[i], betas[i]) for a chunk whose six faces are external.  sd, r
+ * and u are read from *_in and written to *_out: interiors bit-exact with
+ * the reference's in-place results, halo rings carried over.  Needs an
+ * even x and 16-byte aligned fields. */
+/* Output rows per block of that pass: 4, 6, 8 (default) or 14, the block's
+ * rows sharing level 1 through LDS; 0 = each wave forms the three level-1
+ * rows it needs itself.  Results are identical. */
+int tlhip_set_pair_rows(int rows);
+int tlhip3d_ppcg_inner_iterations(int x, int y, int z, int halo_depth, int count,
+        const double* alphas, const double* betas, const int* neighbours,
+        const double* sd_in, double* sd_out, const double* r_in, double* r_out,
+        const double* u_in, double* u_out, const double* kx, const double* ky,
+        const double* kz);
 /* 3d/kernel_interface.h:58 run_solver_copy_u */
 int tlhip3d_solver_copy_u(int x, int y, int z, int halo_depth, double* u0,
         const double* u);
@@ -460,7 +477,9 @@ int tlhip_set_graphs(int on);
 /* Steps per fused PPCG launch used by the driver (1..10; 0 or 1 = one step
  * per launch through tlhip2d_ppcg_inner_iteration).  Default 10: the
  * reference deck's 10 inner steps in one launch (FP64, and FP32 in the
- * mixed-precision solve). */
+ * mixed-precision solve).  In 3D any value above 1 runs the inner steps two
+ * per launch (tlhip3d_ppcg_inner_iterations, even x, every face external),
+ * an odd last step on its own. */
 int tlhip_set_ppcg_fusion(int steps);
 /* Whether the 2D driver's fused PPCG launches form the 2-norm of r that
  * ends each outer iteration (2d/drivers/ppcg_driver.c:121-131) in the last

@@ -477,6 +477,38 @@ struct tlhip_problem {
         return halo_update(1);
     }
 
+    // 3D PPCG inner steps: two per launch (tlhip3d_ppcg_inner_iterations,
+    // the halo updates of sd mirrored in the kernel), a last odd one after a
+    // halo update; one per launch after a halo update when pairs are off.
+    bool pair3d() const
+    {
+        for (int f : neighbours)
+            if (f != -1) return false;
+        return dims == 3 && g_ppcg_fusion > 1 && x % 2 == 0 && r_alt && u_alt;
+    }
+
+    int inner_steps3()
+    {
+        int pp = 0;
+        if (pair3d()) {
+            for (; pp + 2 <= s.ppcg_inner_steps; pp += 2) {
+                TRY(tlhip3d_ppcg_inner_iterations(x, y, z, hd, 2, &cheby_alphas[pp],
+                                                  &cheby_betas[pp], neighbours, sd, sd_alt, r,
+                                                  r_alt, u, u_alt, kx, ky, kz));
+                std::swap(sd, sd_alt);
+                std::swap(r, r_alt);
+                std::swap(u, u_alt);
+            }
+        }
+        for (; pp < s.ppcg_inner_steps; ++pp) {
+            TRY(halo_update(1));
+            TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
+                                             kx, ky, kz, sd, sd_alt));
+            std::swap(sd, sd_alt);
+        }
+        return 0;
+    }
+
     // 3D: calc_w, calc_ur (its rrn unused), ppcg_init, the inner steps each
     // after a depth-1 halo update of sd, the 2-norm of r, calc_p
     // (3d/drivers/ppcg_solver_driver.c:88-140).
@@ -487,12 +519,7 @@ struct tlhip_problem {
         TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
         reset_fields();
         fields[F_SD] = true;
-        for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
-            TRY(halo_update(1));
-            TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
-                                             kx, ky, kz, sd, sd_alt));
-            std::swap(sd, sd_alt);
-        }
+        TRY(inner_steps3());
         reset_fields();
         fields[F_P] = true;
         TRY(tl::loop3d::norm(x, y, z, hd, r, rule));
@@ -886,15 +913,15 @@ struct tlhip_problem {
             else TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
             reset_fields();
             fields[F_SD] = true;
-            for (int pp = 0; pp < s.ppcg_inner_steps; ++pp) {
+            if (dims == 3) {
+                TRY(inner_steps3());
+                rep->n_ppcg_inner += s.ppcg_inner_steps;
+            }
+            for (int pp = 0; dims == 2 && pp < s.ppcg_inner_steps; ++pp) {
                 TRY(halo_update(1));
                 double* so = sd_alt;
-                if (dims == 2)
-                    TRY(tlhip2d_ppcg_inner_iteration(x, y, hd, cheby_alphas[pp], cheby_betas[pp],
-                                                     u, r, kx, ky, sd, so));
-                else
-                    TRY(tlhip3d_ppcg_inner_iteration(x, y, z, hd, cheby_alphas[pp],
-                                                     cheby_betas[pp], u, r, kx, ky, kz, sd, so));
+                TRY(tlhip2d_ppcg_inner_iteration(x, y, hd, cheby_alphas[pp], cheby_betas[pp], u, r,
+                                                 kx, ky, sd, so));
                 sd_alt = sd;
                 sd = so;
                 rep->n_ppcg_inner++;
@@ -1589,12 +1616,13 @@ tlhip_problem* new_problem(const tlhip_settings* s)
     return P;
 }
 
-// Partners the 2D driver's fused paths write into.
+// Partners the driver's fused paths write into.
 void alloc_partners(tlhip_problem* P, int* rc)
 {
-    if (P->dims != 2) return;
-    P->p_alt = dalloc(P->n, rc);      // partner of p: fused calc_p + calc_w, fused Chebyshev
-    if (P->s.solver == 3) P->r_alt = dalloc(P->n, rc);   // partner of r for the fused PPCG loop
+    // partner of p: the 2D fused calc_p + calc_w and fused Chebyshev
+    if (P->dims == 2) P->p_alt = dalloc(P->n, rc);
+    // partner of r: the fused PPCG inner loops (2D ring, 3D two-step launch)
+    if (P->s.solver == 3) P->r_alt = dalloc(P->n, rc);
 }
 }  // namespace
 

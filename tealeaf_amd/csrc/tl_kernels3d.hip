@@ -11,6 +11,7 @@
 // adjacent lane by DPP and y-neighbours are read from the cache lines the
 // waves of the adjacent rows fetch (a block is 4 consecutive rows of one
 // strip, and tiles are dealt out XCD-contiguously).
+#include "tl_lanes.h"
 #include "tl_ops.h"
 #include "tl_loop.h"
 #include "tealeaf_hip.h"
@@ -968,6 +969,339 @@ extern "C" int tlhip3d_ppcg_init(int x, int y, int z, int hd, double theta, cons
     return launch_interior3(g, pick_v(g, r, sd), op, what);
 }
 
+namespace {
+
+// ---------------------------------------------------------------- two PPCG steps
+// Two turns of the 3D PPCG inner loop (3d/drivers/ppcg_solver_driver.c:
+// 170-183: a depth-1 halo update of sd, then ppcg_inner_iteration,
+// 3d/c_kernels/omp3/ppcg_solver.c:32-73) in one pass.  On a single chunk the
+// halo update is a mirror (a neighbour outside the interior is the cell
+// itself), so level 2 of a cell needs level 1 only within one cell of it.
+// A wave owns one output row j of a 128-column window and marches up its
+// planes; it forms level 1 for rows j-1, j and j+1 itself (the y
+// neighbours level 2 needs; the rows j-2..j+2 of sd it reads come from the
+// cache lines the block's other waves fetch), carries planes k-1, k, k+1 of
+// sd (five rows) and of level-1 sd (three rows) in registers, and writes
+// level 2 of plane k-1 after forming level 1 of plane k.  x neighbours come
+// from the adjacent lane; the window's end lanes have none, so a window
+// writes 122 of its 128 columns and windows overlap.  r, u and sd are read
+// from *_in and written to *_out.  Each cell's expressions are the
+// reference's, in its order, so the result is bit-for-bit that of two
+// separate steps.
+constexpr int kP2Out = 122;    // output columns per 128-column window
+int g_pair_rows = 8;           // output rows per block of the LDS variant (0: per-wave rows)
+constexpr int kP2Rows = 4;     // output rows per block (one wave each)
+
+using C2 = Cells<double>;
+
+__device__ __forceinline__ C2 ld3(const double* __restrict__ f, const TlGrid3& g, int j, int k,
+                                  int c)
+{
+    if (c < 0 || c + 2 > g.x || j < 0 || j >= g.y || k < 0 || k >= g.z) return zeros<double>();
+    return ldc(f, ((long)k * g.y + j) * g.x + c);
+}
+
+// A a (3d/shared.h:52-58, same association) for a lane's two cells; x
+// neighbours from the adjacent lanes (don't-care at the wave's ends), the
+// cell itself where the neighbour is outside the interior.
+__device__ __forceinline__ C2 apply7(const C2& cen, const C2& no, const C2& so, const C2& up,
+                                     const C2& dn, const C2& kxc, const C2& kyc, const C2& kyn,
+                                     const C2& kzc, const C2& kzu, int c, int xin0, int xin1)
+{
+    const double lft_e = tl_lane_prev(cen.v[1], cen.v[0]);
+    const double rgt_e = tl_lane_next(cen.v[0], cen.v[1]);
+    const double kxe = tl_lane_next(kxc.v[0], kxc.v[1]);
+    C2 a;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double l = i == 0 ? lft_e : cen.v[0];
+        double r = i == 1 ? rgt_e : cen.v[1];
+        if (c + i == xin0) l = cen.v[i];
+        if (c + i == xin1 - 1) r = cen.v[i];
+        const double kxr = i == 1 ? kxe : kxc.v[1];
+        const double diag = ((1.0 + (kxr + kxc.v[i])) + (kyn.v[i] + kyc.v[i])) +
+                            (kzu.v[i] + kzc.v[i]);
+        a.v[i] = diag * cen.v[i] - (kxr * r + kxc.v[i] * l) - (kyn.v[i] * no.v[i] + kyc.v[i] * so.v[i]) -
+                 (kzu.v[i] * up.v[i] + kzc.v[i] * dn.v[i]);
+    }
+    return a;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_ppcg3d_pair(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, double b1, double a2,
+              double b2, const double* __restrict__ sd_in, double* __restrict__ sd_out,
+              const double* __restrict__ r_in, double* __restrict__ r_out,
+              const double* __restrict__ u_in, double* __restrict__ u_out,
+              const double* __restrict__ kx, const double* __restrict__ ky,
+              const double* __restrict__ kz, TlGate gate)
+{
+    if (gate.closed()) return;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile(blockIdx.x, nwx * nby * nbz);
+    const int wx = tile % nwx, rest = tile / nwx;
+    const int by = rest % nby, bz = rest / nby;
+    const int xin0 = g.hd, xin1 = g.x - g.hd, yin0 = g.hd, yin1 = g.y - g.hd;
+    const int zin0 = g.hd, zin1 = g.z - g.hd;
+    const int j = yin0 + by * kP2Rows + wave;
+    if (j >= yin1) return;                     // uniform per wave; no block sync below
+    const int o0 = xin0 + wx * kP2Out, o1 = min(o0 + kP2Out, xin1);
+    const int cw = (o0 - 2) >= 0 ? (o0 - 2) / 2 * 2 : -((3 - o0) / 2 * 2);   // even, <= o0 - 2
+    const int c = cw + 2 * lane;
+    const bool st0 = c >= o0 && c < o1, st1 = c + 1 >= o0 && c + 1 < o1;
+    const int k0 = zin0 + bz * tz, k1 = min(k0 + tz, zin1);
+    const long X = g.x, XY = (long)g.x * g.y;
+
+    // s0[r][p]: sd at row j-2+r, planes kk-1, kk, kk+1 of the level-1 plane kk
+    // s1[r][p]: level-1 sd at row j-1+r, planes km-1, km, km+1 of the level-2 plane km
+    C2 s0[5][3], s1[3][3];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        s0[r][0] = zeros<double>();
+        s0[r][1] = ld3(sd_in, g, j - 2 + r, k0 - 2, c);
+        s0[r][2] = ld3(sd_in, g, j - 2 + r, k0 - 1, c);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) s1[r][0] = s1[r][1] = s1[r][2] = zeros<double>();
+    // level 1's r and u of row j, and row j's coefficients, one plane back
+    C2 r1p = zeros<double>(), u1p = zeros<double>();
+    C2 kxp = zeros<double>(), kyp = zeros<double>(), kynp = zeros<double>();
+    C2 kzp = zeros<double>(), kzup = zeros<double>();
+
+    for (int kk = k0 - 1; kk <= k1; ++kk) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            s0[r][0] = s0[r][1];
+            s0[r][1] = s0[r][2];
+            s0[r][2] = ld3(sd_in, g, j - 2 + r, kk + 1, c);
+        }
+        C2 kxr_[3], kyr_[4], kzr_[3], kzur_[3], r0_[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            kxr_[r] = ld3(kx, g, j - 1 + r, kk, c);
+            kzr_[r] = ld3(kz, g, j - 1 + r, kk, c);
+            kzur_[r] = ld3(kz, g, j - 1 + r, kk + 1, c);
+            r0_[r] = ld3(r_in, g, j - 1 + r, kk, c);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) kyr_[r] = ld3(ky, g, j - 1 + r, kk, c);
+        const C2 u0 = ld3(u_in, g, j, kk, c);
+        // level 1 at plane kk, rows j-1..j+1 (a row or plane outside the
+        // interior is never read: the mirrors below replace it)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            s1[r][0] = s1[r][1];
+            s1[r][1] = s1[r][2];
+        }
+        C2 r1 = zeros<double>(), u1 = zeros<double>();
+        if (kk >= zin0 && kk < zin1) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int jj = j - 1 + r;
+                const C2 cen = s0[r + 1][1];
+                const C2 no = jj == yin1 - 1 ? cen : s0[r + 2][1];
+                const C2 so = jj == yin0 ? cen : s0[r][1];
+                const C2 up = kk == zin1 - 1 ? cen : s0[r + 1][2];
+                const C2 dn = kk == zin0 ? cen : s0[r + 1][0];
+                const C2 w = apply7(cen, no, so, up, dn, kxr_[r], kyr_[r], kyr_[r + 1], kzr_[r],
+                                    kzur_[r], c, xin0, xin1);
+                C2 rn, sn;
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    rn.v[i] = r0_[r].v[i] - w.v[i];
+                    sn.v[i] = a1 * cen.v[i] + b1 * rn.v[i];
+                }
+                s1[r][2] = sn;
+                if (r == 1) {
+                    r1 = rn;
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) u1.v[i] = u0.v[i] + cen.v[i];
+                }
+            }
+        }
+        // level 2 at plane km = kk - 1
+        const int km = kk - 1;
+        if (km >= k0 && km < k1) {
+            const C2 cen = s1[1][1];
+            const C2 no = j == yin1 - 1 ? cen : s1[2][1];
+            const C2 so = j == yin0 ? cen : s1[0][1];
+            const C2 up = km == zin1 - 1 ? cen : s1[1][2];
+            const C2 dn = km == zin0 ? cen : s1[1][0];
+            const C2 w = apply7(cen, no, so, up, dn, kxp, kyp, kynp, kzp, kzup, c, xin0, xin1);
+            C2 rn, un, sn;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                rn.v[i] = r1p.v[i] - w.v[i];
+                un.v[i] = u1p.v[i] + cen.v[i];
+                sn.v[i] = a2 * cen.v[i] + b2 * rn.v[i];
+            }
+            const long b = (long)km * XY + (long)j * X + c;
+            if (st0 && st1) {
+                stc(r_out, b, rn);
+                stc(u_out, b, un);
+                stc(sd_out, b, sn);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    if (i == 0 ? st0 : st1) {
+                        r_out[b + i] = rn.v[i];
+                        u_out[b + i] = un.v[i];
+                        sd_out[b + i] = sn.v[i];
+                    }
+            }
+        }
+        r1p = r1;
+        u1p = u1;
+        kxp = kxr_[1];
+        kyp = kyr_[1];
+        kynp = kyr_[2];
+        kzp = kzr_[1];
+        kzup = kzur_[1];
+    }
+}
+
+// The same two steps with the block's rows sharing level 0 and level 1
+// through LDS: a block is ROWS + 2 waves, one per row j0-1 .. j0+ROWS; each
+// wave forms level 1 of its own row only, and the waves of rows j0 ..
+// j0+ROWS-1 write level 2.  Per plane: every wave publishes its row of sd,
+// then (barrier) forms level 1 from its neighbours' rows and publishes it,
+// then (barrier) forms level 2 of the plane below from the level-1 rows of
+// its neighbours (double-buffered by plane parity).
+template <int ROWS>
+__global__ void __launch_bounds__((ROWS + 2) * TL_WAVE)
+k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, double b1,
+                  double a2, double b2, const double* __restrict__ sd_in,
+                  double* __restrict__ sd_out, const double* __restrict__ r_in,
+                  double* __restrict__ r_out, const double* __restrict__ u_in,
+                  double* __restrict__ u_out, const double* __restrict__ kx,
+                  const double* __restrict__ ky, const double* __restrict__ kz, TlGate gate)
+{
+    constexpr int W = ROWS + 2;
+    __shared__ C2 s0_row[W][TL_WAVE];
+    __shared__ C2 s1_row[2][W][TL_WAVE];
+    if (gate.closed()) return;                   // uniform per launch
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile(blockIdx.x, nwx * nby * nbz);
+    const int wx = tile % nwx, rest = tile / nwx;
+    const int by = rest % nby, bz = rest / nby;
+    const int xin0 = g.hd, xin1 = g.x - g.hd, yin0 = g.hd, yin1 = g.y - g.hd;
+    const int zin0 = g.hd, zin1 = g.z - g.hd;
+    const int jw = yin0 + by * ROWS - 1 + wave;  // this wave's row
+    const bool l1_row = jw >= yin0 && jw < yin1;
+    const bool out_row = wave >= 1 && wave <= ROWS && l1_row;
+    const int o0 = xin0 + wx * kP2Out, o1 = min(o0 + kP2Out, xin1);
+    const int cw = (o0 - 2) >= 0 ? (o0 - 2) / 2 * 2 : -((3 - o0) / 2 * 2);
+    const int c = cw + 2 * lane;
+    const bool st0 = c >= o0 && c < o1, st1 = c + 1 >= o0 && c + 1 < o1;
+    const int k0 = zin0 + bz * tz, k1 = min(k0 + tz, zin1);
+    const long X = g.x, XY = (long)g.x * g.y;
+
+    // own row: sd at planes kk-1, kk, kk+1; level 1 at planes km-1, km, km+1
+    C2 s0w[3], s1w[3];
+    s0w[0] = zeros<double>();
+    s0w[1] = ld3(sd_in, g, jw, k0 - 2, c);
+    s0w[2] = ld3(sd_in, g, jw, k0 - 1, c);
+    s1w[0] = s1w[1] = s1w[2] = zeros<double>();
+    C2 r1p = zeros<double>(), u1p = zeros<double>();
+    C2 kxp = zeros<double>(), kyp = zeros<double>(), kynp = zeros<double>();
+    C2 kzp = zeros<double>(), kzup = zeros<double>();
+    C2 kz_c = ld3(kz, g, jw, k0 - 1, c);         // kz of the level-1 plane
+
+    // What level 1 of plane kk reads from memory, loaded one plane ahead:
+    // sd at plane kk+1, kx, ky (rows j, j+1), r and u at plane kk, kz at
+    // kk+1, and for the block's first and last wave the row beyond it.
+    struct PlaneIn {
+        C2 sdn, kxc, kyc, kyn, kzu, r0, u0, outer;
+    };
+    auto load_plane = [&](int kk) -> PlaneIn {
+        PlaneIn v;
+        v.sdn = ld3(sd_in, g, jw, kk + 1, c);
+        v.kxc = ld3(kx, g, jw, kk, c);
+        v.kyc = ld3(ky, g, jw, kk, c);
+        v.kyn = ld3(ky, g, jw + 1, kk, c);
+        v.kzu = ld3(kz, g, jw, kk + 1, c);
+        v.r0 = ld3(r_in, g, jw, kk, c);
+        v.u0 = out_row ? ld3(u_in, g, jw, kk, c) : zeros<double>();
+        v.outer = wave == 0 ? ld3(sd_in, g, jw - 1, kk, c)
+                            : (wave == W - 1 ? ld3(sd_in, g, jw + 1, kk, c) : zeros<double>());
+        return v;
+    };
+    PlaneIn nxt = load_plane(k0 - 1);
+
+    for (int kk = k0 - 1; kk <= k1; ++kk) {
+        const PlaneIn cur = nxt;
+        if (kk < k1) nxt = load_plane(kk + 1);    // uniform
+        s0w[0] = s0w[1];
+        s0w[1] = s0w[2];
+        s0w[2] = cur.sdn;
+        const C2 kxc = cur.kxc, kyc = cur.kyc, kyn = cur.kyn, kzu = cur.kzu;
+        const C2 r0 = cur.r0, u0 = cur.u0, outer = cur.outer;
+        s0_row[wave][lane] = s0w[1];
+        __syncthreads();
+        s1w[0] = s1w[1];
+        s1w[1] = s1w[2];
+        C2 r1 = zeros<double>(), u1 = zeros<double>();
+        if (l1_row && kk >= zin0 && kk < zin1) {
+            const C2 cen = s0w[1];
+            const C2 so = jw == yin0 ? cen : (wave == 0 ? outer : s0_row[wave - 1][lane]);
+            const C2 no = jw == yin1 - 1 ? cen : (wave == W - 1 ? outer : s0_row[wave + 1][lane]);
+            const C2 up = kk == zin1 - 1 ? cen : s0w[2];
+            const C2 dn = kk == zin0 ? cen : s0w[0];
+            const C2 w = apply7(cen, no, so, up, dn, kxc, kyc, kyn, kz_c, kzu, c, xin0, xin1);
+            C2 sn;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                r1.v[i] = r0.v[i] - w.v[i];
+                sn.v[i] = a1 * cen.v[i] + b1 * r1.v[i];
+                u1.v[i] = u0.v[i] + cen.v[i];
+            }
+            s1w[2] = sn;
+        }
+        s1_row[kk & 1][wave][lane] = s1w[2];
+        __syncthreads();
+        // level 2 at plane km = kk - 1
+        const int km = kk - 1;
+        if (out_row && km >= k0 && km < k1) {
+            const C2 cen = s1w[1];
+            const C2 so = jw == yin0 ? cen : s1_row[km & 1][wave - 1][lane];
+            const C2 no = jw == yin1 - 1 ? cen : s1_row[km & 1][wave + 1][lane];
+            const C2 up = km == zin1 - 1 ? cen : s1w[2];
+            const C2 dn = km == zin0 ? cen : s1w[0];
+            const C2 w = apply7(cen, no, so, up, dn, kxp, kyp, kynp, kzp, kzup, c, xin0, xin1);
+            C2 rn, un, sn;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                rn.v[i] = r1p.v[i] - w.v[i];
+                un.v[i] = u1p.v[i] + cen.v[i];
+                sn.v[i] = a2 * cen.v[i] + b2 * rn.v[i];
+            }
+            const long b = (long)km * XY + (long)jw * X + c;
+            if (st0 && st1) {
+                stc(r_out, b, rn);
+                stc(u_out, b, un);
+                stc(sd_out, b, sn);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    if (i == 0 ? st0 : st1) {
+                        r_out[b + i] = rn.v[i];
+                        u_out[b + i] = un.v[i];
+                        sd_out[b + i] = sn.v[i];
+                    }
+            }
+        }
+        r1p = r1;
+        u1p = u1;
+        kxp = kxc;
+        kyp = kyc;
+        kynp = kyn;
+        kzp = kz_c;
+        kzup = kzu;
+        kz_c = kzu;
+    }
+}
+
+}  // namespace
+
 extern "C" int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int hd, double alpha,
                                             double beta, double* u, double* r, const double* kx,
                                             const double* ky, const double* kz,
@@ -983,6 +1317,71 @@ extern "C" int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int hd, double 
     TL_TRY(launch_stencil(g, pick_v(g, u, r, kx, ky, kz, sd_in, sd_out), sd_in, kx, ky, kz, op,
                           what));
     return copy_ring(g, sd_in, sd_out, what);
+}
+
+extern "C" int tlhip_set_pair_rows(int rows)
+{
+    if (rows != 0 && rows != 4 && rows != 6 && rows != 8 && rows != 14)
+        return tl_fail(TLHIP_ERR_ARG, "tlhip_set_pair_rows: 0, 4, 6, 8 or 14 (got %d)", rows);
+    g_pair_rows = rows;
+    return 0;
+}
+
+// Two PPCG inner steps per launch; planes per z-tile: tlhip_set_tile_planes
+// when set, else 32 (256^3: 0.384 ms against 0.40 at 16 and 0.46 at 64).
+extern "C" int tlhip3d_ppcg_inner_iterations(int x, int y, int z, int hd, int count,
+                                             const double* alphas, const double* betas,
+                                             const int* neighbours, const double* sd_in,
+                                             double* sd_out, const double* r_in, double* r_out,
+                                             const double* u_in, double* u_out,
+                                             const double* kx, const double* ky,
+                                             const double* kz)
+{
+    const char* what = "tlhip3d_ppcg_inner_iterations";
+    TL_TRY(tl_ensure_init());
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    if (count != 2) return tl_fail(TLHIP_ERR_ARG, "%s: count must be 2 (got %d)", what, count);
+    if (x % 2) return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x (got %d)", what, x);
+    REQUIRE(alphas && betas && neighbours, what, "null coefficients/neighbours");
+    REQUIRE(sd_in && sd_out && r_in && r_out && u_in && u_out && kx && ky && kz, what,
+            "null argument");
+    REQUIRE(sd_in != sd_out && r_in != r_out && u_in != u_out, what,
+            "input and output buffers must be distinct");
+    for (int f = 0; f < 6; ++f)
+        REQUIRE(neighbours[f] == -1, what, "every face must be external (single chunk)");
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    for (const double* f : {sd_in, (const double*)sd_out, r_in, (const double*)r_out, u_in,
+                            (const double*)u_out, kx, ky, kz})
+        REQUIRE(((uintptr_t)f & 15) == 0, what, "fields must be 16-byte aligned");
+    const int nx = x - 2 * hd, ny = y - 2 * hd, nz = z - 2 * hd;
+    const int tz = g_tz_override > 0 ? g_tz_override : 32;
+    const int rows = g_pair_rows;                 // 0: each wave forms its three level-1 rows
+    const int nwx = (int)tl_cdiv(nx, kP2Out), nby = (int)tl_cdiv(ny, rows > 0 ? rows : kP2Rows);
+    const int nbz = (int)tl_cdiv(nz, tz);
+    const long nb = (long)nwx * nby * nbz;
+    if (nb > (1L << 30)) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    TL_TRY(copy_ring(g, sd_in, sd_out, what));
+    TL_TRY(copy_ring(g, r_in, r_out, what));
+    TL_TRY(copy_ring(g, u_in, u_out, what));
+    const int slot = tl_timer_begin(what);
+#define TL_PAIR_ARGS g, nwx, nby, nbz, tz, alphas[0], betas[0], alphas[1], betas[1], sd_in, \
+        sd_out, r_in, r_out, u_in, u_out, kx, ky, kz, tl_gate()
+    switch (rows) {
+    case 4: hipLaunchKernelGGL((k_ppcg3d_pair_lds<4>), dim3((unsigned)nb), dim3(6 * TL_WAVE), 0,
+                               tl_stream(), TL_PAIR_ARGS); break;
+    case 6: hipLaunchKernelGGL((k_ppcg3d_pair_lds<6>), dim3((unsigned)nb), dim3(8 * TL_WAVE), 0,
+                               tl_stream(), TL_PAIR_ARGS); break;
+    case 8: hipLaunchKernelGGL((k_ppcg3d_pair_lds<8>), dim3((unsigned)nb), dim3(10 * TL_WAVE), 0,
+                               tl_stream(), TL_PAIR_ARGS); break;
+    case 14: hipLaunchKernelGGL((k_ppcg3d_pair_lds<14>), dim3((unsigned)nb), dim3(16 * TL_WAVE), 0,
+                                tl_stream(), TL_PAIR_ARGS); break;
+    default: hipLaunchKernelGGL(k_ppcg3d_pair, dim3((unsigned)nb), dim3(kBlock), 0, tl_stream(),
+                                TL_PAIR_ARGS); break;
+    }
+#undef TL_PAIR_ARGS
+    tl_timer_end(slot);
+    return tl_check_launch(what);
 }
 
 // ---------------------------------------------------------------- shared

@@ -137,14 +137,14 @@ def test_fused_rejects_bad_layout(hip):
         b.free()
 
 
-def _run_fusions(hip, text, fusions, fused_norm):
+def _run_fusions(hip, text, fusions, fused_norm, dims=2):
     from tealeaf_amd.problem import Problem
     res = {}
     hip.tlhip_set_ppcg_fused_norm(int(fused_norm))
     try:
         for fusion in fusions:
             hip.tlhip_set_ppcg_fusion(fusion)
-            with Problem.from_deck(text, 2) as prob:
+            with Problem.from_deck(text, dims) as prob:
                 steps = prob.run()
                 res[fusion] = (steps, prob.get("u"), prob.get("energy"), prob.field_summary())
     finally:
@@ -179,6 +179,29 @@ def test_driver_fusion_is_bit_identical(hip, inner):
         assert np.max(np.abs(e - base[2])) <= 1e-13 * np.max(np.abs(base[2]))
         for a, b in zip(summ, base[3]):
             assert abs(a - b) <= 1e-13 * abs(b), (summ, base[3])
+
+
+@pytest.mark.parametrize("resident", [1, 0])
+@pytest.mark.parametrize("n,inner", [(24, 1), (24, 7), (30, 10), (23, 10)])
+def test_driver_3d_pairs_bit_identical(hip, resident, n, inner):
+    """3D PPCG with two inner steps per launch (fusion > 1, even x) gives
+    the same bits as one launch per step after a halo update (fusion 1): odd
+    inner counts end on a single step, an odd x (23 + 2*2 cells) keeps the
+    single steps; on the resident and the host-driven driver."""
+    text = tdeck.cuboid_3d(n, 2, solver=tdeck.PPCG, presteps=10, inner=inner, max_iters=400)
+    hip.tlhip_set_resident(resident)
+    try:
+        res = _run_fusions(hip, text, (1, 10), False, dims=3)
+    finally:
+        hip.tlhip_set_resident(1)
+    base = res[1]
+    steps, u, e, summ = res[10]
+    assert [s["n_ppcg_inner"] for s in steps] == [s["n_ppcg_inner"] for s in base[0]]
+    assert [s["n_ppcg_outer"] for s in steps] == [s["n_ppcg_outer"] for s in base[0]]
+    assert sum(s["n_ppcg_outer"] for s in steps) > 0
+    np.testing.assert_array_equal(u, base[1])
+    np.testing.assert_array_equal(e, base[2])
+    assert summ == base[3]
 
 
 def test_set_ppcg_fusion_range(hip):
