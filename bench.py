@@ -41,6 +41,9 @@ BYTES_PER_CELL = {
     "tlhip2d_ppcg_inner_iteration": 64,   # read sd, kx, ky, r, u; write r, u, sd'
     "tlhip2d_ppcg_inner_iterations": 64,  # the same arrays once for S fused inner steps
     "tlhip2d_cg_calc_pw": 48,             # read p, r, kx, ky; write p', w
+    "tlhip2d_cg_calc_pw_cond": 40,        # read p, r, cond; write p', w
+    "tlhip2d_cg_calc_pw_cond.no_w": 32,   # read p, r, cond; write p' (w kept in registers)
+    "tlhip2d_cg_calc_ur_cond": 48,        # read u, p, r, cond; write u, r
     "tlhip2d_cg_calc_w": 32,              # read p, kx, ky; write w
     "tlhip2d_cg_calc_ur": 48,             # read u, p, r, w; write u, r
     "tlhip2d_cg_calc_p": 24,              # read p, r; write p
@@ -120,12 +123,14 @@ def gpu_run(args):
     total_time = 0.0
     total_iters = 0
     kernel = {}
-    # calc_pw (every leg) and the leg's own sweep; each timed launch adds two
-    # event records to the stream, so the rest (calc_ur: 19% of the kernel
-    # time, never the largest) is left untimed
-    own = {"cg": [], "cheby": ["tlhip2d_cheby_iterations"], "ppcg": [dominant(args)]}
+    # the CG pair (every leg: CG, and the CG phase of Chebyshev and PPCG) and
+    # the leg's own passes; each timed launch adds two event records to the
+    # stream, so the small rest (2-norms, finishers, halo and ring copies) is
+    # left untimed
+    own = {"cg": [], "cheby": ["tlhip2d_cheby_iterations"],
+           "ppcg": [dominant(args), "tlhip2d_cg_calc_pw_cond"]}
     for s in solvers:
-        timed = ["tlhip2d_cg_calc_pw"] + own[s]
+        timed = ["tlhip2d_cg_calc_pw_cond.no_w", "tlhip2d_cg_calc_ur_cond"] + own[s]
         L.tlhip_synchronize()
         torch.cuda.synchronize()
         L.tlhip_kernel_timer_start(",".join(timed).encode())
@@ -199,7 +204,7 @@ def refine_run(args):
             # the last step also times its main kernels (HIP events)
             names = (["tlhip2d_mix_calc_pw32", "tlhip2d_ppcg_inner_iterations_f32",
                       "tlhip2d_mix_norm32"] if prec else
-                     ["tlhip2d_cg_calc_pw", "tlhip2d_ppcg_inner_iterations",
+                     ["tlhip2d_cg_calc_pw_cond", "tlhip2d_ppcg_inner_iterations",
                       "tlhip2d_calculate_2norm"])
             kern = {}
             for tt in range(args.refine_steps):
@@ -299,7 +304,8 @@ def profile_run(args):
     """Counter-collection run (rocprofv3 --pmc): three launches each of every
     hot kernel on synthetic fields -- 2D at --cells^2 (the fused PPCG and
     Chebyshev loops, calc_w, calc_pw, calc_ur, calc_p, the 2-norm,
-    field_summary, cg_init and the single-step PPCG sweep; the FP32 fused
+    field_summary, cg_init and the single-step PPCG sweep; the CG pair from
+    the conductivity and PPCG's calc_pw from it; the FP32 fused
     PPCG and Chebyshev loops), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
     two-step PPCG sweep, the 2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
     2D PPCG solve at 2048^2 for the FP32 CG kernels, and a 1 GiB device copy
@@ -325,6 +331,8 @@ def profile_run(args):
         f64[k] = DeviceArray.from_host(rng.uniform(0.05, 0.2, N))
     f64["rho"] = DeviceArray.from_host(rng.uniform(0.5, 2.0, N))
     f64["e"] = DeviceArray.from_host(rng.uniform(0.1, 1.0, N))
+    f64["cond"] = DeviceArray(N, np.float64)
+    L.tlhip2d_cg_cond(x, y, hd, 2, f64["rho"].ptr, f64["cond"].ptr)
     f64["vol"] = DeviceArray.from_host(np.full(N, 1e-4))
     f32 = {k: DeviceArray.from_host(rng.uniform(-1, 1, P * y).astype(np.float32))
            for k in ("u", "u2", "r", "r2", "sd", "sd2", "p", "w")}
@@ -351,6 +359,13 @@ def profile_run(args):
                                         f64["ky"].ptr, 0.0, None, None)
         L.tlhip2d_cg_calc_pw(x, y, hd, 0.5, C.addressof(pw), f64["p"].ptr, f64["sd"].ptr,
                              f64["r"].ptr, f64["w"].ptr, f64["kx"].ptr, f64["ky"].ptr)
+        # the CG pair from the conductivity, then PPCG's calc_pw from it
+        L.tlhip2d_cg_calc_pw_cond(x, y, hd, 1, 0.5, 0.01, 0.01, C.addressof(pw), f64["p"].ptr,
+                                  f64["sd"].ptr, f64["r"].ptr, None, f64["cond"].ptr)
+        L.tlhip2d_cg_calc_ur_cond(x, y, hd, 1e-3, 0.01, 0.01, C.addressof(pw), f64["u"].ptr,
+                                  f64["sd"].ptr, f64["r"].ptr, f64["cond"].ptr)
+        L.tlhip2d_cg_calc_pw_cond(x, y, hd, 1, 0.5, 0.01, 0.01, C.addressof(pw), f64["p"].ptr,
+                                  f64["sd"].ptr, f64["r"].ptr, f64["w"].ptr, f64["cond"].ptr)
         # 10 Chebyshev iterations (u -> u2, p -> sd2; u0 = sd)
         L.tlhip2d_cheby_iterations(x, y, hd, 10, al.ctypes.data, be.ctypes.data, nb.ctypes.data,
                                    f64["u"].ptr, f64["u2"].ptr, f64["sd"].ptr, f64["p"].ptr,
@@ -590,6 +605,13 @@ _TRAFFIC_2D = [
      "10 inner steps from sd: reads sd, kx, ky, r, u; writes sd, r, u"),
     ("tlhip2d_cg_calc_pw", ["k_calc_pw<double, true>"], 48,
      "calc_p fused into calc_w: reads p, r, kx, ky; writes p', w"),
+    ("tlhip2d_cg_calc_pw_cond.no_w", ["k_calc_pw_cond<true, false>"], 32,
+     "calc_p fused into calc_w from the conductivity, w kept in registers: reads p, r, cond; "
+     "writes p'"),
+    ("tlhip2d_cg_calc_ur_cond", ["k_calc_ur_cond"], 48,
+     "cg_calc_ur forming w = A p from p and cond: reads u, p, r, cond; writes u, r"),
+    ("tlhip2d_cg_calc_pw_cond", ["k_calc_pw_cond<true, true>"], 40,
+     "calc_p fused into calc_w from the conductivity (PPCG): reads p, r, cond; writes p', w"),
     ("tlhip2d_cheby_iterations", ["k_cheby_fused2d<double, 10"], 64,
      "10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
     ("tlhip2d_ppcg_inner_iterations_f32 (from r)", ["k_ppcg_ring2d<float, 10, 2, false>"], 36,

@@ -233,6 +233,27 @@ int tlhip2d_ppcg_inner_iterations_f32_norm(int x, int y, int halo_depth, int pit
 int tlhip2d_cg_calc_pw(int x, int y, int halo_depth, double beta, double* pw,
         const double* p_in, double* p_out, const double* r, double* w,
         const double* kx, const double* ky);
+/* The CG pair of passes with kx and ky formed from the cells' conductivity
+ * instead of loaded.  No counterpart in the reference: one 8-byte field
+ * replaces kx and ky, and w = A p need not be stored between the passes.
+ * cond is run_cg_init's intermediate w (2d/c_kernels/omp3/cg.c:45-53:
+ * density, or 1/density for coefficient 2) over the whole grid, formed by
+ * tlhip2d_cg_cond from the density cg_init reads; every face coefficient is
+ * then cg_init's expression on the same operands (cg.c:55-64), so results
+ * equal those from cg_init's kx and ky bit for bit.  Halo depth >= 2 (where
+ * cg_init's kx reads no stale w) and an even x.
+ * tlhip2d_cg_calc_pw_cond: with update, run_cg_calc_p (2d/kernel_interface.h:42)
+ * and run_cg_calc_w (:38) as tlhip2d_cg_calc_pw; without, run_cg_calc_w on
+ * p_in (p_out and r unused).  w is written unless NULL; *pw += sum w p.
+ * tlhip2d_cg_calc_ur_cond: run_cg_calc_ur (:40) with w = A p formed from p
+ * and cond in the pass: u += alpha p, r -= alpha w, *rrn += sum r r. */
+int tlhip2d_cg_cond(int x, int y, int halo_depth, int coefficient, const double* density,
+        double* cond);
+int tlhip2d_cg_calc_pw_cond(int x, int y, int halo_depth, int update, double beta, double rx,
+        double ry, double* pw, const double* p_in, double* p_out, const double* r,
+        double* w, const double* cond);
+int tlhip2d_cg_calc_ur_cond(int x, int y, int halo_depth, double alpha, double rx, double ry,
+        double* rrn, double* u, const double* p, double* r, const double* cond);
 /* Temporally blocked Chebyshev loop, 1 <= count <= 10 iterations in one
  * pass.  No single counterpart in the kernel interface: it replaces `count`
  * turns of the Chebyshev loop of 2d/drivers/cheby_driver.c (a depth-1 halo
@@ -454,6 +475,15 @@ typedef struct tlhip_problem tlhip_problem;
  * Results are identical either way.  Default on: at 4096^2 on MI355X the
  * fused pass (48 B/cell, 0.162 ms) takes 9% less than the two (0.177 ms). */
 int tlhip_set_cg_fusion(int on);
+/* Whether the driver's 2D CG steps (CG, and the CG phase of Chebyshev and
+ * PPCG; halo depth >= 2, with tlhip_set_cg_fusion on) run as the cond pair
+ * (tlhip2d_cg_calc_pw_cond without storing w, then tlhip2d_cg_calc_ur_cond)
+ * and PPCG's calc_p + calc_w reads cond in place of kx and ky.  u, p and r
+ * are the reference's expressions either way; rrn is summed in another
+ * order, so results agree to rounding.  Default on: at 4096^2 on MI355X a
+ * CG step's two passes take 0.284 ms against 0.333 ms (32 + 48 B/cell
+ * against 48 + 48).  No counterpart in the reference. */
+int tlhip_set_cg_cond(int on);
 /* Whether the driver runs CG, Jacobi and PPCG outer iterations (2D, 3D)
  * device-resident: alpha = rro/pw, beta = rrn/rro, the
  * Jacobi error and the stop and switch tests of cg_driver.c /

@@ -130,6 +130,14 @@ __device__ __forceinline__ float tl_lane_next(float v, float edge)
 
 // Butterfly sum over the 64 lanes; every lane gets the total.  The order of
 // additions is fixed, so the result is reproducible.
+// A face's coefficient as cg_init forms kx and ky (2d/c_kernels/omp3/cg.c:55-64):
+// with w_a the conductivity of the cell below / left of the face and w_b
+// that of the cell itself, r (w_a + w_b) / (2 w_a w_b), operands in that order.
+__device__ __forceinline__ double tl_face_coef(double r, double wa, double wb)
+{
+    return r * (wa + wb) / (2.0 * wa * wb);
+}
+
 __device__ __forceinline__ double tl_wave_sum(double v)
 {
 #pragma unroll

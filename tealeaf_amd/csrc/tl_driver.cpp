@@ -49,6 +49,7 @@ int g_ppcg_fused_norm = 1;   // tlhip_set_ppcg_fused_norm
 // Chebyshev iterations per fused launch (2D, FP64, reference stop test).
 int g_cheby_fusion = 10;
 int g_cg_fusion = 1;     // fuse calc_p into the next calc_w (2D, tlhip2d_cg_calc_pw)
+int g_cg_cond = 1;       // 2D CG passes read the conductivity for kx, ky (tlhip2d_cg_*_cond)
 int g_resident = 1;      // device-resident CG / PPCG iterations (tl_loop.h)
 // Replay the resident loop's iterations from a HIP graph on small grids
 // (up to kGraphCells cells with halo, where an iteration is a handful of
@@ -128,6 +129,8 @@ struct tlhip_problem {
     double *u = nullptr, *u0 = nullptr, *p = nullptr, *r = nullptr, *mi = nullptr, *w = nullptr;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr, *sd = nullptr;
     double *u_alt = nullptr, *sd_alt = nullptr, *r_alt = nullptr, *p_alt = nullptr;
+    double* cond = nullptr;   // 2D: cg_init's conductivity w, for the cond CG passes
+    double rx = 0.0, ry = 0.0;
     double *volume = nullptr, *x_area = nullptr, *y_area = nullptr, *z_area = nullptr;
     double *cell_x = nullptr, *cell_y = nullptr, *cell_z = nullptr;
     double *vertex_x = nullptr, *vertex_y = nullptr, *vertex_z = nullptr;
@@ -217,14 +220,18 @@ struct tlhip_problem {
     }
 
     // -------------------------------------------------------- CG (cg_driver.c)
-    int cg_init_driver(double rx, double ry, double rz, double* rro)
+    int cg_init_driver(double rx_, double ry_, double rz, double* rro)
     {
         *rro = 0.0;
-        if (dims == 2)
+        rx = rx_;
+        ry = ry_;
+        if (dims == 2) {
             TRY(tlhip2d_cg_init(x, y, hd, s.coefficient, rx, ry, rro, density, energy, u, p, r, w, kx, ky));
-        else
+            if (cond_ok()) TRY(tlhip2d_cg_cond(x, y, hd, s.coefficient, density, cond));
+        } else {
             TRY(tlhip3d_cg_solver_init(x, y, z, hd, s.coefficient, rx, ry, rz, rro, density,
                                        energy, u, p, r, w, kx, ky, kz));
+        }
         reset_fields();
         fields[F_U] = fields[F_P] = true;
         TRY(halo_update(1));
@@ -243,11 +250,46 @@ struct tlhip_problem {
     // 0.232 against 0.217 ms per pass).
     bool can_defer_p() const { return g_cg_fusion && dims == 2 && x % 2 == 0 && p_alt; }
 
+    // 2D CG steps (deferred calc_p, halo depth >= 2) run as the cond pair:
+    // calc_pw reads cg_init's conductivity (cond) in place of kx and ky and
+    // keeps w in registers; calc_ur forms w = A p again from p and cond.
+    // u, r and p are the reference's values; rrn is summed in another order.
+    // PPCG's calc_pw, whose w the fused inner loop reads, reads cond too and
+    // stores w (bit for bit the kx/ky pass).
+    bool cond_ok() const
+    {
+        return g_cg_cond && can_defer_p() && hd >= 2 && cond &&
+               (s.coefficient == 1 || s.coefficient == 2);
+    }
+
+    // cg_main_step's calc_w (or pending calc_p + calc_w) and calc_ur as the
+    // cond pair, alpha and beta on the host.
+    int cg_step_cond(double* rro, double* alpha, double* rrn)
+    {
+        double pw = 0.0;
+        if (pending_p) {
+            pending_p = false;
+            TRY(tlhip2d_cg_calc_pw_cond(x, y, hd, 1, pending_beta, rx, ry, &pw, p, p_alt, r,
+                                        nullptr, cond));
+            std::swap(p, p_alt);
+        } else {
+            TRY(tlhip2d_cg_calc_pw_cond(x, y, hd, 0, 0.0, rx, ry, &pw, p, nullptr, nullptr,
+                                        nullptr, cond));
+        }
+        *alpha = *rro / pw;
+        *rrn = 0.0;
+        return tlhip2d_cg_calc_ur_cond(x, y, hd, *alpha, rx, ry, rrn, u, p, r, cond);
+    }
+
     int calc_w_pending(double* pw)
     {
         if (!pending_p) return calc_w(pw);
         pending_p = false;
-        TRY(tlhip2d_cg_calc_pw(x, y, hd, pending_beta, pw, p, p_alt, r, w, kx, ky));
+        if (cond_ok())
+            TRY(tlhip2d_cg_calc_pw_cond(x, y, hd, 1, pending_beta, rx, ry, pw, p, p_alt, r, w,
+                                        cond));
+        else
+            TRY(tlhip2d_cg_calc_pw(x, y, hd, pending_beta, pw, p, p_alt, r, w, kx, ky));
         std::swap(p, p_alt);
         return 0;
     }
@@ -277,12 +319,16 @@ struct tlhip_problem {
 
     int cg_main_step(int tt, double* rro, double* error)
     {
-        double pw = 0.0;
-        TRY(calc_w_pending(&pw));
-        const double alpha = *rro / pw;
-        double rrn = 0.0;
+        double alpha = 0.0, rrn = 0.0;
+        if (cond_ok()) {
+            TRY(cg_step_cond(rro, &alpha, &rrn));
+        } else {
+            double pw = 0.0;
+            TRY(calc_w_pending(&pw));
+            alpha = *rro / pw;
+            TRY(calc_ur(alpha, &rrn));
+        }
         cg_alphas[tt] = alpha;
-        TRY(calc_ur(alpha, &rrn));
         const double beta = rrn / *rro;
         cg_betas[tt] = beta;
         TRY(calc_p_deferred(beta));
@@ -396,14 +442,29 @@ struct tlhip_problem {
     {
         if (!pending_p) return tl::loop2d::calc_w(x, y, hd, p, w, kx, ky);
         pending_p = false;
-        TRY(tl::loop2d::calc_pw(x, y, hd, p, p_alt, r, w, kx, ky));
+        if (cond_ok())
+            TRY(tl::loop2d::calc_pw_cond(x, y, hd, true, p, p_alt, r, w, cond, rx, ry));
+        else
+            TRY(tl::loop2d::calc_pw(x, y, hd, p, p_alt, r, w, kx, ky));
         std::swap(p, p_alt);
         return 0;
     }
 
     int cg_body(const tl::loop::StopRule& rule)
     {
-        if (can_defer_p()) {
+        if (cond_ok()) {
+            if (pending_p) {
+                pending_p = false;
+                TRY(tl::loop2d::calc_pw_cond(x, y, hd, true, p, p_alt, r, nullptr, cond, rx, ry));
+                std::swap(p, p_alt);
+            } else {
+                TRY(tl::loop2d::calc_pw_cond(x, y, hd, false, p, nullptr, nullptr, nullptr, cond,
+                                             rx, ry));
+            }
+            TRY(tl::loop2d::calc_ur_cond(x, y, hd, u, p, r, cond, rx, ry, rule));
+            pending_p = true;
+            return 0;
+        } else if (can_defer_p()) {
             TRY(calc_w_resident());
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
             pending_p = true;    // calc_p waits for the next calc_w (beta stays on the device)
@@ -1499,6 +1560,7 @@ struct tlhip_problem {
             if (a) (void)tlhip_free(a);
         if (kz32) (void)tlhip_free(kz32);
         if (alpha32) (void)tlhip_free(alpha32);
+        if (cond) (void)tlhip_free(cond);
         if (adopted) {      // the caller's fields stay; only the partners of r and p are ours
             for (double* a : {r == home[2] ? r_alt : r, p == home[3] ? p_alt : p})
                 if (a) (void)tlhip_free(a);
@@ -1534,6 +1596,12 @@ extern "C" int tlhip_set_cheby_fusion(int steps)
 extern "C" int tlhip_set_cg_fusion(int on)
 {
     g_cg_fusion = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int tlhip_set_cg_cond(int on)
+{
+    g_cg_cond = on ? 1 : 0;
     return 0;
 }
 
@@ -1621,6 +1689,9 @@ void alloc_partners(tlhip_problem* P, int* rc)
 {
     // partner of p: the 2D fused calc_p + calc_w and fused Chebyshev
     if (P->dims == 2) P->p_alt = dalloc(P->n, rc);
+    // cg_init's conductivity for the 2D cond CG passes (CG, Chebyshev, PPCG)
+    if (P->dims == 2 && P->hd >= 2 && P->s.solver >= 1 && P->s.solver <= 3)
+        P->cond = dalloc(P->n, rc);
     // partner of r: the fused PPCG inner loops (2D ring, 3D two-step launch)
     if (P->s.solver == 3) P->r_alt = dalloc(P->n, rc);
 }

@@ -771,6 +771,216 @@ k_calc_pw_rows(PitchGrid g, int cw0, int nbx, int nby, int ty, T beta, const T* 
 }
 
 
+// kx of a lane's cells on one row and ky of two rows' faces, formed from
+// the conductivity w (cg_init's w: density, or 1/density with the
+// reciprocal coefficient; 2d/c_kernels/omp3/cg.c:45-53): kx from the row's
+// w (wc) and the left neighbour of the wave's first cell (wl_edge), ky from
+// the row below (wd) and the row (wc).
+__device__ __forceinline__ Cells<double> kx_of_row(double rx, const Cells<double>& wc,
+                                                   double wl_edge)
+{
+    Cells<double> k;
+    k.v[0] = tl_face_coef(rx, tl_lane_prev(wc.v[1], wl_edge), wc.v[0]);
+    k.v[1] = tl_face_coef(rx, wc.v[0], wc.v[1]);
+    return k;
+}
+
+__device__ __forceinline__ Cells<double> ky_of_rows(double ry, const Cells<double>& wd,
+                                                    const Cells<double>& wc)
+{
+    Cells<double> k;
+    k.v[0] = tl_face_coef(ry, wd.v[0], wc.v[0]);
+    k.v[1] = tl_face_coef(ry, wd.v[1], wc.v[1]);
+    return k;
+}
+
+// The CG pair of passes with kx and ky formed from the cells' conductivity
+// instead of loaded: cond holds cg_init's w and every face coefficient is
+// cg_init's expression on the same operands (tl_face_coef), so the stencil
+// sees kx and ky bit for bit while one 8-byte field per cell replaces two.
+// The first pass is k_calc_pw's (FP64): p' = beta p + r with UPDATE, w = A p'
+// and sum w p'; with WOUT false w stays in registers and only its dot
+// product leaves the kernel.  The second (k_calc_ur_cond) forms the same w
+// again for cg_calc_ur.  cond is read at rows and columns hd-1 .. n-hd,
+// which cg_init's w covers for halo depth >= 2 (cg.c:45 starts at 1).
+template <bool UPDATE, bool WOUT>
+__global__ void __launch_bounds__(kBlock)
+k_calc_pw_cond(PitchGrid g, int cw0, int ns, int nbx, int nby, int ty, double beta_arg,
+               const double* __restrict__ beta_dev, const double* __restrict__ p_in,
+               double* __restrict__ p_out, const double* __restrict__ r, double* __restrict__ w,
+               const double* __restrict__ cond, double rx, double ry, TlReduce rd, TlGate gate)
+{
+    if (gate.closed()) return;
+    const double beta = UPDATE && beta_dev ? *beta_dev : beta_arg;
+    constexpr int N = 2;
+    using C = Cells<double>;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile_of(blockIdx.x, nbx * nby);
+    const int sx = (tile % nbx) * kWaves + wave, by = tile / nbx;
+    const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
+    const int c = cw0 + sx * (TL_WAVE * N) + N * lane;
+    const bool ld_ok = sx < ns && c + N <= g.X;
+    const bool all_out = c >= xin0 && c + N <= xin1;
+    const bool any_out = sx < ns && c + N > xin0 && c < xin1;
+    const long X = g.X;
+    auto ld = [&](const double* __restrict__ f, long i) -> C {
+        return ld_ok ? ldc(f, i) : zeros<double>();
+    };
+    auto pnew = [&](long i) -> C {
+        C a = ld(p_in, i);
+        if constexpr (UPDATE) {
+            const C b = ld(r, i);
+#pragma unroll
+            for (int k = 0; k < N; ++k) a.v[k] = beta * a.v[k] + b.v[k];
+        }
+        return a;
+    };
+    auto pnew1 = [&](long i) -> double {
+        if constexpr (UPDATE) return beta * p_in[i] + r[i];
+        return p_in[i];
+    };
+    double acc[1] = {0.0};
+    const int j0 = yin0 + by * ty;
+    const int j1 = min(j0 + ty, yin1);
+    if (sx < ns && j0 < j1) {                    // uniform per wave
+        long i0 = (long)j0 * X + c;
+        C cen = pnew(i0);
+        C dn = j0 == yin0 ? cen : pnew(i0 - X);  // y neighbours outside the interior mirror
+        C wc = ld(cond, i0);
+        C kyc = ky_of_rows(ry, ld(cond, i0 - X), wc);
+        for (int j = j0; j < j1; ++j, i0 += X) {
+            const C up = j == yin1 - 1 ? cen : pnew(i0 + X);
+            const C wu = ld(cond, i0 + X);
+            const C kyu = ky_of_rows(ry, wc, wu);
+            double el = 0.0, er = 0.0, ek = 0.0, wl = 0.0;
+            if (lane == 0 && c >= 1) {
+                el = pnew1(i0 - 1);
+                wl = cond[i0 - 1];
+            }
+            if (lane == TL_WAVE - 1 && c + N < g.X) {
+                er = pnew1(i0 + N);
+                ek = tl_face_coef(rx, wc.v[N - 1], cond[i0 + N]);
+            }
+            const C kxc = kx_of_row(rx, wc, wl);
+            const C av = apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, el, er, ek);
+            if (any_out) {
+                if (all_out) {
+                    if constexpr (WOUT) stc(w, i0, av);
+                    if constexpr (UPDATE) stc(p_out, i0, cen);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) acc[0] += av.v[k] * cen.v[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+                        if (c + k >= xin0 && c + k < xin1) {
+                            if constexpr (WOUT) w[i0 + k] = av.v[k];
+                            if constexpr (UPDATE) p_out[i0 + k] = cen.v[k];
+                            acc[0] += av.v[k] * cen.v[k];
+                        }
+                }
+            }
+            dn = cen;
+            cen = up;
+            kyc = kyu;
+            wc = wu;
+        }
+    }
+    tl_reduce_partial<1, kBlock>(acc, rd);
+}
+
+// cg_calc_ur (2d/c_kernels/omp3/cg.c:116-143) with w = A p formed here from
+// p and cond as k_calc_pw_cond forms it: u += alpha p, r -= alpha w,
+// sum r r.  u and r are the reference's bit for bit; the sum differs only
+// by summation order.  alpha is the argument or, in a resident loop, the
+// loop's device value.  Same march and tiling as k_calc_pw_cond.
+__global__ void __launch_bounds__(kBlock)
+k_calc_ur_cond(PitchGrid g, int cw0, int ns, int nbx, int nby, int ty, double alpha_arg,
+               const double* __restrict__ alpha_dev, double* __restrict__ u,
+               const double* __restrict__ p, double* __restrict__ r,
+               const double* __restrict__ cond, double rx, double ry, TlReduce rd, TlGate gate)
+{
+    if (gate.closed()) return;
+    const double alpha = alpha_dev ? *alpha_dev : alpha_arg;
+    constexpr int N = 2;
+    using C = Cells<double>;
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile_of(blockIdx.x, nbx * nby);
+    const int sx = (tile % nbx) * kWaves + wave, by = tile / nbx;
+    const int xin0 = g.hd, xin1 = g.hd + g.nx, yin0 = g.hd, yin1 = g.hd + g.ny;
+    const int c = cw0 + sx * (TL_WAVE * N) + N * lane;
+    const bool ld_ok = sx < ns && c + N <= g.X;
+    const bool all_out = c >= xin0 && c + N <= xin1;
+    const bool any_out = sx < ns && c + N > xin0 && c < xin1;
+    const long X = g.X;
+    auto ld = [&](const double* __restrict__ f, long i) -> C {
+        return ld_ok ? ldc(f, i) : zeros<double>();
+    };
+    double acc[1] = {0.0};
+    const int j0 = yin0 + by * ty;
+    const int j1 = min(j0 + ty, yin1);
+    if (sx < ns && j0 < j1) {                    // uniform per wave
+        long i0 = (long)j0 * X + c;
+        C cen = ld(p, i0);
+        C dn = j0 == yin0 ? cen : ld(p, i0 - X);
+        C wc = ld(cond, i0);
+        C kyc = ky_of_rows(ry, ld(cond, i0 - X), wc);
+        for (int j = j0; j < j1; ++j, i0 += X) {
+            const C up = j == yin1 - 1 ? cen : ld(p, i0 + X);
+            const C wu = ld(cond, i0 + X);
+            const C kyu = ky_of_rows(ry, wc, wu);
+            const C uu = ld(u, i0), rr = ld(r, i0);
+            double el = 0.0, er = 0.0, ek = 0.0, wl = 0.0;
+            if (lane == 0 && c >= 1) {
+                el = p[i0 - 1];
+                wl = cond[i0 - 1];
+            }
+            if (lane == TL_WAVE - 1 && c + N < g.X) {
+                er = p[i0 + N];
+                ek = tl_face_coef(rx, wc.v[N - 1], cond[i0 + N]);
+            }
+            const C kxc = kx_of_row(rx, wc, wl);
+            const C av = apply_a_lane(cen, up, dn, kxc, kyu, kyc, c, xin0, xin1, el, er, ek);
+            C un, rn;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                un.v[k] = uu.v[k] + alpha * cen.v[k];
+                rn.v[k] = rr.v[k] - alpha * av.v[k];
+            }
+            if (any_out) {
+                if (all_out) {
+                    stc(u, i0, un);
+                    stc(r, i0, rn);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) acc[0] += rn.v[k] * rn.v[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+                        if (c + k >= xin0 && c + k < xin1) {
+                            u[i0 + k] = un.v[k];
+                            r[i0 + k] = rn.v[k];
+                            acc[0] += rn.v[k] * rn.v[k];
+                        }
+                }
+            }
+            dn = cen;
+            cen = up;
+            kyc = kyu;
+            wc = wu;
+        }
+    }
+    tl_reduce_partial<1, kBlock>(acc, rd);
+}
+
+// cond = cg_init's w (cg.c:45-53) over the whole grid: density, or
+// 1/density with the reciprocal coefficient (coefficient 2).
+__global__ void __launch_bounds__(kBlock)
+k_cond2d(long n, int coef, const double* __restrict__ density, double* __restrict__ cond)
+{
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock)
+        cond[c] = coef == 1 ? density[c] : 1.0 / density[c];
+}
+
+
 constexpr int kPwRows = 32;   // rows per wave of k_calc_pw (measured best at 4096^2)
 
 // Everything outside the interior of up to three fields, src -> dst.
@@ -1132,6 +1342,101 @@ int launch_pw(const char* what, int x, int y, int hd, int pitch, bool update, do
     return tl_check_launch(what);
 }
 
+// Tiling of the FP64 march kernels (k_calc_pw's, shared by the cond pair).
+struct March {
+    PitchGrid g;
+    int cw0, ns, nbx, nby, ty;
+    unsigned nb() const { return (unsigned)(nbx * nby); }
+};
+
+March march_geom(int x, int y, int hd)
+{
+    March m;
+    m.g = PitchGrid{x, hd, x - 2 * hd, y - 2 * hd};
+    m.cw0 = hd / 2 * 2;
+    m.ns = (int)tl_cdiv(hd + m.g.nx - m.cw0, TL_WAVE * 2);
+    m.nbx = (int)tl_cdiv(m.ns, kWaves);
+    m.ty = tl_tile_rows_override() > 0 ? tl_tile_rows_override() : kPwRows;
+    while ((long)m.nbx * tl_cdiv(m.g.ny, m.ty) > TL_MAX_RED_BLOCKS) m.ty *= 2;
+    m.nby = (int)tl_cdiv(m.g.ny, m.ty);
+    return m;
+}
+
+int check_cond_grid(const char* what, int x, int y, int hd)
+{
+    TL_TRY(tl_ensure_init());
+    if (hd < 2 || x % 2 || x < 2 * hd + 1 || y < 2 * hd + 1 || (long)x * y > (1L << 31) - 1)
+        return tl_fail(TLHIP_ERR_ARG,
+                       "%s: needs halo depth >= 2 and an even x (x=%d y=%d hd=%d)", what, x, y,
+                       hd);
+    return 0;
+}
+
+// Checks, the ring copy p_in -> p_out (with update and ring) and the
+// k_calc_pw_cond launch (w written unless null) under the current gate;
+// *nb: its partials.
+int launch_pw_cond(const char* what, int x, int y, int hd, bool update, double beta,
+                   const double* beta_dev, const double* p_in, double* p_out, const double* r,
+                   double* w, const double* cond, double rx, double ry, unsigned* nb, bool ring)
+{
+    TL_TRY(check_cond_grid(what, x, y, hd));
+    TL_REQUIRE(p_in && cond && (!update || (p_out && r)), what, "null argument");
+    TL_REQUIRE(w != p_in && w != cond && (!update || (p_out != p_in && p_out != w &&
+                                                      p_out != r && p_out != cond)),
+               what, "p_out and w must be distinct from the inputs");
+    TL_REQUIRE(all_aligned(p_in, p_out, r, w, cond), what, "fields must be 16-byte aligned");
+    const March m = march_geom(x, y, hd);
+    *nb = m.nb();
+    if (update && ring) {
+        Ring3<double> rg{};
+        rg.src[0] = p_in;
+        rg.dst[0] = p_out;
+        rg.n = 1;
+        const long nring = (long)(y - m.g.ny) * x + (long)(x - m.g.nx) * m.g.ny;
+        hipLaunchKernelGGL(k_copy_rings2d<double>, dim3(flat_blocks(nring)), dim3(kBlock), 0,
+                           tl_stream(), m.g, y, rg, tl_gate());
+        TL_TRY(tl_check_launch(what));
+    }
+    // kernel-timer label: the entry point's name when w is stored, with
+    // ".no_w" appended when w stays in registers (another byte count)
+    const int slot = tl_timer_begin(w ? "tlhip2d_cg_calc_pw_cond" : "tlhip2d_cg_calc_pw_cond.no_w");
+#define TL_PWC(U, W)                                                                          \
+    hipLaunchKernelGGL((k_calc_pw_cond<U, W>), dim3(*nb), dim3(kBlock), 0, tl_stream(), m.g,    \
+                       m.cw0, m.ns, m.nbx, m.nby, m.ty, beta, beta_dev, p_in, p_out, r, w,      \
+                       cond, rx, ry, tl_reduce_slots(), tl_gate())
+    if (update) {
+        if (w) TL_PWC(true, true);
+        else TL_PWC(true, false);
+    } else {
+        if (w) TL_PWC(false, true);
+        else TL_PWC(false, false);
+    }
+#undef TL_PWC
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
+// Checks and the k_calc_ur_cond launch under the current gate; *nb: its
+// partials.
+int launch_ur_cond(const char* what, int x, int y, int hd, double alpha,
+                   const double* alpha_dev, double* u, const double* p, double* r,
+                   const double* cond, double rx, double ry, unsigned* nb)
+{
+    TL_TRY(check_cond_grid(what, x, y, hd));
+    TL_REQUIRE(u && p && r && cond, what, "null argument");
+    TL_REQUIRE(u != p && u != r && r != p && u != cond && r != cond, what,
+               "u, p, r and cond must be distinct buffers");
+    TL_REQUIRE(all_aligned(u, p, r, cond), what, "fields must be 16-byte aligned");
+    const March m = march_geom(x, y, hd);
+    *nb = m.nb();
+    const int slot = tl_timer_begin(what);
+    hipLaunchKernelGGL(k_calc_ur_cond, dim3(*nb), dim3(kBlock), 0, tl_stream(), m.g, m.cw0, m.ns,
+                       m.nbx, m.nby, m.ty, alpha, alpha_dev, u, p, r, cond, rx, ry,
+                       tl_reduce_slots(), tl_gate());
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
 // The entry points' form: pw finished and fetched to the host (+=).
 template <class T>
 int calc_pw(const char* what, int x, int y, int hd, int pitch, bool update, double beta,
@@ -1208,6 +1513,29 @@ int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const doubl
     TL_TRY(launch_pw<double>(what, x, y, hd, x, true, 0.0, loop::beta(), p_in, p_out, r, w, kx,
                              ky, &nb, false));
     return loop::finish_alpha(nb, what);
+}
+
+// The cond pair in a resident loop: calc_pw_cond (beta from the loop with
+// update; w written unless null) with pw finished into alpha, and
+// calc_ur_cond (alpha from the loop) with rrn finished into beta and the
+// stop rule.  p_out's halo is left as calc_pw leaves it.
+int calc_pw_cond(int x, int y, int hd, bool update, const double* p_in, double* p_out,
+                 const double* r, double* w, const double* cond, double rx, double ry)
+{
+    const char* what = "tlhip2d_cg_calc_pw_cond";
+    unsigned nb = 0;
+    TL_TRY(launch_pw_cond(what, x, y, hd, update, 0.0, update ? loop::beta() : nullptr, p_in,
+                          p_out, r, w, cond, rx, ry, &nb, false));
+    return loop::finish_alpha(nb, what);
+}
+
+int calc_ur_cond(int x, int y, int hd, double* u, const double* p, double* r, const double* cond,
+                 double rx, double ry, const loop::StopRule& rule)
+{
+    const char* what = "tlhip2d_cg_calc_ur_cond";
+    unsigned nb = 0;
+    TL_TRY(launch_ur_cond(what, x, y, hd, 0.0, loop::alpha(), u, p, r, cond, rx, ry, &nb));
+    return loop::finish_beta(nb, rule, what);
 }
 
 }  // namespace loop2d
@@ -1370,3 +1698,45 @@ int cheby_iterations32(int x, int y, int hd, int P, int count, const double* alp
 
 }  // namespace mix
 }  // namespace tl
+
+// ------------------------------------------------ CG from the conductivity
+
+extern "C" int tlhip2d_cg_cond(int x, int y, int hd, int coefficient, const double* density,
+                               double* cond)
+{
+    const char* what = "tlhip2d_cg_cond";
+    TL_TRY(tl_ensure_init());
+    if (x < 1 || y < 1 || hd < 1) return tl_fail(TLHIP_ERR_ARG, "%s: bad grid", what);
+    if (coefficient != 1 && coefficient != 2)
+        return tl_fail(TLHIP_ERR_ARG, "%s: coefficient %d is not valid", what, coefficient);
+    TL_REQUIRE(density && cond && density != cond, what, "bad arguments");
+    const long n = (long)x * y;
+    hipLaunchKernelGGL(k_cond2d, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), n,
+                       coefficient, density, cond);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip2d_cg_calc_pw_cond(int x, int y, int hd, int update, double beta, double rx,
+                                       double ry, double* pw, const double* p_in, double* p_out,
+                                       const double* r, double* w, const double* cond)
+{
+    const char* what = "tlhip2d_cg_calc_pw_cond";
+    TL_REQUIRE(pw, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_pw_cond(what, x, y, hd, update != 0, beta, nullptr, p_in, p_out, r, w, cond,
+                          rx, ry, &nb, true));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return fetch_add(pw, what);
+}
+
+extern "C" int tlhip2d_cg_calc_ur_cond(int x, int y, int hd, double alpha, double rx, double ry,
+                                       double* rrn, double* u, const double* p, double* r,
+                                       const double* cond)
+{
+    const char* what = "tlhip2d_cg_calc_ur_cond";
+    TL_REQUIRE(rrn, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_ur_cond(what, x, y, hd, alpha, nullptr, u, p, r, cond, rx, ry, &nb));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return fetch_add(rrn, what);
+}

@@ -187,8 +187,8 @@ k_cg_init_fields(TlGrid2 g, int coef, double rx, double ry, const double* __rest
                                           : w[c - 1];
                 const double wd = jj >= 2 ? (coef == 1 ? density[c - g.x] : 1.0 / density[c - g.x])
                                           : w[c - g.x];
-                kx[c] = rx * (wl + wc) / (2.0 * wl * wc);
-                ky[c] = ry * (wd + wc) / (2.0 * wd * wc);
+                kx[c] = tl_face_coef(rx, wl, wc);
+                ky[c] = tl_face_coef(ry, wd, wc);
             }
         }
     }

@@ -97,6 +97,14 @@ int calc_p(int x, int y, int hd, double* p, const double* r);
 // pw finished into alpha (tlhip2d_cg_calc_pw's pass).
 int calc_pw(int x, int y, int hd, const double* p_in, double* p_out, const double* r, double* w,
             const double* kx, const double* ky);
+// The CG pair with kx and ky formed from cg_init's conductivity (cond):
+// calc_p (beta from the loop, with update) fused into calc_w, w written
+// unless null, pw finished into alpha; and cg_calc_ur forming w = A p
+// again, rrn finished into beta and the stop rule.  Halo depth >= 2, even x.
+int calc_pw_cond(int x, int y, int hd, bool update, const double* p_in, double* p_out,
+                 const double* r, double* w, const double* cond, double rx, double ry);
+int calc_ur_cond(int x, int y, int hd, double* u, const double* p, double* r, const double* cond,
+                 double rx, double ry, const loop::StopRule& rule);
 int norm(int x, int y, int hd, const double* buf, const loop::StopRule& rule);
 // One Jacobi sweep from u_in into u_out (run_jacobi_iterate without the copy
 // of u into r: the sweep reads u_in where the reference reads that copy).
