@@ -612,11 +612,11 @@ _TRAFFIC_2D = [
      "cg_calc_ur forming w = A p from p and cond: reads u, p, r, cond; writes u, r"),
     ("tlhip2d_cg_calc_pw_cond", ["k_calc_pw_cond<true, true>"], 40,
      "calc_p fused into calc_w from the conductivity (PPCG): reads p, r, cond; writes p', w"),
-    ("tlhip2d_cheby_iterations", ["k_cheby_fused2d<double, 10"], 64,
+    ("tlhip2d_cheby_iterations", ["k_ppcg_ring2d<double, 10, 3,"], 64,
      "10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
     ("tlhip2d_ppcg_inner_iterations_f32 (from r)", ["k_ppcg_ring2d<float, 10, 2, false>"], 36,
      "FP32 fields, cg_calc_ur + ppcg_init + 10 inner steps"),
-    ("tlhip2d_cheby_iterations_f32", ["k_cheby_fused2d<float, 10"], 32,
+    ("tlhip2d_cheby_iterations_f32", ["k_ppcg_ring2d<float, 10, 3,"], 32,
      "FP32 fields, 10 Chebyshev iterations: reads u, u0, p, kx, ky; writes u', p', r"),
     ("tlhip2d_ppcg_inner_iteration", ["k_stencil2d<2, tl::OpPpcgInner>"], 64,
      "one unfused inner step: reads sd, kx, ky, r, u; writes sd', r, u"),
