@@ -254,6 +254,18 @@ int tlhip2d_cg_calc_pw_cond(int x, int y, int halo_depth, int update, double bet
         double* w, const double* cond);
 int tlhip2d_cg_calc_ur_cond(int x, int y, int halo_depth, double alpha, double rx, double ry,
         double* rrn, double* u, const double* p, double* r, const double* cond);
+/* The same three in 3D (3d/c_kernels/omp3/cg_solver.c:47-75 for cond and
+ * the face coefficients; run_cg_solver_calc_p, _calc_w and _calc_ur,
+ * 3d/kernel_interface.h:30-36): kx, ky and kz formed from cond, one field
+ * in place of three.  Halo depth >= 2 and an even x. */
+int tlhip3d_cg_cond(int x, int y, int z, int halo_depth, int coefficient,
+        const double* density, double* cond);
+int tlhip3d_cg_calc_pw_cond(int x, int y, int z, int halo_depth, int update, double beta,
+        double rx, double ry, double rz, double* pw, const double* p_in, double* p_out,
+        const double* r, double* w, const double* cond);
+int tlhip3d_cg_calc_ur_cond(int x, int y, int z, int halo_depth, double alpha, double rx,
+        double ry, double rz, double* rrn, double* u, const double* p, double* r,
+        const double* cond);
 /* Temporally blocked Chebyshev loop, 1 <= count <= 10 iterations in one
  * pass.  No single counterpart in the kernel interface: it replaces `count`
  * turns of the Chebyshev loop of 2d/drivers/cheby_driver.c (a depth-1 halo

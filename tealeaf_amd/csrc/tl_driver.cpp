@@ -129,8 +129,8 @@ struct tlhip_problem {
     double *u = nullptr, *u0 = nullptr, *p = nullptr, *r = nullptr, *mi = nullptr, *w = nullptr;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr, *sd = nullptr;
     double *u_alt = nullptr, *sd_alt = nullptr, *r_alt = nullptr, *p_alt = nullptr;
-    double* cond = nullptr;   // 2D: cg_init's conductivity w, for the cond CG passes
-    double rx = 0.0, ry = 0.0;
+    double* cond = nullptr;   // cg_init's conductivity w, for the cond CG passes
+    double rx = 0.0, ry = 0.0, rz = 0.0;
     double *volume = nullptr, *x_area = nullptr, *y_area = nullptr, *z_area = nullptr;
     double *cell_x = nullptr, *cell_y = nullptr, *cell_z = nullptr;
     double *vertex_x = nullptr, *vertex_y = nullptr, *vertex_z = nullptr;
@@ -220,17 +220,19 @@ struct tlhip_problem {
     }
 
     // -------------------------------------------------------- CG (cg_driver.c)
-    int cg_init_driver(double rx_, double ry_, double rz, double* rro)
+    int cg_init_driver(double rx_, double ry_, double rz_, double* rro)
     {
         *rro = 0.0;
         rx = rx_;
         ry = ry_;
+        rz = rz_;
         if (dims == 2) {
             TRY(tlhip2d_cg_init(x, y, hd, s.coefficient, rx, ry, rro, density, energy, u, p, r, w, kx, ky));
             if (cond_ok()) TRY(tlhip2d_cg_cond(x, y, hd, s.coefficient, density, cond));
         } else {
             TRY(tlhip3d_cg_solver_init(x, y, z, hd, s.coefficient, rx, ry, rz, rro, density,
                                        energy, u, p, r, w, kx, ky, kz));
+            if (cond3_ok()) TRY(tlhip3d_cg_cond(x, y, z, hd, s.coefficient, density, cond));
         }
         reset_fields();
         fields[F_U] = fields[F_P] = true;
@@ -262,6 +264,32 @@ struct tlhip_problem {
                (s.coefficient == 1 || s.coefficient == 2);
     }
 
+    // 3D: the CG steps as the 3D cond pair, calc_p deferred into the next
+    // calc_pw (the pending calc_p applied as calc_p + halo update before
+    // any other reader of p: PPCG's calc_w, Chebyshev's start).
+    bool cond3_ok() const
+    {
+        return g_cg_cond && g_cg_fusion && dims == 3 && x % 2 == 0 && hd >= 2 && cond &&
+               p_alt && (s.coefficient == 1 || s.coefficient == 2);
+    }
+
+    int cg_step_cond3(double* rro, double* alpha, double* rrn)
+    {
+        double pw = 0.0;
+        if (pending_p) {
+            pending_p = false;
+            TRY(tlhip3d_cg_calc_pw_cond(x, y, z, hd, 1, pending_beta, rx, ry, rz, &pw, p, p_alt,
+                                        r, nullptr, cond));
+            std::swap(p, p_alt);
+        } else {
+            TRY(tlhip3d_cg_calc_pw_cond(x, y, z, hd, 0, 0.0, rx, ry, rz, &pw, p, nullptr,
+                                        nullptr, nullptr, cond));
+        }
+        *alpha = *rro / pw;
+        *rrn = 0.0;
+        return tlhip3d_cg_calc_ur_cond(x, y, z, hd, *alpha, rx, ry, rz, rrn, u, p, r, cond);
+    }
+
     // cg_main_step's calc_w (or pending calc_p + calc_w) and calc_ur as the
     // cond pair, alpha and beta on the host.
     int cg_step_cond(double* rro, double* alpha, double* rrn)
@@ -284,6 +312,10 @@ struct tlhip_problem {
     int calc_w_pending(double* pw)
     {
         if (!pending_p) return calc_w(pw);
+        if (dims == 3) {
+            TRY(flush_p());
+            return calc_w(pw);
+        }
         pending_p = false;
         if (cond_ok())
             TRY(tlhip2d_cg_calc_pw_cond(x, y, hd, 1, pending_beta, rx, ry, pw, p, p_alt, r, w,
@@ -296,7 +328,7 @@ struct tlhip_problem {
 
     int calc_p_deferred(double beta)
     {
-        if (!can_defer_p()) return calc_p(beta);
+        if (!can_defer_p() && !cond3_ok()) return calc_p(beta);
         pending_p = true;
         pending_beta = beta;
         return 0;
@@ -322,6 +354,8 @@ struct tlhip_problem {
         double alpha = 0.0, rrn = 0.0;
         if (cond_ok()) {
             TRY(cg_step_cond(rro, &alpha, &rrn));
+        } else if (cond3_ok()) {
+            TRY(cg_step_cond3(rro, &alpha, &rrn));
         } else {
             double pw = 0.0;
             TRY(calc_w_pending(&pw));
@@ -464,6 +498,19 @@ struct tlhip_problem {
             TRY(tl::loop2d::calc_ur_cond(x, y, hd, u, p, r, cond, rx, ry, rule));
             pending_p = true;
             return 0;
+        } else if (cond3_ok()) {
+            if (pending_p) {
+                pending_p = false;
+                TRY(tl::loop3d::calc_pw_cond(x, y, z, hd, true, p, p_alt, r, nullptr, cond, rx, ry,
+                                             rz));
+                std::swap(p, p_alt);
+            } else {
+                TRY(tl::loop3d::calc_pw_cond(x, y, z, hd, false, p, nullptr, nullptr, nullptr,
+                                             cond, rx, ry, rz));
+            }
+            TRY(tl::loop3d::calc_ur_cond(x, y, z, hd, u, p, r, cond, rx, ry, rz, rule));
+            pending_p = true;
+            return 0;            // p is mirrored, u read pointwise: halo at the end
         } else if (can_defer_p()) {
             TRY(calc_w_resident());
             TRY(tl::loop2d::calc_ur(x, y, hd, u, p, r, w, rule));
@@ -575,6 +622,7 @@ struct tlhip_problem {
     // (3d/drivers/ppcg_solver_driver.c:88-140).
     int ppcg_body3(const tl::loop::StopRule& rule)
     {
+        TRY(flush_p());          // the CG phase's last calc_p, when deferred
         TRY(tl::loop3d::calc_w(x, y, z, hd, p, w, kx, ky, kz));
         TRY(tl::loop3d::calc_ur_only(x, y, z, hd, u, p, r, w));
         TRY(tlhip3d_ppcg_init(x, y, z, hd, theta, r, sd));
@@ -739,7 +787,7 @@ struct tlhip_problem {
             tl::loop::Report rp;
             TRY(run_resident(from, sw, true,
                              [&](const tl::loop::StopRule& rule, int) { return cg_body(rule); }, rro,
-                             error, &last, &rp, true, can_defer_p()));
+                             error, &last, &rp, true, can_defer_p() || cond3_ok()));
             rep->n_cg_steps += last - from + 1;
             *tt = last;
             TRY(confirm(rp, done));
@@ -1687,11 +1735,12 @@ tlhip_problem* new_problem(const tlhip_settings* s)
 // Partners the driver's fused paths write into.
 void alloc_partners(tlhip_problem* P, int* rc)
 {
-    // partner of p: the 2D fused calc_p + calc_w and fused Chebyshev
-    if (P->dims == 2) P->p_alt = dalloc(P->n, rc);
-    // cg_init's conductivity for the 2D cond CG passes (CG, Chebyshev, PPCG)
-    if (P->dims == 2 && P->hd >= 2 && P->s.solver >= 1 && P->s.solver <= 3)
-        P->cond = dalloc(P->n, rc);
+    // cg_init's conductivity for the cond CG passes (CG, Chebyshev, PPCG)
+    const bool cg = P->hd >= 2 && P->s.solver >= 1 && P->s.solver <= 3;
+    if (cg) P->cond = dalloc(P->n, rc);
+    // partner of p: the fused calc_p + calc_w (2D; 3D with cond) and the
+    // 2D fused Chebyshev loop
+    if (P->dims == 2 || cg) P->p_alt = dalloc(P->n, rc);
     // partner of r: the fused PPCG inner loops (2D ring, 3D two-step launch)
     if (P->s.solver == 3) P->r_alt = dalloc(P->n, rc);
 }

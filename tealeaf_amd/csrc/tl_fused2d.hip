@@ -771,29 +771,6 @@ k_calc_pw_rows(PitchGrid g, int cw0, int nbx, int nby, int ty, T beta, const T* 
 }
 
 
-// kx of a lane's cells on one row and ky of two rows' faces, formed from
-// the conductivity w (cg_init's w: density, or 1/density with the
-// reciprocal coefficient; 2d/c_kernels/omp3/cg.c:45-53): kx from the row's
-// w (wc) and the left neighbour of the wave's first cell (wl_edge), ky from
-// the row below (wd) and the row (wc).
-__device__ __forceinline__ Cells<double> kx_of_row(double rx, const Cells<double>& wc,
-                                                   double wl_edge)
-{
-    Cells<double> k;
-    k.v[0] = tl_face_coef(rx, tl_lane_prev(wc.v[1], wl_edge), wc.v[0]);
-    k.v[1] = tl_face_coef(rx, wc.v[0], wc.v[1]);
-    return k;
-}
-
-__device__ __forceinline__ Cells<double> ky_of_rows(double ry, const Cells<double>& wd,
-                                                    const Cells<double>& wc)
-{
-    Cells<double> k;
-    k.v[0] = tl_face_coef(ry, wd.v[0], wc.v[0]);
-    k.v[1] = tl_face_coef(ry, wd.v[1], wc.v[1]);
-    return k;
-}
-
 // The CG pair of passes with kx and ky formed from the cells' conductivity
 // instead of loaded: cond holds cg_init's w and every face coefficient is
 // cg_init's expression on the same operands (tl_face_coef), so the stencil

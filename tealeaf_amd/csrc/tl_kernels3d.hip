@@ -1300,6 +1300,248 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
     }
 }
 
+// The CG pair in 3D with kx, ky and kz formed from cg_init's conductivity
+// (cond: density, or 1/density with the reciprocal coefficient;
+// 3d/c_kernels/omp3/cg_solver.c:47-75) instead of loaded, each face
+// coefficient with cg_init's expression on the same operands (tl_face_coef):
+// one 8-byte field replaces three.  A block is kC3Rows waves, one per row j0
+// .. j0+kC3Rows-1 of a 128-column window (122 columns written, windows
+// overlap as in the two-step kernel), marching up a tile of planes.  Per
+// plane every wave publishes its row of the stencil field and of cond to LDS
+// (double-buffered by plane parity: one barrier per plane); its y
+// neighbours come from the adjacent waves (the block's first and last waves
+// load the rows beyond the block), its z neighbours and kz from its own
+// planes k-1, k, k+1 in registers.  Neighbours outside the interior are the
+// cell (the depth-1 reflective halo update before each pass).
+// UR false: calc_p fused into calc_w: the stencil field is p' = beta p + r
+// with UPDATE (stored to p_out; cg_solver.c:172-194 then :105-135), else
+// p; w = A p' stored unless w is null; sum w p'.
+// UR true: cg_calc_ur (cg_solver.c:138-170) with w = A p formed here:
+// u += alpha p, r -= alpha w, sum r r.
+// Every cell's values are the reference's bits; the sums differ only by
+// summation order.  Needs halo depth >= 2 (cond is read at hd-1 .. n-hd)
+// and an even x.
+constexpr int kC3Rows = 8;
+
+template <bool UR, bool UPDATE>
+__global__ void __launch_bounds__(kC3Rows * TL_WAVE)
+k_cg3_cond(TlGrid3 g, int nwx, int nby, int nbz, int tz, double sc_arg,
+           const double* __restrict__ sc_dev, const double* __restrict__ p,
+           double* __restrict__ p_out, double* __restrict__ r, double* __restrict__ w,
+           double* __restrict__ u, const double* __restrict__ cond, double rx, double ry,
+           double rz, TlReduce rd, TlGate gate)
+{
+    constexpr int R = kC3Rows;
+    __shared__ C2 s_f[2][R][TL_WAVE];
+    __shared__ C2 s_w[2][R][TL_WAVE];
+    if (gate.closed()) return;                   // uniform per launch
+    const double sc = sc_dev ? *sc_dev : sc_arg; // beta (UPDATE) or alpha (UR)
+    const int lane = threadIdx.x & (TL_WAVE - 1), wave = threadIdx.x / TL_WAVE;
+    const int tile = xcd_tile(blockIdx.x, nwx * nby * nbz);
+    const int wx = tile % nwx, rest = tile / nwx;
+    const int by = rest % nby, bz = rest / nby;
+    const int xin0 = g.hd, xin1 = g.x - g.hd, yin0 = g.hd, yin1 = g.y - g.hd;
+    const int zin0 = g.hd, zin1 = g.z - g.hd;
+    const int jw = yin0 + by * R + wave;         // this wave's row
+    const bool row_ok = jw < yin1;
+    const int o0 = xin0 + wx * kP2Out, o1 = min(o0 + kP2Out, xin1);
+    const int cw = (o0 - 2) >= 0 ? (o0 - 2) / 2 * 2 : -((3 - o0) / 2 * 2);
+    const int c = cw + 2 * lane;
+    const bool st0 = c >= o0 && c < o1, st1 = c + 1 >= o0 && c + 1 < o1;
+    const int k0 = zin0 + bz * tz, k1 = min(k0 + tz, zin1);
+    const long X = g.x, XY = (long)g.x * g.y;
+    // Loads without lane-divergent branches: a lane past the row's end
+    // reads the row's last pair (it writes nothing, and no written cell
+    // takes its values: the last interior cell mirrors), a row past the
+    // grid the grid's last row (only for rows that write nothing); rows
+    // jw-1 >= hd-1 and planes k0-1 .. k1+1 lie in the grid.
+    const int cc = min(c, g.x - 2);
+    auto ldr = [&](const double* __restrict__ f, int j, int k) -> C2 {
+        return ldc(f, ((long)k * g.y + min(j, g.y - 1)) * X + cc);
+    };
+    // the stencil field from its raw loads: p' = beta p + r with UPDATE, p
+    struct Raw {
+        C2 p, r;
+    };
+    auto raw = [&](int j, int k) -> Raw {
+        Raw a{ldr(p, j, k), zeros<double>()};
+        if constexpr (UPDATE) a.r = ldr(r, j, k);
+        return a;
+    };
+    auto comb = [&](const Raw& a) -> C2 {
+        C2 o = a.p;
+        if constexpr (UPDATE) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) o.v[i] = sc * a.p.v[i] + a.r.v[i];
+        }
+        return o;
+    };
+    auto pick = [](bool t, const C2& a, const C2& b) -> C2 {
+        C2 o;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) o.v[i] = t ? a.v[i] : b.v[i];
+        return o;
+    };
+    const int wso = max(wave - 1, 0), wno = min(wave + 1, R - 1);
+    const int jo = wave == 0 ? jw - 1 : jw + 1;  // the row beyond the block (first, last wave)
+    const bool edge = wave == 0 || wave == R - 1;
+    C2 fc = comb(raw(jw, k0));
+    C2 fm = pick(k0 == zin0, fc, comb(raw(jw, k0 - 1)));
+    C2 wc = ldr(cond, jw, k0);
+    C2 kzc = ky_of_rows(rz, ldr(cond, jw, k0 - 1), wc);
+    // loads run one plane ahead: the own row's plane kk+2 and the outer
+    // row's plane kk+1 are in flight while plane kk is formed (planes up to
+    // k1+1 <= z-hd+1 lie in the grid for halo depth >= 2)
+    Raw rp = raw(jw, k0 + 1), ro = edge ? raw(jo, k0) : Raw{fc, zeros<double>()};
+    C2 wp = ldr(cond, jw, k0 + 1), wo = edge ? ldr(cond, jo, k0) : wc;
+    C2 uu = zeros<double>(), rr = zeros<double>();   // UR: u and r of plane kk, one ahead
+    if constexpr (UR) {
+        uu = ldr(u, jw, k0);
+        rr = ldr(r, jw, k0);
+    }
+    double acc[1] = {0.0};
+    for (int kk = k0; kk < k1; ++kk) {
+        const int b = kk & 1;
+        const Raw rp_k = rp, ro_k = ro;
+        const C2 wp_k = wp, wo_k = wo, uu_k = uu, rr_k = rr;
+        rp = raw(jw, kk + 2);
+        wp = ldr(cond, jw, kk + 2);
+        if constexpr (UR) {
+            uu = ldr(u, jw, kk + 1);
+            rr = ldr(r, jw, kk + 1);
+        }
+        if (edge) {                              // uniform per wave
+            ro = raw(jo, kk + 1);
+            wo = ldr(cond, jo, kk + 1);
+        }
+        s_f[b][wave][lane] = fc;
+        s_w[b][wave][lane] = wc;
+        __syncthreads();
+        const C2 fs = s_f[b][wso][lane], fn = s_f[b][wno][lane];
+        const C2 ws_l = s_w[b][wso][lane], wn_l = s_w[b][wno][lane];
+        const C2 fp = comb(rp_k);
+        if (row_ok) {                            // uniform per wave
+            const C2 fo = comb(ro_k);
+            const C2 so = pick(jw == yin0, fc, pick(wave == 0, fo, fs));
+            const C2 no = pick(jw == yin1 - 1, fc, pick(wave == R - 1, fo, fn));
+            const C2 ws = pick(wave == 0, wo_k, ws_l);
+            const C2 wn = pick(wave == R - 1, wo_k, wn_l);
+            const C2 up = pick(kk == zin1 - 1, fc, fp);
+            const C2 dn = pick(kk == zin0, fc, fm);
+            const C2 wpk = wp_k;
+            const C2 kxc = kx_of_row(rx, wc, wc.v[0]);   // the wave's first lane writes nothing
+            const C2 kyc = ky_of_rows(ry, ws, wc), kyn = ky_of_rows(ry, wc, wn);
+            const C2 kzu = ky_of_rows(rz, wc, wpk);
+            const C2 av = apply7(fc, no, so, up, dn, kxc, kyc, kyn, kzc, kzu, c, xin0, xin1);
+            const long i0 = (long)kk * XY + (long)jw * X + c;
+            if constexpr (!UR) {
+                if (st0 && st1) {
+                    if (w) stc(w, i0, av);
+                    if constexpr (UPDATE) stc(p_out, i0, fc);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) acc[0] += av.v[i] * fc.v[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+                        if (i == 0 ? st0 : st1) {
+                            if (w) w[i0 + i] = av.v[i];
+                            if constexpr (UPDATE) p_out[i0 + i] = fc.v[i];
+                            acc[0] += av.v[i] * fc.v[i];
+                        }
+                }
+            } else {
+                C2 un, rn;
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    un.v[i] = uu_k.v[i] + sc * fc.v[i];
+                    rn.v[i] = rr_k.v[i] - sc * av.v[i];
+                }
+                if (st0 && st1) {
+                    stc(u, i0, un);
+                    stc(r, i0, rn);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) acc[0] += rn.v[i] * rn.v[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+                        if (i == 0 ? st0 : st1) {
+                            u[i0 + i] = un.v[i];
+                            r[i0 + i] = rn.v[i];
+                            acc[0] += rn.v[i] * rn.v[i];
+                        }
+                }
+            }
+            kzc = kzu;
+        }
+        fm = fc;
+        fc = fp;
+        wc = wp_k;
+    }
+    tl_reduce_partial<1, kC3Rows * TL_WAVE>(acc, rd);
+}
+
+// cond = cg_init's w over the whole grid (cg_solver.c:47-59).
+__global__ void __launch_bounds__(kBlock)
+k_cond3d(long n, int coef, const double* __restrict__ density, double* __restrict__ cond)
+{
+    for (long c = blockIdx.x * (long)kBlock + threadIdx.x; c < n; c += (long)gridDim.x * kBlock)
+        cond[c] = coef == 1 ? density[c] : 1.0 / density[c];
+}
+
+// Checks and the k_cg3_cond launch under the current gate (ring: p_in's
+// halo ring copied to p_out first, as the entry point's form keeps it);
+// *nb: its partials.
+int launch_cg3_cond(const char* what, const TlGrid3& g, bool ur, bool update, double sc,
+                    const double* sc_dev, const double* p, double* p_out, double* r, double* w,
+                    double* u, const double* cond, double rx, double ry, double rz,
+                    unsigned* nb, bool ring)
+{
+    TL_TRY(tl_ensure_init());
+    TL_TRY(check_grid(g, what));
+    if (g.hd < 2 || g.x % 2)
+        return tl_fail(TLHIP_ERR_ARG, "%s: needs halo depth >= 2 and an even x (x=%d hd=%d)",
+                       what, g.x, g.hd);
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    REQUIRE(p && cond, what, "null argument");
+    if (ur) {
+        REQUIRE(u && r, what, "null argument");
+        REQUIRE(u != p && u != r && r != p && u != cond && r != cond, what,
+                "u, p, r and cond must be distinct buffers");
+    } else if (update) {
+        REQUIRE(p_out && r, what, "null argument");
+        REQUIRE(p_out != p && p_out != r && p_out != cond && p_out != w && w != p && w != r &&
+                    w != cond, what, "p_out and w must be distinct from the inputs");
+    } else {
+        REQUIRE(w != p && w != cond, what, "w must be distinct from the inputs");
+    }
+    for (const double* f : {p, (const double*)p_out, (const double*)r, (const double*)w,
+                            (const double*)u, cond})
+        REQUIRE(((uintptr_t)f & 15) == 0, what, "fields must be 16-byte aligned");
+    const int nx = g.x - 2 * g.hd, ny = g.y - 2 * g.hd, nz = g.z - 2 * g.hd;
+    // planes per tile: tlhip_set_tile_planes when set, else 16 (256^3:
+    // 0.160 + 0.189 ms for the pair against 0.175 + 0.195 at 8 planes and
+    // 0.167 + 0.213 at 32)
+    const int tz = g_tz_override > 0 ? g_tz_override : 16;
+    const int nwx = (int)tl_cdiv(nx, kP2Out), nby = (int)tl_cdiv(ny, kC3Rows);
+    const int nbz = (int)tl_cdiv(nz, tz);
+    const long nbl = (long)nwx * nby * nbz;
+    if (nbl > TL_MAX_RED_BLOCKS) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    *nb = (unsigned)nbl;
+    if (!ur && update && ring) TL_TRY(copy_ring(g, p, p_out, what));
+    const char* label = ur ? what : (w ? what : "tlhip3d_cg_calc_pw_cond.no_w");
+    const int slot = tl_timer_begin(label);
+#define TL_C3(U, P)                                                                            \
+    hipLaunchKernelGGL((k_cg3_cond<U, P>), dim3(*nb), dim3(kC3Rows * TL_WAVE), 0, tl_stream(), \
+                       g, nwx, nby, nbz, tz, sc, sc_dev, p, p_out, r, w, u, cond, rx, ry, rz,  \
+                       tl_reduce_slots(), tl_gate())
+    if (ur) TL_C3(true, false);
+    else if (update) TL_C3(false, true);
+    else TL_C3(false, false);
+#undef TL_C3
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
 }  // namespace
 
 extern "C" int tlhip3d_ppcg_inner_iteration(int x, int y, int z, int hd, double alpha,
@@ -1537,5 +1779,72 @@ int norm_err(int x, int y, int z, int hd, const double* buf, const loop::StopRul
     return loop::finish_err(nb, loop::kErrAddTest, rule, what);
 }
 
+// The cond pair in a resident loop (tl_loop.h).
+int calc_pw_cond(int x, int y, int z, int hd, bool update, const double* p_in, double* p_out,
+                 const double* r, double* w, const double* cond, double rx, double ry, double rz)
+{
+    const char* what = "tlhip3d_cg_calc_pw_cond";
+    unsigned nb = 0;
+    TL_TRY(launch_cg3_cond(what, TlGrid3{x, y, z, hd}, false, update, 0.0,
+                           update ? loop::beta() : nullptr, p_in, p_out, (double*)r, w, nullptr,
+                           cond, rx, ry, rz, &nb, false));
+    return loop::finish_alpha(nb, what);
+}
+
+int calc_ur_cond(int x, int y, int z, int hd, double* u, const double* p, double* r,
+                 const double* cond, double rx, double ry, double rz,
+                 const loop::StopRule& rule)
+{
+    const char* what = "tlhip3d_cg_calc_ur_cond";
+    unsigned nb = 0;
+    TL_TRY(launch_cg3_cond(what, TlGrid3{x, y, z, hd}, true, false, 0.0, loop::alpha(), p,
+                           nullptr, r, nullptr, u, cond, rx, ry, rz, &nb, false));
+    return loop::finish_beta(nb, rule, what);
+}
+
 }  // namespace loop3d
 }  // namespace tl
+
+// ------------------------------------------------ CG from the conductivity
+
+extern "C" int tlhip3d_cg_cond(int x, int y, int z, int hd, int coefficient,
+                               const double* density, double* cond)
+{
+    const char* what = "tlhip3d_cg_cond";
+    TL_TRY(tl_ensure_init());
+    if (x < 1 || y < 1 || z < 1 || hd < 1) return tl_fail(TLHIP_ERR_ARG, "%s: bad grid", what);
+    if (coefficient != 1 && coefficient != 2)
+        return tl_fail(TLHIP_ERR_ARG, "%s: coefficient %d is not valid", what, coefficient);
+    REQUIRE(density && cond && density != cond, what, "bad arguments");
+    const long n = (long)x * y * z;
+    hipLaunchKernelGGL(k_cond3d, dim3(flat_blocks(n)), dim3(kBlock), 0, tl_stream(), n,
+                       coefficient, density, cond);
+    return tl_check_launch(what);
+}
+
+extern "C" int tlhip3d_cg_calc_pw_cond(int x, int y, int z, int hd, int update, double beta,
+                                       double rx, double ry, double rz, double* pw,
+                                       const double* p_in, double* p_out, const double* r,
+                                       double* w, const double* cond)
+{
+    const char* what = "tlhip3d_cg_calc_pw_cond";
+    REQUIRE(pw, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_cg3_cond(what, TlGrid3{x, y, z, hd}, false, update != 0, beta, nullptr, p_in,
+                           p_out, (double*)r, w, nullptr, cond, rx, ry, rz, &nb, true));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return fetch_add(pw, what);
+}
+
+extern "C" int tlhip3d_cg_calc_ur_cond(int x, int y, int z, int hd, double alpha, double rx,
+                                       double ry, double rz, double* rrn, double* u,
+                                       const double* p, double* r, const double* cond)
+{
+    const char* what = "tlhip3d_cg_calc_ur_cond";
+    REQUIRE(rrn, what, "null argument");
+    unsigned nb = 0;
+    TL_TRY(launch_cg3_cond(what, TlGrid3{x, y, z, hd}, true, false, alpha, nullptr, p, nullptr,
+                           r, nullptr, u, cond, rx, ry, rz, &nb, false));
+    TL_TRY(tl_finish_reduce(1, nb, what));
+    return fetch_add(rrn, what);
+}

@@ -152,6 +152,29 @@ __device__ __forceinline__ Cells<T> diag_lane(const Cells<T>& kxc, T kxe, const 
     return d;
 }
 
+// kx of a lane's cells on one row and ky of two rows' faces, formed from
+// the conductivity w (cg_init's w: density, or 1/density with the
+// reciprocal coefficient; 2d/c_kernels/omp3/cg.c:45-53): kx from the row's
+// w (wc) and the left neighbour of the wave's first cell (wl_edge), ky from
+// the row below (wd) and the row (wc); kz likewise from two planes.
+__device__ __forceinline__ Cells<double> kx_of_row(double rx, const Cells<double>& wc,
+                                                   double wl_edge)
+{
+    Cells<double> k;
+    k.v[0] = tl_face_coef(rx, tl_lane_prev(wc.v[1], wl_edge), wc.v[0]);
+    k.v[1] = tl_face_coef(rx, wc.v[0], wc.v[1]);
+    return k;
+}
+
+__device__ __forceinline__ Cells<double> ky_of_rows(double ry, const Cells<double>& wd,
+                                                    const Cells<double>& wc)
+{
+    Cells<double> k;
+    k.v[0] = tl_face_coef(ry, wd.v[0], wc.v[0]);
+    k.v[1] = tl_face_coef(ry, wd.v[1], wc.v[1]);
+    return k;
+}
+
 // a * b + c.  FP64: two roundings, the reference's arithmetic.  FP32 (the
 // sweeps of the FP32-refined solve, which FP64 residual replacement
 // corrects): one fused multiply-add, fewer instructions in the issue-bound

@@ -138,6 +138,12 @@ int calc_w(int x, int y, int z, int hd, const double* p, double* w, const double
 int calc_ur(int x, int y, int z, int hd, double* u, const double* p, double* r, const double* w,
             const loop::StopRule& rule);
 int calc_p(int x, int y, int z, int hd, double* p, const double* r);
+// The 3D cond pair (loop2d's, with kz): halo depth >= 2, even x.
+int calc_pw_cond(int x, int y, int z, int hd, bool update, const double* p_in, double* p_out,
+                 const double* r, double* w, const double* cond, double rx, double ry, double rz);
+int calc_ur_cond(int x, int y, int z, int hd, double* u, const double* p, double* r,
+                 const double* cond, double rx, double ry, double rz,
+                 const loop::StopRule& rule);
 // cg_calc_ur with alpha from the device and no finisher (PPCG's outer step,
 // whose rrn the driver replaces by the 2-norm after the inner loop).
 int calc_ur_only(int x, int y, int z, int hd, double* u, const double* p, double* r,
