@@ -487,10 +487,11 @@ typedef struct tlhip_problem tlhip_problem;
  * Results are identical either way.  Default on: at 4096^2 on MI355X the
  * fused pass (48 B/cell, 0.162 ms) takes 9% less than the two (0.177 ms). */
 int tlhip_set_cg_fusion(int on);
-/* Whether the driver's 2D CG steps (CG, and the CG phase of Chebyshev and
- * PPCG; halo depth >= 2, with tlhip_set_cg_fusion on) run as the cond pair
- * (tlhip2d_cg_calc_pw_cond without storing w, then tlhip2d_cg_calc_ur_cond)
- * and PPCG's calc_p + calc_w reads cond in place of kx and ky.  u, p and r
+/* Whether the driver's CG steps (CG, and the CG phase of Chebyshev and
+ * PPCG; 2D and 3D, halo depth >= 2, even x, with tlhip_set_cg_fusion on)
+ * run as the cond pair (tlhip2d/3d_cg_calc_pw_cond without storing w, then
+ * tlhip2d/3d_cg_calc_ur_cond) and 2D PPCG's calc_p + calc_w reads cond in
+ * place of kx and ky.  u, p and r
  * are the reference's expressions either way; rrn is summed in another
  * order, so results agree to rounding.  Default on: at 4096^2 on MI355X a
  * CG step's two passes take 0.284 ms against 0.333 ms (32 + 48 B/cell
