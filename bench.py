@@ -123,17 +123,23 @@ def gpu_run(args):
     total_time = 0.0
     total_iters = 0
     kernel = {}
-    # the CG pair (every leg: CG, and the CG phase of Chebyshev and PPCG) and
-    # the leg's own passes; each timed launch adds two event records to the
-    # stream, so the small rest (2-norms, finishers, halo and ring copies) is
-    # left untimed
-    own = {"cg": [], "cheby": ["tlhip2d_cheby_iterations"],
+    # Each timed launch adds two event records to the stream (about 4% of a
+    # CG iteration), so: the fused PPCG launch (the kernel with the most time)
+    # and PPCG's calc_pw in the PPCG leg, the fused Chebyshev launch and the
+    # CG pair in the Chebyshev leg (its 1559 CG iterations; the same kernels
+    # at the same size run every CG iteration of all three legs), nothing
+    # in the CG leg; the small rest (2-norms, finishers, halo and ring
+    # copies) untimed.
+    own = {"cg": [],
+           "cheby": ["tlhip2d_cheby_iterations", "tlhip2d_cg_calc_pw_cond.no_w",
+                     "tlhip2d_cg_calc_ur_cond"],
            "ppcg": [dominant(args), "tlhip2d_cg_calc_pw_cond"]}
     for s in solvers:
-        timed = ["tlhip2d_cg_calc_pw_cond.no_w", "tlhip2d_cg_calc_ur_cond"] + own[s]
+        timed = own[s]
         L.tlhip_synchronize()
         torch.cuda.synchronize()
-        L.tlhip_kernel_timer_start(",".join(timed).encode())
+        if timed:
+            L.tlhip_kernel_timer_start(",".join(timed).encode())
         t0 = time.perf_counter()
         reps = [probs[s].step(tt) for tt in range(args.warmup, total_steps)]
         L.tlhip_synchronize()
@@ -148,7 +154,9 @@ def gpu_run(args):
             k["total_ms"] += ms.value
             if nl.value:
                 k["by_solver"][s] = {"launches": nl.value, "ms": ms.value}
-        L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))   # stop
+        if timed:
+            ms, nl = ctypes.c_double(), ctypes.c_long()
+            L.tlhip_kernel_timer_read(ctypes.addressof(ms), ctypes.addressof(nl))   # stop
         printed = sum(r["printed_cg_iters"] + r["printed_inner_iters"] for r in reps)
         sweeps = {"cg_steps": sum(r["n_cg_steps"] for r in reps),
                   "cheby_steps": sum(r["n_cheby_steps"] for r in reps),
@@ -306,8 +314,9 @@ def profile_run(args):
     Chebyshev loops, calc_w, calc_pw, calc_ur, calc_p, the 2-norm,
     field_summary, cg_init and the single-step PPCG sweep; the CG pair from
     the conductivity and PPCG's calc_pw from it; the FP32 fused
-    PPCG and Chebyshev loops), 3D at 256^3 (calc_w, calc_ur, calc_p, the PPCG inner sweep, the
-    two-step PPCG sweep, the 2-norm, field_summary, cg_init) -- then one timestep of the FP32-refined
+    PPCG and Chebyshev loops), 3D at 256^3 (calc_w, calc_ur, calc_p, the CG pair from the
+    conductivity, the PPCG inner sweep, the two-step PPCG sweep, the 2-norm, field_summary,
+    cg_init) -- then one timestep of the FP32-refined
     2D PPCG solve at 2048^2 for the FP32 CG kernels, and a 1 GiB device copy
     (8 B read + 8 B written per element) whose FETCH_SIZE / WRITE_SIZE
     calibrate the counters."""
@@ -407,6 +416,8 @@ def profile_run(args):
     g["rho"] = DeviceArray.from_host(rng.uniform(0.5, 2.0, N3))
     g["e"] = DeviceArray.from_host(rng.uniform(0.1, 1.0, N3))
     g["vol"] = DeviceArray.from_host(np.full(N3, 1e-6))
+    g["cond"] = DeviceArray(N3, np.float64)
+    L.tlhip3d_cg_cond(X3, X3, X3, hd, 2, g["rho"].ptr, g["cond"].ptr)
     nb6 = np.full(6, -1, dtype=np.int32)
     for _ in range(3):
         L.tlhip3d_cg_solver_calc_w(X3, X3, X3, hd, C.addressof(pw), g["p"].ptr, g["w"].ptr,
@@ -414,6 +425,10 @@ def profile_run(args):
         L.tlhip3d_cg_solver_calc_ur(X3, X3, X3, hd, 1e-3, C.addressof(pw), g["u"].ptr, g["p"].ptr,
                                     g["r"].ptr, g["w"].ptr)
         L.tlhip3d_cg_solver_calc_p(X3, X3, X3, hd, 0.5, g["p"].ptr, g["r"].ptr)
+        L.tlhip3d_cg_calc_pw_cond(X3, X3, X3, hd, 1, 0.5, 0.01, 0.01, 0.01, C.addressof(pw),
+                                  g["p"].ptr, g["sd2"].ptr, g["r"].ptr, None, g["cond"].ptr)
+        L.tlhip3d_cg_calc_ur_cond(X3, X3, X3, hd, 1e-3, 0.01, 0.01, 0.01, C.addressof(pw),
+                                  g["u"].ptr, g["sd2"].ptr, g["r"].ptr, g["cond"].ptr)
         L.tlhip3d_ppcg_inner_iteration(X3, X3, X3, hd, 0.7, 0.2, g["u"].ptr, g["r"].ptr,
                                        g["kx"].ptr, g["ky"].ptr, g["kz"].ptr, g["sd"].ptr,
                                        g["sd2"].ptr)
@@ -636,6 +651,11 @@ _TRAFFIC_3D = [
     ("tlhip3d_cg_solver_calc_ur", ["k_interior<2, tl::OpCalcUr>"], 48,
      "reads u, p, r, w; writes u, r"),
     ("tlhip3d_cg_solver_calc_p", ["k_interior<2, tl::OpCalcP>"], 24, "reads p, r; writes p"),
+    ("tlhip3d_cg_calc_pw_cond.no_w", ["k_cg3_cond<false, true>"], 32,
+     "calc_p fused into calc_w from the conductivity, w kept in registers: reads p, r, cond; "
+     "writes p'"),
+    ("tlhip3d_cg_calc_ur_cond", ["k_cg3_cond<true, false>"], 48,
+     "cg_calc_ur forming w = A p from p and cond: reads u, p, r, cond; writes u, r"),
     ("tlhip3d_ppcg_inner_iteration", ["k_stencil3d<2, tl::OpPpcgInner>"], 72,
      "reads sd, kx, ky, kz, r, u; writes sd', r, u"),
     ("tlhip3d_ppcg_inner_iterations", ["k_copy_ring3d", "k_copy_ring3d", "k_copy_ring3d",
