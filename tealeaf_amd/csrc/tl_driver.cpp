@@ -1214,7 +1214,20 @@ struct tlhip_problem {
         TRY(calc_w32_pending(&pw));
         TRY(tl::mix::calc_ur32(x, y, z, hd, P, *rro / pw, &unused, d32, p32, r32, w32));
         if (s.ppcg_inner_steps > 0) TRY(tl::mix::ppcg_init3_32(x, y, z, hd, P, theta, r32, sd32));
-        for (int k = 0; k < s.ppcg_inner_steps; ++k) {
+        int k = 0;
+        if (g_ppcg_fusion > 1 && x % 2 == 0) {
+            // two steps per pass, bit-identical to two single steps
+            for (; k + 2 <= s.ppcg_inner_steps; k += 2) {
+                TRY(tl::mix::ppcg_pair3_32(x, y, z, hd, P, &cheby_alphas[k], &cheby_betas[k],
+                                           sd32, sd32b, r32, r32b, d32, d32b, kx32, ky32,
+                                           kz32));
+                std::swap(sd32, sd32b);
+                std::swap(r32, r32b);
+                std::swap(d32, d32b);
+                rep->n_ppcg_inner += 2;
+            }
+        }
+        for (; k < s.ppcg_inner_steps; ++k) {
             TRY(tl::mix::ppcg_inner3_32(x, y, z, hd, P, cheby_alphas[k], cheby_betas[k], sd32,
                                         sd32b, r32, d32, kx32, ky32, kz32));
             std::swap(sd32, sd32b);

@@ -14,6 +14,7 @@
 #include "tl_lanes.h"
 #include "tl_ops.h"
 #include "tl_loop.h"
+#include "tl_mixed.h"
 #include "tealeaf_hip.h"
 
 #include <stdint.h>
@@ -1159,22 +1160,99 @@ k_ppcg3d_pair(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, double b1
     }
 }
 
+// Two cells per lane in either precision: Cells<double> for FP64, F2 for
+// the FP32 fields of the FP32-refined solve (rows of pitch >= x).
+struct F2 {
+    float v[2];
+};
+template <class T>
+struct Pair2;
+template <>
+struct Pair2<double> {
+    using type = C2;
+};
+template <>
+struct Pair2<float> {
+    using type = F2;
+};
+template <class T>
+using P2 = typename Pair2<T>::type;
+
+template <class T>
+__device__ __forceinline__ P2<T> zeros2()
+{
+    P2<T> z;
+    z.v[0] = z.v[1] = T(0);
+    return z;
+}
+__device__ __forceinline__ C2 ld2(const double* __restrict__ f, long i) { return ldc(f, i); }
+__device__ __forceinline__ F2 ld2(const float* __restrict__ f, long i)
+{
+    const float2 a = *reinterpret_cast<const float2*>(f + i);
+    return F2{{a.x, a.y}};
+}
+__device__ __forceinline__ void st2(double* __restrict__ f, long i, const C2& v) { stc(f, i, v); }
+__device__ __forceinline__ void st2(float* __restrict__ f, long i, const F2& v)
+{
+    float2 a;
+    a.x = v.v[0];
+    a.y = v.v[1];
+    *reinterpret_cast<float2*>(f + i) = a;
+}
+
+// ld3 with a row pitch of its own (FP32 rows are padded)
+template <class T>
+__device__ __forceinline__ P2<T> ld3p(const T* __restrict__ f, const TlGrid3& g, int pitch,
+                                      int j, int k, int c)
+{
+    if (c < 0 || c + 2 > g.x || j < 0 || j >= g.y || k < 0 || k >= g.z) return zeros2<T>();
+    return ld2(f, ((long)k * g.y + j) * pitch + c);
+}
+
+// apply7 in either precision (the same association; FP32 as the FP32
+// 7-point group kernel forms it)
+template <class V>
+__device__ __forceinline__ V apply7t(const V& cen, const V& no, const V& so, const V& up,
+                                     const V& dn, const V& kxc, const V& kyc, const V& kyn,
+                                     const V& kzc, const V& kzu, int c, int xin0, int xin1)
+{
+    using T = decltype(cen.v[0] + cen.v[0]);
+    const T lft_e = tl_lane_prev(cen.v[1], cen.v[0]);
+    const T rgt_e = tl_lane_next(cen.v[0], cen.v[1]);
+    const T kxe = tl_lane_next(kxc.v[0], kxc.v[1]);
+    V a;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        T l = i == 0 ? lft_e : cen.v[0];
+        T r = i == 1 ? rgt_e : cen.v[1];
+        if (c + i == xin0) l = cen.v[i];
+        if (c + i == xin1 - 1) r = cen.v[i];
+        const T kxr = i == 1 ? kxe : kxc.v[1];
+        const T diag = ((T(1) + (kxr + kxc.v[i])) + (kyn.v[i] + kyc.v[i])) + (kzu.v[i] + kzc.v[i]);
+        a.v[i] = diag * cen.v[i] - (kxr * r + kxc.v[i] * l) - (kyn.v[i] * no.v[i] + kyc.v[i] * so.v[i]) -
+                 (kzu.v[i] * up.v[i] + kzc.v[i] * dn.v[i]);
+    }
+    return a;
+}
+
 // The same two steps with the block's rows sharing level 0 and level 1
 // through LDS: a block is ROWS + 2 waves, one per row j0-1 .. j0+ROWS; each
 // wave forms level 1 of its own row only, and the waves of rows j0 ..
 // j0+ROWS-1 write level 2.  Per plane: every wave publishes its row of sd,
 // then (barrier) forms level 1 from its neighbours' rows and publishes it,
 // then (barrier) forms level 2 of the plane below from the level-1 rows of
-// its neighbours (double-buffered by plane parity).
-template <int ROWS>
+// its neighbours (double-buffered by plane parity).  T = float runs the
+// FP32 fields of the FP32-refined solve (rows of `pitch` floats; the u role
+// is the correction d), with the FP32 7-point group kernel's expressions.
+template <class T, int ROWS>
 __global__ void __launch_bounds__((ROWS + 2) * TL_WAVE)
-k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, double b1,
-                  double a2, double b2, const double* __restrict__ sd_in,
-                  double* __restrict__ sd_out, const double* __restrict__ r_in,
-                  double* __restrict__ r_out, const double* __restrict__ u_in,
-                  double* __restrict__ u_out, const double* __restrict__ kx,
-                  const double* __restrict__ ky, const double* __restrict__ kz, TlGate gate)
+k_ppcg3d_pair_lds(TlGrid3 g, int pitch, int nwx, int nby, int nbz, int tz, T a1, T b1, T a2,
+                  T b2, const T* __restrict__ sd_in, T* __restrict__ sd_out,
+                  const T* __restrict__ r_in, T* __restrict__ r_out, const T* __restrict__ u_in,
+                  T* __restrict__ u_out, const T* __restrict__ kx, const T* __restrict__ ky,
+                  const T* __restrict__ kz, TlGate gate)
 {
+    using C2 = P2<T>;
     constexpr int W = ROWS + 2;
     __shared__ C2 s0_row[W][TL_WAVE];
     __shared__ C2 s1_row[2][W][TL_WAVE];
@@ -1193,18 +1271,18 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
     const int c = cw + 2 * lane;
     const bool st0 = c >= o0 && c < o1, st1 = c + 1 >= o0 && c + 1 < o1;
     const int k0 = zin0 + bz * tz, k1 = min(k0 + tz, zin1);
-    const long X = g.x, XY = (long)g.x * g.y;
+    const long X = pitch, XY = (long)pitch * g.y;
 
     // own row: sd at planes kk-1, kk, kk+1; level 1 at planes km-1, km, km+1
     C2 s0w[3], s1w[3];
-    s0w[0] = zeros<double>();
-    s0w[1] = ld3(sd_in, g, jw, k0 - 2, c);
-    s0w[2] = ld3(sd_in, g, jw, k0 - 1, c);
-    s1w[0] = s1w[1] = s1w[2] = zeros<double>();
-    C2 r1p = zeros<double>(), u1p = zeros<double>();
-    C2 kxp = zeros<double>(), kyp = zeros<double>(), kynp = zeros<double>();
-    C2 kzp = zeros<double>(), kzup = zeros<double>();
-    C2 kz_c = ld3(kz, g, jw, k0 - 1, c);         // kz of the level-1 plane
+    s0w[0] = zeros2<T>();
+    s0w[1] = ld3p(sd_in, g, pitch, jw, k0 - 2, c);
+    s0w[2] = ld3p(sd_in, g, pitch, jw, k0 - 1, c);
+    s1w[0] = s1w[1] = s1w[2] = zeros2<T>();
+    C2 r1p = zeros2<T>(), u1p = zeros2<T>();
+    C2 kxp = zeros2<T>(), kyp = zeros2<T>(), kynp = zeros2<T>();
+    C2 kzp = zeros2<T>(), kzup = zeros2<T>();
+    C2 kz_c = ld3p(kz, g, pitch, jw, k0 - 1, c);         // kz of the level-1 plane
 
     // What level 1 of plane kk reads from memory, loaded one plane ahead:
     // sd at plane kk+1, kx, ky (rows j, j+1), r and u at plane kk, kz at
@@ -1214,15 +1292,15 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
     };
     auto load_plane = [&](int kk) -> PlaneIn {
         PlaneIn v;
-        v.sdn = ld3(sd_in, g, jw, kk + 1, c);
-        v.kxc = ld3(kx, g, jw, kk, c);
-        v.kyc = ld3(ky, g, jw, kk, c);
-        v.kyn = ld3(ky, g, jw + 1, kk, c);
-        v.kzu = ld3(kz, g, jw, kk + 1, c);
-        v.r0 = ld3(r_in, g, jw, kk, c);
-        v.u0 = out_row ? ld3(u_in, g, jw, kk, c) : zeros<double>();
-        v.outer = wave == 0 ? ld3(sd_in, g, jw - 1, kk, c)
-                            : (wave == W - 1 ? ld3(sd_in, g, jw + 1, kk, c) : zeros<double>());
+        v.sdn = ld3p(sd_in, g, pitch, jw, kk + 1, c);
+        v.kxc = ld3p(kx, g, pitch, jw, kk, c);
+        v.kyc = ld3p(ky, g, pitch, jw, kk, c);
+        v.kyn = ld3p(ky, g, pitch, jw + 1, kk, c);
+        v.kzu = ld3p(kz, g, pitch, jw, kk + 1, c);
+        v.r0 = ld3p(r_in, g, pitch, jw, kk, c);
+        v.u0 = out_row ? ld3p(u_in, g, pitch, jw, kk, c) : zeros2<T>();
+        v.outer = wave == 0 ? ld3p(sd_in, g, pitch, jw - 1, kk, c)
+                            : (wave == W - 1 ? ld3p(sd_in, g, pitch, jw + 1, kk, c) : zeros2<T>());
         return v;
     };
     PlaneIn nxt = load_plane(k0 - 1);
@@ -1239,14 +1317,14 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
         __syncthreads();
         s1w[0] = s1w[1];
         s1w[1] = s1w[2];
-        C2 r1 = zeros<double>(), u1 = zeros<double>();
+        C2 r1 = zeros2<T>(), u1 = zeros2<T>();
         if (l1_row && kk >= zin0 && kk < zin1) {
             const C2 cen = s0w[1];
             const C2 so = jw == yin0 ? cen : (wave == 0 ? outer : s0_row[wave - 1][lane]);
             const C2 no = jw == yin1 - 1 ? cen : (wave == W - 1 ? outer : s0_row[wave + 1][lane]);
             const C2 up = kk == zin1 - 1 ? cen : s0w[2];
             const C2 dn = kk == zin0 ? cen : s0w[0];
-            const C2 w = apply7(cen, no, so, up, dn, kxc, kyc, kyn, kz_c, kzu, c, xin0, xin1);
+            const C2 w = apply7t(cen, no, so, up, dn, kxc, kyc, kyn, kz_c, kzu, c, xin0, xin1);
             C2 sn;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
@@ -1266,7 +1344,7 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
             const C2 no = jw == yin1 - 1 ? cen : s1_row[km & 1][wave + 1][lane];
             const C2 up = km == zin1 - 1 ? cen : s1w[2];
             const C2 dn = km == zin0 ? cen : s1w[0];
-            const C2 w = apply7(cen, no, so, up, dn, kxp, kyp, kynp, kzp, kzup, c, xin0, xin1);
+            const C2 w = apply7t(cen, no, so, up, dn, kxp, kyp, kynp, kzp, kzup, c, xin0, xin1);
             C2 rn, un, sn;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
@@ -1276,9 +1354,9 @@ k_ppcg3d_pair_lds(TlGrid3 g, int nwx, int nby, int nbz, int tz, double a1, doubl
             }
             const long b = (long)km * XY + (long)jw * X + c;
             if (st0 && st1) {
-                stc(r_out, b, rn);
-                stc(u_out, b, un);
-                stc(sd_out, b, sn);
+                st2(r_out, b, rn);
+                st2(u_out, b, un);
+                st2(sd_out, b, sn);
             } else {
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
@@ -1607,19 +1685,19 @@ extern "C" int tlhip3d_ppcg_inner_iterations(int x, int y, int z, int hd, int co
     TL_TRY(copy_ring(g, r_in, r_out, what));
     TL_TRY(copy_ring(g, u_in, u_out, what));
     const int slot = tl_timer_begin(what);
-#define TL_PAIR_ARGS g, nwx, nby, nbz, tz, alphas[0], betas[0], alphas[1], betas[1], sd_in, \
+#define TL_PAIR_ARGS nwx, nby, nbz, tz, alphas[0], betas[0], alphas[1], betas[1], sd_in, \
         sd_out, r_in, r_out, u_in, u_out, kx, ky, kz, tl_gate()
     switch (rows) {
-    case 4: hipLaunchKernelGGL((k_ppcg3d_pair_lds<4>), dim3((unsigned)nb), dim3(6 * TL_WAVE), 0,
-                               tl_stream(), TL_PAIR_ARGS); break;
-    case 6: hipLaunchKernelGGL((k_ppcg3d_pair_lds<6>), dim3((unsigned)nb), dim3(8 * TL_WAVE), 0,
-                               tl_stream(), TL_PAIR_ARGS); break;
-    case 8: hipLaunchKernelGGL((k_ppcg3d_pair_lds<8>), dim3((unsigned)nb), dim3(10 * TL_WAVE), 0,
-                               tl_stream(), TL_PAIR_ARGS); break;
-    case 14: hipLaunchKernelGGL((k_ppcg3d_pair_lds<14>), dim3((unsigned)nb), dim3(16 * TL_WAVE), 0,
-                                tl_stream(), TL_PAIR_ARGS); break;
+    case 4: hipLaunchKernelGGL((k_ppcg3d_pair_lds<double, 4>), dim3((unsigned)nb),
+                               dim3(6 * TL_WAVE), 0, tl_stream(), g, x, TL_PAIR_ARGS); break;
+    case 6: hipLaunchKernelGGL((k_ppcg3d_pair_lds<double, 6>), dim3((unsigned)nb),
+                               dim3(8 * TL_WAVE), 0, tl_stream(), g, x, TL_PAIR_ARGS); break;
+    case 8: hipLaunchKernelGGL((k_ppcg3d_pair_lds<double, 8>), dim3((unsigned)nb),
+                               dim3(10 * TL_WAVE), 0, tl_stream(), g, x, TL_PAIR_ARGS); break;
+    case 14: hipLaunchKernelGGL((k_ppcg3d_pair_lds<double, 14>), dim3((unsigned)nb),
+                                dim3(16 * TL_WAVE), 0, tl_stream(), g, x, TL_PAIR_ARGS); break;
     default: hipLaunchKernelGGL(k_ppcg3d_pair, dim3((unsigned)nb), dim3(kBlock), 0, tl_stream(),
-                                TL_PAIR_ARGS); break;
+                                g, TL_PAIR_ARGS); break;
     }
 #undef TL_PAIR_ARGS
     tl_timer_end(slot);
@@ -1848,3 +1926,46 @@ extern "C" int tlhip3d_cg_calc_ur_cond(int x, int y, int z, int hd, double alpha
     TL_TRY(tl_finish_reduce(1, nb, what));
     return fetch_add(rrn, what);
 }
+
+// ------------------------------------------------ FP32 two-step PPCG (3D)
+
+namespace tl {
+namespace mix {
+
+// Two FP32 PPCG inner steps in one pass (tl_mixed.h): the 8-row LDS
+// kernel on FP32 fields of row pitch P; r, d and sd from *_in to *_out
+// (interiors only: no FP32 kernel reads a halo), bit-identical to two
+// ppcg_inner3_32 turns.
+int ppcg_pair3_32(int x, int y, int z, int hd, int P, const double* alphas,
+                  const double* betas, const float* sd_in, float* sd_out, const float* r_in,
+                  float* r_out, const float* d_in, float* d_out, const float* kx,
+                  const float* ky, const float* kz)
+{
+    const char* what = "tlhip3d_mix_ppcg_pair32";
+    const TlGrid3 g{x, y, z, hd};
+    TL_TRY(check_grid(g, what));
+    if (x % 2 || P < x || P % 4)
+        return tl_fail(TLHIP_ERR_ARG, "%s: needs an even x and a pitch >= x, multiple of 4",
+                       what);
+    if (interior_empty(g)) return tl_fail(TLHIP_ERR_ARG, "%s: empty interior", what);
+    REQUIRE(sd_in && sd_out && r_in && r_out && d_in && d_out && kx && ky && kz, what,
+            "null argument");
+    REQUIRE(sd_in != sd_out && r_in != r_out && d_in != d_out, what,
+            "input and output buffers must be distinct");
+    const int nx = x - 2 * hd, ny = y - 2 * hd, nz = z - 2 * hd;
+    const int tz = g_tz_override > 0 ? g_tz_override : 32;
+    const int nwx = (int)tl_cdiv(nx, kP2Out), nby = (int)tl_cdiv(ny, 8);
+    const int nbz = (int)tl_cdiv(nz, tz);
+    const long nb = (long)nwx * nby * nbz;
+    if (nb > (1L << 30)) return tl_fail(TLHIP_ERR_ARG, "%s: grid too large", what);
+    const int slot = tl_timer_begin(what);
+    hipLaunchKernelGGL((k_ppcg3d_pair_lds<float, 8>), dim3((unsigned)nb), dim3(10 * TL_WAVE), 0,
+                       tl_stream(), g, P, nwx, nby, nbz, tz, (float)alphas[0], (float)betas[0],
+                       (float)alphas[1], (float)betas[1], sd_in, sd_out, r_in, r_out, d_in, d_out,
+                       kx, ky, kz, tl_gate());
+    tl_timer_end(slot);
+    return tl_check_launch(what);
+}
+
+}  // namespace mix
+}  // namespace tl

@@ -66,6 +66,12 @@ int calc_w3_32(int x, int y, int z, int hd, int P, double* pw, const float* p, f
 // sd = r / theta
 int ppcg_init3_32(int x, int y, int z, int hd, int P, double theta, const float* r, float* sd);
 // one PPCG inner step: w = A sd_in, r -= w, d += sd_in, sd_out = alpha sd_in + beta r
+// Two FP32 PPCG inner steps per pass (tl_kernels3d.hip): bit-identical to
+// two ppcg_inner3_32 turns; r, d, sd from *_in to *_out; even x.
+int ppcg_pair3_32(int x, int y, int z, int hd, int P, const double* alphas,
+                  const double* betas, const float* sd_in, float* sd_out, const float* r_in,
+                  float* r_out, const float* d_in, float* d_out, const float* kx,
+                  const float* ky, const float* kz);
 int ppcg_inner3_32(int x, int y, int z, int hd, int P, double alpha, double beta,
                    const float* sd_in, float* sd_out, float* r, float* d, const float* kx,
                    const float* ky, const float* kz);

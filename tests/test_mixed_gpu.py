@@ -212,3 +212,27 @@ def test_mixed_precision_3d_128(hip, name, text, blk):
     err = np.linalg.norm((f_m["u"] - f_d["u"])[inner]) / np.linalg.norm(f_d["u"][inner])
     assert err < 1e-6, err
     assert _work(reps_m) <= 1.25 * _work(reps_d) + 10, (_work(reps_m), _work(reps_d))
+
+
+@pytest.mark.parametrize("n,inner", [(20, 8), (26, 7), (24, 10)])
+def test_mixed_3d_pairs_bit_identical(hip, n, inner):
+    """The FP32-refined 3D PPCG with two FP32 inner steps per pass (fusion >
+    1, even x) gives the same bits as one pass per step (fusion 1): u,
+    summary, iteration counts and true-residual evaluations; an odd inner
+    count ends on a single step."""
+    text = tdeck.cuboid_3d(n, 2, solver=tdeck.PPCG, inner=inner, coefficient=2,
+                           max_iters=4000)
+    s, states = tdeck.parse_deck(text, 3)
+    out = {}
+    try:
+        for fusion in (1, 10):
+            hip.tlhip_set_ppcg_fusion(fusion)
+            out[fusion] = _gpu_solve(s, states, None, 1, TOL, steps=2)
+    finally:
+        hip.tlhip_set_ppcg_fusion(10)
+    (r1, f1, s1), (r10, f10, s10) = out[1], out[10]
+    for key in ("n_cg_steps", "n_ppcg_outer", "n_ppcg_inner", "n_true_residuals"):
+        assert [r[key] for r in r1] == [r[key] for r in r10], key
+    assert sum(r["n_ppcg_outer"] for r in r10) > 0
+    np.testing.assert_array_equal(f10["u"], f1["u"])
+    assert s10 == s1
