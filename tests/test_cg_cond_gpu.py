@@ -7,7 +7,11 @@ coefficient with cg_init's expression (cg.c:55-64).  Against the oracle's
 cg_init (kx, ky), calc_p, a depth-1 reflective halo update of p, calc_w and
 calc_ur (cg.c:88-164) on the same inputs: p, w, u and r bit for bit, the
 dot products within 1e-13 (summation order).  Grids cover several strips and
-row blocks, partial ones, halo depths 2 and 3, and both coefficients.
+row blocks, partial ones, halo depths 2 and 3, and both coefficients.  The
+3D pair (tlhip3d_cg_*_cond, kz too; 3d/c_kernels/omp3/cg_solver.c) is held
+to the same bar against the oracle's 3D kernels, on grids with partial row
+blocks and short z-tiles; the driver with either pair agrees with its
+kx/ky(/kz) passes to rounding.
 """
 import numpy as np
 import pytest
